@@ -1,0 +1,166 @@
+/*
+ * slam_odom.h -- C ABI of the MI355X (gfx950) dense RGB-D odometry library
+ * (libslam_odom.so).
+ *
+ * Drop-in for the reference's src/odom path (siw-engineering/slam, an
+ * ElasticFusion fork): one handle is one RGBDOdometryef object
+ * (src/odom/RGBDOdometryef.h:28-131).  The GL textures of the reference
+ * (GPUTexture*, mapped through CUDA-GL interop) are replaced by frame views:
+ * a pointer, a row pitch, a pixel format and whether the pointer is host or
+ * device memory.  Every entry point returns an int status (ODOM_OK = 0).
+ *
+ * Threading: a handle owns one HIP stream and is not re-entrant; distinct
+ * handles may be used from distinct host threads.  Inputs are borrowed only
+ * for the duration of the call.
+ */
+#ifndef SLAM_ODOM_H
+#define SLAM_ODOM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ODOM_OK 0
+#define ODOM_ERR_ARG (-1)      /* bad argument (null, size or format mismatch) */
+#define ODOM_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define ODOM_ERR_STATE (-3)    /* call order violated */
+#define ODOM_ERR_NODEV (-4)    /* no gfx950 device */
+
+/* pixel formats of odom_frame_view (the reference's texture formats,
+ * elastic_fusion_file.cpp:301-303, FillIn.cpp:29-42) */
+#define ODOM_FMT_DEPTH_U16 1   /* uint16 depth, 1000 units per metre */
+#define ODOM_FMT_RGB8 2        /* 3 bytes per pixel */
+#define ODOM_FMT_RGBA8 3       /* 4 bytes per pixel (alpha ignored) */
+#define ODOM_FMT_RGBA32F 4     /* float4 per pixel: vertex (x,y,z,w) or normal */
+
+#define ODOM_MEM_HOST 0
+#define ODOM_MEM_DEVICE 1
+
+typedef struct odom_frame_view {
+    const void *ptr;
+    int pitch_bytes;           /* row pitch; 0 = tightly packed */
+    int width, height;
+    int format;                /* ODOM_FMT_* */
+    int mem;                   /* ODOM_MEM_* */
+} odom_frame_view;
+
+typedef struct odom_params {
+    int width, height;
+    float cx, cy, fx, fy;
+    float dist_thresh;         /* default 0.10 m (RGBDOdometryef.h:35) */
+    float angle_thresh;        /* default sin(20 deg) (RGBDOdometryef.h:36) */
+} odom_params;
+
+/* Reported per call (RGBDOdometryef.h:62-70 plus flags). */
+typedef struct odom_stats {
+    float last_icp_error, last_icp_count;
+    float last_rgb_error, last_rgb_count;
+    float last_so3_error, last_so3_count;
+    int so3_iterations;
+    int gn_iterations;         /* Gauss-Newton iterations run */
+    int degenerate;            /* last solve had no information (identity update) */
+    int reverted;              /* the 0.3 m revert was taken */
+} odom_stats;
+
+typedef struct odom_ctx *odom_handle;
+
+/* Defaults of the RGBDOdometryef constructor (RGBDOdometryef.h:32-36). */
+int odom_default_params(odom_params *p, int width, int height, float cx, float cy, float fx, float fy);
+
+/* RGBDOdometryef::RGBDOdometryef, RGBDOdometryef.cpp:21-111.  Allocates every
+ * device buffer once. */
+int odom_create(const odom_params *p, odom_handle *out);
+/* RGBDOdometryef::~RGBDOdometryef, RGBDOdometryef.cpp:113-116 */
+int odom_destroy(odom_handle h);
+
+/* The depth pre-filter of the reference's frame loop: depth_bilateral.frag:30-76
+ * as run by ComputePack::compute (elastic_fusion_file.cpp:342-346).  dst is a
+ * width*height uint16 buffer in memory space dst_mem. */
+int odom_bilateral_depth(odom_handle h, const odom_frame_view *raw, float max_depth_m, void *dst, int dst_mem);
+
+/* RGBDOdometryef::initICP(GPUTexture *filteredDepth, float depthCutoff),
+ * RGBDOdometryef.cpp:118-142. */
+int odom_init_icp_depth(odom_handle h, const odom_frame_view *filtered_depth, float depth_cutoff);
+/* Bilateral pre-filter fused with initICP(filteredDepth): the pair of calls
+ * ComputePack FILTER + initICP of elastic_fusion_file.cpp:346, 370. */
+int odom_init_icp_raw_depth(odom_handle h, const odom_frame_view *raw_depth, float max_depth_m,
+                            float depth_cutoff);
+/* RGBDOdometryef::initICP(GPUTexture *v, GPUTexture *n, float), RGBDOdometryef.cpp:144-167 */
+int odom_init_icp_maps(odom_handle h, const odom_frame_view *vertices, const odom_frame_view *normals,
+                       float depth_cutoff);
+/* RGBDOdometryef::initICPModel, RGBDOdometryef.cpp:169-206.  pose: row-major
+ * 4x4 camera-to-world of the model prediction. */
+int odom_init_icp_model(odom_handle h, const odom_frame_view *vertices, const odom_frame_view *normals,
+                        float depth_cutoff, const float pose[16]);
+/* RGBDOdometryef::initRGB, RGBDOdometryef.cpp:243-247 */
+int odom_init_rgb(odom_handle h, const odom_frame_view *rgb);
+/* RGBDOdometryef::initRGBModel, RGBDOdometryef.cpp:237-241 */
+int odom_init_rgb_model(odom_handle h, const odom_frame_view *rgb);
+/* RGBDOdometryef::initFirstRGB, RGBDOdometryef.cpp:249-265 */
+int odom_init_first_rgb(odom_handle h, const odom_frame_view *rgb);
+
+/* RGBDOdometryef::getIncrementalTransformation, RGBDOdometryef.cpp:267-595.
+ * trans[3] and rot[9] (row-major) are read as the prior and overwritten with
+ * the result.  Synchronous. */
+int odom_track(odom_handle h, float trans[3], float rot[9], int rgb_only, float icp_weight, int pyramid,
+               int fast_odom, int so3);
+/* Asynchronous form: the prior is the handle's device-resident pose (the
+ * result of the previous call, or odom_set_pose); nothing is read back. */
+int odom_track_async(odom_handle h, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3);
+int odom_set_pose(odom_handle h, const float trans[3], const float rot[9]);
+int odom_get_pose(odom_handle h, float trans[3], float rot[9]);
+int odom_synchronize(odom_handle h);
+
+/* RGBDOdometryef::getCovariance, RGBDOdometryef.cpp:597-600: lastA^-1 */
+int odom_get_covariance(odom_handle h, double cov[36]);
+/* public lastA / lastb (RGBDOdometryef.h:69-70); A row-major */
+int odom_get_last_system(odom_handle h, double A[36], double b[6]);
+int odom_get_stats(odom_handle h, odom_stats *s);
+
+/* ---- kernel-level entry points: the launchers of src/odom/utils.cuh:62-175,
+ * run on the handle's current pyramids with explicit constants.  Used for
+ * parity checks and autotuning. ---- */
+
+/* icpStep (reduce.cu:418-490) at `level`: out[29] = 27 JtJ/Jtr products,
+ * sum r^2, inliers (JtJJtrSE3 order, types.cuh:85-96); mask (host, optional,
+ * width_l*height_l bytes) gets 1 per inlier. */
+int odom_icp_step(odom_handle h, int level, const float Rcurr[9], const float tcurr[3],
+                  const float Rprev_inv[9], const float tprev[3], double out[29], uint8_t *mask);
+/* computeDerivativeImages (utils.cu:608-638) on all levels of the next image. */
+int odom_compute_derivatives(odom_handle h);
+/* computeRgbResidual (reduce.cu:869-935): count and int32 sum of diff^2;
+ * mask gets 1 per valid correspondence. */
+int odom_rgb_residual(odom_handle h, int level, const float krkinv[9], const float kt[3], int32_t *count,
+                      int32_t *sigma_sum, uint8_t *mask);
+/* rgbStep (reduce.cu:626-678) with the correspondences of krkinv/kt */
+int odom_rgb_step(odom_handle h, int level, float sigma, const float krkinv[9], const float kt[3],
+                  double out[29]);
+/* so3Step (reduce.cu:1082-1140) on level 2 of lastNextImage/nextImage */
+int odom_so3_step(odom_handle h, const float H[9], const float kinv[9], const float krlr[9], double out[11],
+                  uint8_t *mask);
+/* Copy an internal buffer to host memory.  Names: depth (uint16), vmap_curr,
+ * nmap_curr, vmap_g_prev, nmap_g_prev (3 planes float), last_depth,
+ * next_depth (float), last_image, next_image, last_next_image (uint8), dxy
+ * (int16 pairs). */
+int odom_read_buffer(odom_handle h, const char *name, int level, void *dst, size_t bytes);
+
+/* Launch shapes of the two Gauss-Newton step kernels (the role of the
+ * reference's GPUConfig.h table, GPUConfig.h:7-149).  kernel 0 = ICP /
+ * association pass, 1 = photometric pass. */
+int odom_set_step_shape(odom_handle h, int kernel, int level, int threads, int pixels_per_thread);
+int odom_get_step_shape(odom_handle h, int kernel, int level, int *threads, int *pixels_per_thread);
+/* Times the supported shapes on the handle's current inputs and keeps the
+ * fastest per kernel and level. */
+int odom_autotune(odom_handle h, int rgb_only, float icp_weight);
+
+/* Device name / arch of the current device, for logs. */
+int odom_device_info(char *name, int name_len, char *arch, int arch_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

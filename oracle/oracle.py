@@ -1,0 +1,363 @@
+"""ctypes bindings of the CPU oracle (``liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  This is the checker of the HIP product path; only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import it.  See ``odom_oracle.h`` for what each function restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_U16 = ctypes.POINTER(ctypes.c_uint16)
+_S16 = ctypes.POINTER(ctypes.c_int16)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_V = ctypes.c_void_p
+
+
+class Intr(ctypes.Structure):
+    _fields_ = [("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float)]
+
+
+DATATERM = np.dtype([("zero_x", "<i2"), ("zero_y", "<i2"), ("one_x", "<i2"), ("one_y", "<i2"),
+                     ("diff", "<f4"), ("valid", "<i4")])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_level_intr.restype = Intr
+        L.orc_level_intr.argtypes = [Intr, ctypes.c_int]
+        L.orc_bexpf.restype = ctypes.c_float
+        L.orc_bexpf.argtypes = [ctypes.c_float]
+        L.orc_create.restype = _V
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6
+        L.orc_destroy.argtypes = [_V]
+        L.orc_set_precision.argtypes = [_V, ctypes.c_int]
+        L.orc_init_icp_depth.argtypes = [_V, _V, ctypes.c_float]
+        L.orc_init_icp_maps.argtypes = [_V, _V, _V, ctypes.c_float]
+        L.orc_init_icp_model.argtypes = [_V, _V, _V, ctypes.c_float, _V]
+        L.orc_init_rgb.argtypes = [_V, _V, ctypes.c_int]
+        L.orc_init_rgb_model.argtypes = [_V, _V, ctypes.c_int]
+        L.orc_init_first_rgb.argtypes = [_V, _V, ctypes.c_int]
+        L.orc_track.argtypes = [_V, _V, _V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_get_stats.argtypes = [_V, _V]
+        L.orc_get_last_system.argtypes = [_V, _V, _V]
+        L.orc_get_covariance.argtypes = [_V, _V]
+        L.orc_get_covariance.restype = ctypes.c_int
+        for n, rt in (("orc_buf_f32", _F), ("orc_buf_u8", _U8), ("orc_buf_s16", _S16), ("orc_buf_u16", _U16)):
+            getattr(L, n).restype = rt
+            getattr(L, n).argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
+        L.orc_ldlt_solve.restype = ctypes.c_int
+        L.orc_ldlt_solve.argtypes = [ctypes.c_int, _V, _V, _V]
+        L.orc_rodrigues.argtypes = [_V, _V]
+        L.orc_icp_step.argtypes = [_V] * 4 + [ctypes.c_int, ctypes.c_int, Intr] + [_V] * 4 + \
+            [ctypes.c_float, ctypes.c_float, _V, _V]
+        L.orc_icp_step_f64.argtypes = [_V] * 4 + [ctypes.c_int, ctypes.c_int, Intr] + [_V] * 4 + \
+            [ctypes.c_double, ctypes.c_double, _V, _V]
+        L.orc_rgb_residual.argtypes = [ctypes.c_float] + [_V] * 6 + [ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                                                     _V, _V, _V, _V, _V]
+        L.orc_rgb_step.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_float, _V, Intr, _V, _V,
+                                   ctypes.c_float, _V]
+        L.orc_so3_step.argtypes = [_V, _V, ctypes.c_int, ctypes.c_int, _V, _V, _V, _V, _V]
+        L.orc_unpack_se3.argtypes = [_V, _V, _V]
+        L.orc_bilateral_u16.argtypes = [_V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        L.orc_pyrdown_u16.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V]
+        L.orc_vmap.argtypes = [_V, ctypes.c_int, ctypes.c_int, Intr, ctypes.c_float, _V]
+        L.orc_nmap.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V]
+        L.orc_copy_maps.argtypes = [_V, _V, ctypes.c_int, ctypes.c_int, _V, _V]
+        L.orc_resize_map.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V, ctypes.c_int]
+        L.orc_transform_maps.argtypes = [_V, _V, ctypes.c_int, ctypes.c_int, _V, _V]
+        L.orc_vertices_to_depth.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_float, _V]
+        L.orc_pyrdown_f32.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V]
+        L.orc_intensity.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _V]
+        L.orc_pyrdown_u8.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V]
+        L.orc_derivative.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V, _V]
+        L.orc_point_cloud.argtypes = [_V, ctypes.c_int, ctypes.c_int, Intr, _V]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def intr(fx, fy, cx, cy) -> Intr:
+    return Intr(fx, fy, cx, cy)
+
+
+def level_intr(k: Intr, level: int) -> Intr:
+    return lib().orc_level_intr(k, level)
+
+
+# ------------------------------------------------------------------ image ops
+
+def bilateral(depth, maxD):
+    d = _c(depth, np.uint16)
+    out = np.empty_like(d)
+    lib().orc_bilateral_u16(_p(d), _p(out), d.shape[1], d.shape[0], maxD)
+    return out
+
+
+def pyrdown_u16(src):
+    s = _c(src, np.uint16)
+    out = np.empty((s.shape[0] // 2, s.shape[1] // 2), np.uint16)
+    lib().orc_pyrdown_u16(_p(s), s.shape[1], s.shape[0], _p(out))
+    return out
+
+
+def vmap(depth, k: Intr, cutoff):
+    d = _c(depth, np.uint16)
+    out = np.empty((3,) + d.shape, np.float32)
+    lib().orc_vmap(_p(d), d.shape[1], d.shape[0], k, cutoff, _p(out))
+    return out
+
+
+def nmap(vm):
+    v = _c(vm, np.float32)
+    out = np.empty_like(v)
+    lib().orc_nmap(_p(v), v.shape[2], v.shape[1], _p(out))
+    return out
+
+
+def copy_maps(v4, n4):
+    v, n = _c(v4, np.float32), _c(n4, np.float32)
+    H, W = v.shape[:2]
+    vo = np.empty((3, H, W), np.float32)
+    no = np.empty((3, H, W), np.float32)
+    lib().orc_copy_maps(_p(v), _p(n), W, H, _p(vo), _p(no))
+    return vo, no
+
+
+def resize_map(m, normalize):
+    s = _c(m, np.float32)
+    out = np.empty((3, s.shape[1] // 2, s.shape[2] // 2), np.float32)
+    lib().orc_resize_map(_p(s), s.shape[2], s.shape[1], _p(out), int(normalize))
+    return out
+
+
+def transform_maps(vm, nm, R, t):
+    v, n = _c(vm, np.float32).copy(), _c(nm, np.float32).copy()
+    Rf, tf = _c(R, np.float32), _c(t, np.float32)
+    lib().orc_transform_maps(_p(v), _p(n), v.shape[2], v.shape[1], _p(Rf), _p(tf))
+    return v, n
+
+
+def vertices_to_depth(v4, cutoff=6.0):
+    v = _c(v4, np.float32)
+    out = np.empty(v.shape[:2], np.float32)
+    lib().orc_vertices_to_depth(_p(v), v.shape[1], v.shape[0], cutoff, _p(out))
+    return out
+
+
+def pyrdown_f32(src):
+    s = _c(src, np.float32)
+    out = np.empty((s.shape[0] // 2, s.shape[1] // 2), np.float32)
+    lib().orc_pyrdown_f32(_p(s), s.shape[1], s.shape[0], _p(out))
+    return out
+
+
+def intensity(rgb):
+    r = _c(rgb, np.uint8)
+    out = np.empty(r.shape[:2], np.uint8)
+    lib().orc_intensity(_p(r), r.shape[1], r.shape[0], r.shape[2], _p(out))
+    return out
+
+
+def pyrdown_u8(src):
+    s = _c(src, np.uint8)
+    out = np.empty((s.shape[0] // 2, s.shape[1] // 2), np.uint8)
+    lib().orc_pyrdown_u8(_p(s), s.shape[1], s.shape[0], _p(out))
+    return out
+
+
+def derivative(img):
+    s = _c(img, np.uint8)
+    dx = np.empty(s.shape, np.int16)
+    dy = np.empty(s.shape, np.int16)
+    lib().orc_derivative(_p(s), s.shape[1], s.shape[0], _p(dx), _p(dy))
+    return dx, dy
+
+
+def point_cloud(depth, k: Intr):
+    d = _c(depth, np.float32)
+    out = np.empty((3,) + d.shape, np.float32)
+    lib().orc_point_cloud(_p(d), d.shape[1], d.shape[0], k, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------- steps
+
+def icp_step(vc, nc, vp, np_, k: Intr, Rc, tc, Rpi, tp, dist=0.1, angle=None, fp64=False):
+    import math
+    if angle is None:
+        angle = math.sin(20.0 * 3.14159254 / 180.0)
+    arrs = [_c(a, np.float32) for a in (vc, nc, vp, np_)]
+    H, W = arrs[0].shape[1:]
+    out = np.zeros(29, np.float64)
+    mask = np.zeros((H, W), np.uint8)
+    dt = np.float64 if fp64 else np.float32
+    cs = [_c(a, dt) for a in (Rc, tc, Rpi, tp)]
+    fn = lib().orc_icp_step_f64 if fp64 else lib().orc_icp_step
+    fn(*[_p(a) for a in arrs], W, H, k, *[_p(a) for a in cs], dist, angle, _p(out), _p(mask))
+    return out, mask
+
+
+def rgb_residual(minScale, dIdx, dIdy, lastDepth, nextDepth, lastImage, nextImage, krkinv, kt,
+                 maxDepthDelta=0.07):
+    a = [_c(dIdx, np.int16), _c(dIdy, np.int16), _c(lastDepth, np.float32), _c(nextDepth, np.float32),
+         _c(lastImage, np.uint8), _c(nextImage, np.uint8)]
+    H, W = a[0].shape
+    corres = np.zeros((H, W), DATATERM)
+    cnt = np.zeros(1, np.int32)
+    sig = np.zeros(1, np.int32)
+    K, t = _c(krkinv, np.float32), _c(kt, np.float32)
+    lib().orc_rgb_residual(minScale, *[_p(x) for x in a], W, H, maxDepthDelta, _p(K), _p(t), _p(corres),
+                           _p(cnt), _p(sig))
+    return corres, int(cnt[0]), int(sig[0])
+
+
+def rgb_step(corres, sigma, lastDepth, k: Intr, dIdx, dIdy, sobelScale=0.125):
+    c = np.ascontiguousarray(corres)
+    H, W = c.shape
+    out = np.zeros(29, np.float64)
+    ld, dx, dy = _c(lastDepth, np.float32), _c(dIdx, np.int16), _c(dIdy, np.int16)
+    lib().orc_rgb_step(_p(c), W, H, sigma, _p(ld), k, _p(dx), _p(dy), sobelScale, _p(out))
+    return out
+
+
+def so3_step(last, nxt, H, kinv, krlr):
+    a, b = _c(last, np.uint8), _c(nxt, np.uint8)
+    h, w = a.shape
+    out = np.zeros(11, np.float64)
+    mask = np.zeros((h, w), np.uint8)
+    m = [_c(x, np.float32) for x in (H, kinv, krlr)]
+    lib().orc_so3_step(_p(a), _p(b), w, h, *[_p(x) for x in m], _p(out), _p(mask))
+    return out, mask
+
+
+def unpack_se3(sums):
+    s = _c(sums, np.float64)
+    A = np.zeros(36, np.float64)
+    b = np.zeros(6, np.float64)
+    lib().orc_unpack_se3(_p(s), _p(A), _p(b))
+    return A.reshape(6, 6), b
+
+
+def ldlt_solve(A, b):
+    A = _c(A, np.float64)
+    b = _c(b, np.float64)
+    x = np.zeros(b.shape[0], np.float64)
+    z = lib().orc_ldlt_solve(b.shape[0], _p(A), _p(b), _p(x))
+    return x, z
+
+
+def rodrigues(r):
+    r = _c(r, np.float64)
+    R = np.zeros(9, np.float64)
+    lib().orc_rodrigues(_p(r), _p(R))
+    return R.reshape(3, 3)
+
+
+def bexpf(x: float) -> float:
+    return lib().orc_bexpf(x)
+
+
+# -------------------------------------------------------------------- tracker
+
+class Odometry:
+    """The oracle's RGBDOdometryef (RGBDOdometryef.h:28-131)."""
+
+    def __init__(self, width, height, cx, cy, fx, fy, distThresh=0.10, angleThresh=None, fp64=False):
+        import math
+        if angleThresh is None:
+            angleThresh = math.sin(20.0 * 3.14159254 / 180.0)
+        self.w, self.h = width, height
+        self._h = lib().orc_create(width, height, cx, cy, fx, fy, distThresh, angleThresh)
+        lib().orc_set_precision(self._h, int(fp64))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def initICP(self, depth, depthCutoff=20.0):
+        d = _c(depth, np.uint16)
+        lib().orc_init_icp_depth(self._h, _p(d), depthCutoff)
+
+    def initICPMaps(self, v4, n4, depthCutoff=20.0):
+        v, n = _c(v4, np.float32), _c(n4, np.float32)
+        lib().orc_init_icp_maps(self._h, _p(v), _p(n), depthCutoff)
+
+    def initICPModel(self, v4, n4, depthCutoff, pose):
+        v, n, p = _c(v4, np.float32), _c(n4, np.float32), _c(pose, np.float32)
+        lib().orc_init_icp_model(self._h, _p(v), _p(n), depthCutoff, _p(p))
+
+    def initRGB(self, rgb):
+        r = _c(rgb, np.uint8)
+        lib().orc_init_rgb(self._h, _p(r), r.shape[2])
+
+    def initRGBModel(self, rgb):
+        r = _c(rgb, np.uint8)
+        lib().orc_init_rgb_model(self._h, _p(r), r.shape[2])
+
+    def initFirstRGB(self, rgb):
+        r = _c(rgb, np.uint8)
+        lib().orc_init_first_rgb(self._h, _p(r), r.shape[2])
+
+    def getIncrementalTransformation(self, trans, rot, rgbOnly=False, icpWeight=10.0, pyramid=True,
+                                     fastOdom=False, so3=True):
+        t = _c(trans, np.float32).copy()
+        R = _c(rot, np.float32).copy()
+        lib().orc_track(self._h, _p(t), _p(R), int(rgbOnly), icpWeight, int(pyramid), int(fastOdom), int(so3))
+        return t, R
+
+    def stats(self):
+        s = np.zeros(7, np.float32)
+        lib().orc_get_stats(self._h, _p(s))
+        keys = ["lastICPError", "lastICPCount", "lastRGBError", "lastRGBCount", "lastSO3Error",
+                "lastSO3Count", "so3Iterations"]
+        return dict(zip(keys, s.tolist()))
+
+    def last_system(self):
+        A = np.zeros(36, np.float64)
+        b = np.zeros(6, np.float64)
+        lib().orc_get_last_system(self._h, _p(A), _p(b))
+        return A.reshape(6, 6), b
+
+    def getCovariance(self):
+        c = np.zeros(36, np.float64)
+        if lib().orc_get_covariance(self._h, _p(c)) != 0:
+            raise np.linalg.LinAlgError("lastA is singular")
+        return c.reshape(6, 6)
+
+    def buf(self, name, level):
+        h, w = self.h >> level, self.w >> level
+        L = lib()
+        for fn, dt, shape in ((L.orc_buf_f32, np.float32, None), (L.orc_buf_u8, np.uint8, None),
+                              (L.orc_buf_s16, np.int16, None), (L.orc_buf_u16, np.uint16, None)):
+            p = fn(self._h, name.encode(), level)
+            if p:
+                planes = 3 if name.startswith(("vmap", "nmap")) else 1
+                n = planes * h * w
+                a = np.ctypeslib.as_array(p, shape=(n,)).copy()
+                return a.reshape((3, h, w) if planes == 3 else (h, w))
+        raise KeyError(name)

@@ -1,0 +1,823 @@
+// odom_host.cpp -- the RGBDOdometry host object behind the C ABI of
+// include/slam_odom.h.
+//
+// It keeps the reference's object model (RGBDOdometryef.{h,cpp}): one object
+// owns every pyramid buffer, allocated once in the constructor
+// (RGBDOdometryef.cpp:21-111), and keeps the cross-call state (vmaps_tmp,
+// lastNextImage, lastA/lastb).  Unlike the reference it never returns to the
+// host inside a frame: each init* call and the whole Gauss-Newton loop of
+// getIncrementalTransformation are enqueued on the object's stream, the
+// per-iteration solve runs on the device, and only the synchronous entry
+// points wait.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "odom_internal.h"
+#include "odom_math.h"
+#include "slam_odom.h"
+
+using namespace slam;
+
+namespace {
+
+constexpr int kMinThreads = 128;
+
+int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int fmt_bpp(int fmt)
+{
+    switch (fmt) {
+        case ODOM_FMT_DEPTH_U16: return 2;
+        case ODOM_FMT_RGB8: return 3;
+        case ODOM_FMT_RGBA8: return 4;
+        case ODOM_FMT_RGBA32F: return 16;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+struct odom_ctx {
+    odom_params prm{};
+    int w[kLevels], h[kLevels], n[kLevels];
+    float fx[kLevels], fy[kLevels], cx[kLevels], cy[kLevels];
+    hipStream_t stream = nullptr;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+
+    uint16_t* depth[kLevels];
+    float *vcur[kLevels], *ncur[kLevels], *vprev[kLevels], *nprev[kLevels];
+    float *lastDepth[kLevels], *nextDepth[kLevels];
+    uint8_t *lastImage[kLevels], *nextImage[kLevels], *lastNextImage[kLevels];
+    short2* dxy[kLevels];
+    float* zsrc[kLevels];
+    float* vz = nullptr;              // z plane of the last vertex input (vmaps_tmp)
+    uint16_t* stage_u16 = nullptr;
+    uint8_t* stage_rgb = nullptr;
+    float4* stage_v4 = nullptr;
+    float4* stage_n4 = nullptr;
+    uint8_t* dbg_mask = nullptr;
+    TrackState* st = nullptr;
+    PairDesc* d_pair = nullptr;
+    float* part_a = nullptr;
+    float* part_b = nullptr;
+    unsigned int* part_ai = nullptr;
+    double* d_out11 = nullptr;
+    float* d_c27 = nullptr;
+    int maxblk = 0;
+
+    PairDesc h_pair{};
+    StepShape shape_a[kLevels], shape_b[kLevels];
+    TrackState h_st{};
+    bool have_state = false;
+    hipError_t last_hip = hipSuccess;
+};
+
+namespace {
+
+#define HIPCHK(ctx, x)                                    \
+    do {                                                  \
+        hipError_t e_ = (x);                              \
+        if (e_ != hipSuccess) {                           \
+            (ctx)->last_hip = e_;                         \
+            fprintf(stderr, "slam_odom: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return ODOM_ERR_HIP;                          \
+        }                                                 \
+    } while (0)
+
+int check_launch(odom_ctx* c)
+{
+    hipError_t e = hipPeekAtLastError();
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->last_hip = e;
+        fprintf(stderr, "slam_odom: kernel launch failed: %s\n", hipGetErrorString(e));
+        return ODOM_ERR_HIP;
+    }
+    return ODOM_OK;
+}
+
+// Rebuild the host descriptor from the buffers and shapes, and upload it.
+int upload_pair(odom_ctx* c)
+{
+    PairDesc& D = c->h_pair;
+    for (int l = 0; l < kLevels; ++l) {
+        LevelDesc& L = D.lv[l];
+        L.w = c->w[l]; L.h = c->h[l];
+        L.fx = c->fx[l]; L.fy = c->fy[l]; L.cx = c->cx[l]; L.cy = c->cy[l];
+        L.invFx = 1.0f / c->fx[l];
+        L.invFy = 1.0f / c->fy[l];
+        L.vcur = c->vcur[l]; L.ncur = c->ncur[l]; L.vprev = c->vprev[l]; L.nprev = c->nprev[l];
+        L.zsrc = c->zsrc[l]; L.dxy = c->dxy[l]; L.inext = c->nextImage[l];
+        L.zlast = c->lastDepth[l]; L.ilast = c->lastImage[l];
+        const float sobelScale = (float)(1.0 / pow(2.0, 3));
+        const double minGrad[kLevels] = {5, 3, 1};
+        L.minScale = (float)(pow(minGrad[l], 2.0) / pow((double)sobelScale, 2.0));
+        const StepShape& sa = c->shape_a[l];
+        const StepShape& sb = c->shape_b[l];
+        D.nblk_a[l] = cdiv(c->n[l], sa.threads * sa.ppt);
+        D.nblk_b[l] = cdiv(c->n[l], sb.threads * sb.ppt);
+        D.blk0_a[l] = 0;
+        D.blk0_b[l] = 0;
+    }
+    D.st = c->st;
+    D.part_a = c->part_a;
+    D.part_b = c->part_b;
+    D.part_ai = c->part_ai;
+    D.so3_last = c->lastNextImage[2];
+    D.so3_next = c->nextImage[2];
+    HIPCHK(c, hipMemcpyAsync(c->d_pair, &D, sizeof(PairDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ODOM_OK;
+}
+
+// Resolve a frame view to a device pointer and row pitch (bytes); host data
+// is copied into `staging` first.
+int resolve(odom_ctx* c, const odom_frame_view* v, int fmt_a, int fmt_b, void* staging, const void** dptr,
+            int* pitch, int* fmt_out)
+{
+    if (!v || !v->ptr) return ODOM_ERR_ARG;
+    if (v->format != fmt_a && v->format != fmt_b) return ODOM_ERR_ARG;
+    if (v->width != c->w[0] || v->height != c->h[0]) return ODOM_ERR_ARG;
+    const int bpp = fmt_bpp(v->format);
+    const int row = v->width * bpp;
+    const int p = v->pitch_bytes ? v->pitch_bytes : row;
+    if (p < row) return ODOM_ERR_ARG;
+    if (v->format == ODOM_FMT_RGBA32F && (p % 16) != 0) return ODOM_ERR_ARG;
+    if (v->format == ODOM_FMT_DEPTH_U16 && (p % 2) != 0) return ODOM_ERR_ARG;
+    *fmt_out = v->format;
+    if (v->mem == ODOM_MEM_HOST) {
+        HIPCHK(c, hipMemcpy2DAsync(staging, row, v->ptr, p, row, v->height, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *dptr = staging;
+        *pitch = row;
+    } else if (v->mem == ODOM_MEM_DEVICE) {
+        *dptr = v->ptr;
+        *pitch = p;
+    } else {
+        return ODOM_ERR_ARG;
+    }
+    return ODOM_OK;
+}
+
+int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
+{
+    for (int l = 1; l < kLevels; ++l) launch_pyrdown_u16(c->depth[l - 1], c->w[l - 1], c->h[l - 1], c->depth[l], c->stream);
+    VNLevels a{};
+    for (int l = 0; l < kLevels; ++l) {
+        a.depth[l] = c->depth[l];
+        a.vmap[l] = c->vcur[l];
+        a.nmap[l] = c->ncur[l];
+        a.w[l] = c->w[l];
+        a.h[l] = c->h[l];
+        a.fx_inv[l] = 1.f / c->fx[l];
+        a.fy_inv[l] = 1.f / c->fy[l];
+        a.cx[l] = c->cx[l];
+        a.cy[l] = c->cy[l];
+    }
+    a.cutoff = cutoff;
+    launch_vmap_nmap(a, c->stream);
+    return check_launch(c);
+}
+
+int enqueue_model(odom_ctx* c, const odom_frame_view* v, const odom_frame_view* n, const float* pose, bool model)
+{
+    const void *pv, *pn;
+    int pitch_v, pitch_n, f;
+    int rc = resolve(c, v, ODOM_FMT_RGBA32F, ODOM_FMT_RGBA32F, c->stage_v4, &pv, &pitch_v, &f);
+    if (rc) return rc;
+    rc = resolve(c, n, ODOM_FMT_RGBA32F, ODOM_FMT_RGBA32F, c->stage_n4, &pn, &pitch_n, &f);
+    if (rc) return rc;
+    ModelLevels a{};
+    a.v4 = (const float4*)pv;
+    a.n4 = (const float4*)pn;
+    a.v_pitch = pitch_v / 16;
+    a.n_pitch = pitch_n / 16;
+    for (int l = 0; l < kLevels; ++l) {
+        a.vmap[l] = model ? c->vprev[l] : c->vcur[l];
+        a.nmap[l] = model ? c->nprev[l] : c->ncur[l];
+        a.w[l] = c->w[l];
+        a.h[l] = c->h[l];
+    }
+    a.vz = c->vz;
+    a.transform = model ? 1 : 0;
+    if (model) {
+        // Rcam = modelPose.topLeftCorner(3,3), tcam = topRightCorner (RGBDOdometryef.cpp:194-198)
+        const float R[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
+        const float t[3] = {pose[3], pose[7], pose[11]};
+        memcpy(a.R, R, sizeof R);
+        memcpy(a.t, t, sizeof t);
+    }
+    launch_model_maps(a, c->stream);
+    return check_launch(c);
+}
+
+// populateRGBDData, RGBDOdometryef.cpp:208-235 (maxDepthRGB 6.0, :37)
+int enqueue_populate(odom_ctx* c, const odom_frame_view* rgb, float** dstDepth, uint8_t** dstImage)
+{
+    const void* p;
+    int pitch, fmt;
+    int rc = resolve(c, rgb, ODOM_FMT_RGB8, ODOM_FMT_RGBA8, c->stage_rgb, &p, &pitch, &fmt);
+    if (rc) return rc;
+    if (dstDepth) {
+        launch_vertices_to_depth(c->vz, c->w[0], c->h[0], 6.0f, dstDepth[0], c->stream);
+        for (int l = 0; l + 1 < kLevels; ++l)
+            launch_pyrdown_f32(dstDepth[l], c->w[l], c->h[l], dstDepth[l + 1], c->stream);
+    }
+    launch_intensity((const uint8_t*)p, pitch, fmt == ODOM_FMT_RGB8 ? 3 : 4, c->w[0], c->h[0], dstImage[0], c->stream);
+    for (int l = 0; l + 1 < kLevels; ++l) launch_pyrdown_u8(dstImage[l], c->w[l], c->h[l], dstImage[l + 1], c->stream);
+    return check_launch(c);
+}
+
+TrackParams make_params(odom_ctx* c, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3)
+{
+    TrackParams P{};
+    P.icp = (!rgbOnly && icpWeight > 0) ? 1 : 0;
+    P.rgb = (rgbOnly || icpWeight < 100) ? 1 : 0;
+    P.rgbOnly = rgbOnly ? 1 : 0;
+    P.so3 = so3 ? 1 : 0;
+    P.icpWeight = icpWeight;
+    P.distThres = c->prm.dist_thresh;
+    P.angleThres = c->prm.angle_thresh;
+    P.sobelScale = (float)(1.0 / pow(2.0, 3));
+    P.maxDepthDelta = 0.07f;
+    P.iters[0] = fastOdom ? 3 : 10;
+    P.iters[1] = pyramid ? 5 : 0;
+    P.iters[2] = pyramid ? 4 : 0;
+    return P;
+}
+
+StepLaunch step_launch(odom_ctx* c, int kernel, int level)
+{
+    StepLaunch L;
+    L.pairs = c->d_pair;
+    L.npairs = 1;
+    L.blk_pair = nullptr;
+    L.shape = kernel == 0 ? c->shape_a[level] : c->shape_b[level];
+    L.total_blocks = kernel == 0 ? c->h_pair.nblk_a[level] : c->h_pair.nblk_b[level];
+    return L;
+}
+
+void prep_levels(odom_ctx* c, PrepLevels& a)
+{
+    for (int l = 0; l < kLevels; ++l) {
+        a.img[l] = c->nextImage[l];
+        a.depth[l] = c->nextDepth[l];
+        a.dxy[l] = c->dxy[l];
+        a.zsrc[l] = c->zsrc[l];
+        a.w[l] = c->w[l];
+        a.h[l] = c->h[l];
+        a.minScale[l] = c->h_pair.lv[l].minScale;
+    }
+}
+
+// getIncrementalTransformation (RGBDOdometryef.cpp:267-595) without the
+// host round trips: derivatives, SO3 loop, begin, 2 passes per Gauss-Newton
+// iteration, end, image swap.
+int enqueue_track(odom_ctx* c, const TrackParams& P)
+{
+    if (P.rgb) {
+        PrepLevels a{};
+        prep_levels(c, a);
+        launch_rgb_prep(a, c->stream);
+    }
+    if (P.so3) launch_so3(c->d_pair, 1, 0, c->stream);
+    std::vector<int> sched;
+    for (int l = kLevels - 1; l >= 0; --l)
+        for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
+    const int first = sched.empty() ? -1 : sched[0] / 64;
+    launch_track_begin(c->d_pair, 1, P, first, c->stream);
+    for (size_t k = 0; k < sched.size(); ++k) {
+        const int level = sched[k] / 64, iter = sched[k] % 64;
+        const int next = (k + 1 < sched.size()) ? sched[k + 1] / 64 : -1;
+        int brk = -1;
+        for (int l = level - 1; l >= 0 && brk < 0; --l)
+            if (P.iters[l] > 0) brk = l;
+        launch_step_a(step_launch(c, 0, level), level, iter, next, brk, P, 0, nullptr, c->stream);
+        if (P.rgb) launch_step_b(step_launch(c, 1, level), level, iter, next, P, 0, c->stream);
+    }
+    launch_track_end(c->d_pair, 1, P, c->stream);
+    if (P.so3)
+        for (int l = 0; l < kLevels; ++l) launch_swap_u8(c->lastNextImage[l], c->nextImage[l], (size_t)c->n[l], c->stream);
+    return check_launch(c);
+}
+
+int read_state(odom_ctx* c)
+{
+    HIPCHK(c, hipMemcpyAsync(&c->h_st, c->st, sizeof(TrackState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return ODOM_OK;
+}
+
+template <class T>
+int write_field(odom_ctx* c, T* dev_field, const T* host, size_t count)
+{
+    HIPCHK(c, hipMemcpyAsync(dev_field, host, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    return ODOM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int odom_default_params(odom_params* p, int width, int height, float cx, float cy, float fx, float fy)
+{
+    if (!p) return ODOM_ERR_ARG;
+    p->width = width;
+    p->height = height;
+    p->cx = cx;
+    p->cy = cy;
+    p->fx = fx;
+    p->fy = fy;
+    p->dist_thresh = 0.10f;
+    p->angle_thresh = sinf(20.f * 3.14159254f / 180.f);
+    return ODOM_OK;
+}
+
+int odom_device_info(char* name, int name_len, char* arch, int arch_len)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ODOM_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return ODOM_ERR_NODEV;
+    if (name && name_len > 0) snprintf(name, name_len, "%s", prop.name);
+    if (arch && arch_len > 0) snprintf(arch, arch_len, "%s", prop.gcnArchName);
+    return ODOM_OK;
+}
+
+int odom_create(const odom_params* p, odom_handle* out)
+{
+    if (!p || !out) return ODOM_ERR_ARG;
+    if (p->width < 16 || p->height < 16 || p->width > 8192 || p->height > 8192) return ODOM_ERR_ARG;
+    char arch[128] = {0};
+    if (odom_device_info(nullptr, 0, arch, sizeof arch) != ODOM_OK) return ODOM_ERR_NODEV;
+    if (strncmp(arch, "gfx950", 6) != 0) {
+        fprintf(stderr, "slam_odom: device is %s; this library is built for gfx950 only\n", arch);
+        return ODOM_ERR_NODEV;
+    }
+    odom_ctx* c = new odom_ctx();
+    c->prm = *p;
+    for (int l = 0; l < kLevels; ++l) {
+        c->w[l] = p->width >> l;
+        c->h[l] = p->height >> l;
+        c->n[l] = c->w[l] * c->h[l];
+        // CameraModel::operator()(level), Camera.h:14-18
+        const int div = 1 << l;
+        c->fx[l] = p->fx / (float)div;
+        c->fy[l] = p->fy / (float)div;
+        c->cx[l] = p->cx / (float)div;
+        c->cy[l] = p->cy / (float)div;
+        c->shape_a[l] = StepShape{256, 4};
+        c->shape_b[l] = StepShape{256, 4};
+    }
+    c->maxblk = cdiv(c->n[0], kMinThreads);
+    // one arena for every device buffer
+    const size_t N0 = (size_t)c->n[0];
+    struct Req { void** dst; size_t bytes; };
+    std::vector<Req> reqs;
+    for (int l = 0; l < kLevels; ++l) {
+        const size_t n = (size_t)c->n[l];
+        reqs.push_back({(void**)&c->depth[l], n * 2});
+        reqs.push_back({(void**)&c->vcur[l], n * 12});
+        reqs.push_back({(void**)&c->ncur[l], n * 12});
+        reqs.push_back({(void**)&c->vprev[l], n * 12});
+        reqs.push_back({(void**)&c->nprev[l], n * 12});
+        reqs.push_back({(void**)&c->lastDepth[l], n * 4});
+        reqs.push_back({(void**)&c->nextDepth[l], n * 4});
+        reqs.push_back({(void**)&c->lastImage[l], n});
+        reqs.push_back({(void**)&c->nextImage[l], n});
+        reqs.push_back({(void**)&c->lastNextImage[l], n});
+        reqs.push_back({(void**)&c->dxy[l], n * 4});
+        reqs.push_back({(void**)&c->zsrc[l], n * 4});
+    }
+    reqs.push_back({(void**)&c->vz, N0 * 4});
+    reqs.push_back({(void**)&c->stage_u16, N0 * 2});
+    reqs.push_back({(void**)&c->stage_rgb, N0 * 4});
+    reqs.push_back({(void**)&c->stage_v4, N0 * 16});
+    reqs.push_back({(void**)&c->stage_n4, N0 * 16});
+    reqs.push_back({(void**)&c->dbg_mask, N0});
+    reqs.push_back({(void**)&c->st, sizeof(TrackState)});
+    reqs.push_back({(void**)&c->d_pair, sizeof(PairDesc)});
+    reqs.push_back({(void**)&c->part_a, (size_t)c->maxblk * kPartStride * 4});
+    reqs.push_back({(void**)&c->part_b, (size_t)c->maxblk * kPartStride * 4});
+    reqs.push_back({(void**)&c->part_ai, (size_t)c->maxblk * 2 * 4});
+    reqs.push_back({(void**)&c->d_out11, 16 * 8});
+    reqs.push_back({(void**)&c->d_c27, 32 * 4});
+    size_t total = 0;
+    for (auto& r : reqs) total += align_up(r.bytes, 256);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->arena, total) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ODOM_ERR_HIP;
+    }
+    c->arena_bytes = total;
+    size_t off = 0;
+    for (auto& r : reqs) {
+        *r.dst = c->arena + off;
+        off += align_up(r.bytes, 256);
+    }
+    // zero everything once (the reference leaves buffers uninitialised; zero
+    // makes a call-order mistake deterministic)
+    if (hipMemsetAsync(c->arena, 0, total, c->stream) != hipSuccess) {
+        odom_destroy(c);
+        return ODOM_ERR_HIP;
+    }
+    // initial stats (RGBDOdometryef.cpp:26-33) and identity pose
+    TrackState s{};
+    s.lastICPCount = (float)(p->width * p->height);
+    s.lastRGBCount = (float)(p->width * p->height);
+    s.lastSO3Count = (float)(p->width * p->height);
+    s.pose_out[0] = s.pose_out[5] = s.pose_out[10] = 1.0f;
+    s.break_level = -1;
+    for (int k = 0; k < 9; ++k) s.so3R[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    if (hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        upload_pair(c) != ODOM_OK) {
+        odom_destroy(c);
+        return ODOM_ERR_HIP;
+    }
+    c->h_st = s;
+    *out = c;
+    return ODOM_OK;
+}
+
+int odom_destroy(odom_handle c)
+{
+    if (!c) return ODOM_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ODOM_OK;
+}
+
+int odom_bilateral_depth(odom_handle c, const odom_frame_view* raw, float maxD, void* dst, int dst_mem)
+{
+    if (!c || !dst) return ODOM_ERR_ARG;
+    const void* p;
+    int pitch, fmt;
+    int rc = resolve(c, raw, ODOM_FMT_DEPTH_U16, ODOM_FMT_DEPTH_U16, c->stage_u16, &p, &pitch, &fmt);
+    if (rc) return rc;
+    if (dst_mem == ODOM_MEM_DEVICE) {
+        launch_bilateral((const uint16_t*)p, pitch / 2, (uint16_t*)dst, c->w[0], c->h[0], maxD, c->stream);
+        return check_launch(c);
+    }
+    // host destination: filter into scratch (the RGB staging area), then copy out
+    uint16_t* tmp = (uint16_t*)c->stage_rgb;
+    launch_bilateral((const uint16_t*)p, pitch / 2, tmp, c->w[0], c->h[0], maxD, c->stream);
+    rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(dst, tmp, (size_t)c->n[0] * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ODOM_OK;
+}
+
+int odom_init_icp_depth(odom_handle c, const odom_frame_view* depth, float cutoff)
+{
+    if (!c) return ODOM_ERR_ARG;
+    const void* p;
+    int pitch, fmt;
+    int rc = resolve(c, depth, ODOM_FMT_DEPTH_U16, ODOM_FMT_DEPTH_U16, c->stage_u16, &p, &pitch, &fmt);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy2DAsync(c->depth[0], (size_t)c->w[0] * 2, p, pitch, (size_t)c->w[0] * 2, c->h[0],
+                               hipMemcpyDeviceToDevice, c->stream));
+    return enqueue_depth_pyramid_maps(c, cutoff);
+}
+
+int odom_init_icp_raw_depth(odom_handle c, const odom_frame_view* raw, float maxD, float cutoff)
+{
+    if (!c) return ODOM_ERR_ARG;
+    const void* p;
+    int pitch, fmt;
+    int rc = resolve(c, raw, ODOM_FMT_DEPTH_U16, ODOM_FMT_DEPTH_U16, c->stage_u16, &p, &pitch, &fmt);
+    if (rc) return rc;
+    launch_bilateral((const uint16_t*)p, pitch / 2, c->depth[0], c->w[0], c->h[0], maxD, c->stream);
+    return enqueue_depth_pyramid_maps(c, cutoff);
+}
+
+int odom_init_icp_maps(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff)
+{
+    (void)cutoff;
+    if (!c) return ODOM_ERR_ARG;
+    return enqueue_model(c, v, n, nullptr, false);
+}
+
+int odom_init_icp_model(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff,
+                        const float pose[16])
+{
+    (void)cutoff;
+    if (!c || !pose) return ODOM_ERR_ARG;
+    return enqueue_model(c, v, n, pose, true);
+}
+
+int odom_init_rgb(odom_handle c, const odom_frame_view* rgb)
+{
+    if (!c) return ODOM_ERR_ARG;
+    return enqueue_populate(c, rgb, c->nextDepth, c->nextImage);
+}
+
+int odom_init_rgb_model(odom_handle c, const odom_frame_view* rgb)
+{
+    if (!c) return ODOM_ERR_ARG;
+    return enqueue_populate(c, rgb, c->lastDepth, c->lastImage);
+}
+
+int odom_init_first_rgb(odom_handle c, const odom_frame_view* rgb)
+{
+    if (!c) return ODOM_ERR_ARG;
+    return enqueue_populate(c, rgb, nullptr, c->lastNextImage);
+}
+
+int odom_set_pose(odom_handle c, const float trans[3], const float rot[9])
+{
+    if (!c || !trans || !rot) return ODOM_ERR_ARG;
+    float pose[12];
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) pose[a * 4 + b] = rot[a * 3 + b];
+        pose[a * 4 + 3] = trans[a];
+    }
+    return write_field(c, c->st->pose_out, pose, 12);
+}
+
+int odom_get_pose(odom_handle c, float trans[3], float rot[9])
+{
+    if (!c || !trans || !rot) return ODOM_ERR_ARG;
+    int rc = read_state(c);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) rot[a * 3 + b] = c->h_st.pose_out[a * 4 + b];
+        trans[a] = c->h_st.pose_out[a * 4 + 3];
+    }
+    return ODOM_OK;
+}
+
+int odom_synchronize(odom_handle c)
+{
+    if (!c) return ODOM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ODOM_OK;
+}
+
+int odom_track_async(odom_handle c, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3)
+{
+    if (!c) return ODOM_ERR_ARG;
+    const TrackParams P = make_params(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;  // RGBDOdometryef.cpp:558-561
+    return enqueue_track(c, P);
+}
+
+int odom_track(odom_handle c, float trans[3], float rot[9], int rgbOnly, float icpWeight, int pyramid,
+               int fastOdom, int so3)
+{
+    if (!c || !trans || !rot) return ODOM_ERR_ARG;
+    int rc = odom_set_pose(c, trans, rot);
+    if (rc) return rc;
+    rc = odom_track_async(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (rc) return rc;
+    return odom_get_pose(c, trans, rot);
+}
+
+int odom_get_stats(odom_handle c, odom_stats* s)
+{
+    if (!c || !s) return ODOM_ERR_ARG;
+    int rc = read_state(c);
+    if (rc) return rc;
+    const TrackState& t = c->h_st;
+    s->last_icp_error = t.lastICPError;
+    s->last_icp_count = t.lastICPCount;
+    s->last_rgb_error = t.lastRGBError;
+    s->last_rgb_count = t.lastRGBCount;
+    s->last_so3_error = t.lastSO3Error;
+    s->last_so3_count = t.lastSO3Count;
+    s->so3_iterations = t.so3_iters;
+    s->gn_iterations = t.iters_done;
+    s->degenerate = t.degenerate;
+    s->reverted = t.reverted;
+    return ODOM_OK;
+}
+
+int odom_get_last_system(odom_handle c, double A[36], double b[6])
+{
+    if (!c || !A || !b) return ODOM_ERR_ARG;
+    int rc = read_state(c);
+    if (rc) return rc;
+    memcpy(A, c->h_st.lastA, sizeof(double) * 36);
+    memcpy(b, c->h_st.lastb, sizeof(double) * 6);
+    return ODOM_OK;
+}
+
+// lastA^-1 by Gauss-Jordan elimination with partial pivoting (the LU inverse
+// of RGBDOdometryef.cpp:599).
+int odom_get_covariance(odom_handle c, double cov[36])
+{
+    if (!c || !cov) return ODOM_ERR_ARG;
+    int rc = read_state(c);
+    if (rc) return rc;
+    double a[36], inv[36];
+    memcpy(a, c->h_st.lastA, sizeof a);
+    for (int i = 0; i < 36; ++i) inv[i] = (i % 7 == 0) ? 1.0 : 0.0;
+    for (int col = 0; col < 6; ++col) {
+        int p = col;
+        for (int r = col + 1; r < 6; ++r)
+            if (fabs(a[r * 6 + col]) > fabs(a[p * 6 + col])) p = r;
+        if (a[p * 6 + col] == 0.0) return ODOM_ERR_STATE;
+        if (p != col)
+            for (int k = 0; k < 6; ++k) {
+                std::swap(a[col * 6 + k], a[p * 6 + k]);
+                std::swap(inv[col * 6 + k], inv[p * 6 + k]);
+            }
+        const double d = a[col * 6 + col];
+        for (int k = 0; k < 6; ++k) { a[col * 6 + k] /= d; inv[col * 6 + k] /= d; }
+        for (int r = 0; r < 6; ++r) {
+            if (r == col) continue;
+            const double f = a[r * 6 + col];
+            for (int k = 0; k < 6; ++k) { a[r * 6 + k] -= f * a[col * 6 + k]; inv[r * 6 + k] -= f * inv[col * 6 + k]; }
+        }
+    }
+    memcpy(cov, inv, sizeof inv);
+    return ODOM_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+
+int odom_icp_step(odom_handle c, int level, const float Rc[9], const float tc[3], const float Rpi[9],
+                  const float tp[3], double out[29], uint8_t* mask)
+{
+    if (!c || level < 0 || level >= kLevels || !Rc || !tc || !Rpi || !tp || !out) return ODOM_ERR_ARG;
+    int rc;
+    if ((rc = write_field(c, c->st->Rcurr, Rc, 9)) || (rc = write_field(c, c->st->tcurr, tc, 3)) ||
+        (rc = write_field(c, c->st->Rprev_inv, Rpi, 9)) || (rc = write_field(c, c->st->tprev, tp, 3)))
+        return rc;
+    TrackParams P = make_params(c, 0, 100.0f, 1, 0, 0);
+    P.icp = 1;
+    P.rgb = 0;
+    launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    if ((rc = read_state(c))) return rc;
+    memcpy(out, c->h_st.icp_sums, sizeof(double) * kSE3);
+    if (mask) {
+        HIPCHK(c, hipMemcpy(mask, c->dbg_mask, (size_t)c->n[level], hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n[level]; ++i) mask[i] = mask[i] & 1u;
+    }
+    return ODOM_OK;
+}
+
+int odom_compute_derivatives(odom_handle c)
+{
+    if (!c) return ODOM_ERR_ARG;
+    PrepLevels a{};
+    prep_levels(c, a);
+    launch_rgb_prep(a, c->stream);
+    return check_launch(c);
+}
+
+int odom_rgb_residual(odom_handle c, int level, const float K[9], const float kt[3], int32_t* count,
+                      int32_t* sigma, uint8_t* mask)
+{
+    if (!c || level < 0 || level >= kLevels || !K || !kt || !count || !sigma) return ODOM_ERR_ARG;
+    int rc;
+    if ((rc = write_field(c, c->st->krkinv, K, 9)) || (rc = write_field(c, c->st->kt, kt, 3))) return rc;
+    TrackParams P = make_params(c, 0, 10.0f, 1, 0, 0);
+    P.icp = 0;
+    P.rgb = 1;
+    launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    if ((rc = read_state(c))) return rc;
+    *count = c->h_st.rgb_count;
+    *sigma = c->h_st.rgb_sigma;
+    if (mask) {
+        HIPCHK(c, hipMemcpy(mask, c->dbg_mask, (size_t)c->n[level], hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n[level]; ++i) mask[i] = (mask[i] >> 1) & 1u;
+    }
+    return ODOM_OK;
+}
+
+int odom_rgb_step(odom_handle c, int level, float sigma, const float K[9], const float kt[3], double out[29])
+{
+    if (!c || level < 0 || level >= kLevels || !K || !kt || !out) return ODOM_ERR_ARG;
+    int rc;
+    if ((rc = write_field(c, c->st->krkinv, K, 9)) || (rc = write_field(c, c->st->kt, kt, 3)) ||
+        (rc = write_field(c, &c->st->sigmaVal, &sigma, 1)))
+        return rc;
+    TrackParams P = make_params(c, 0, 10.0f, 1, 0, 0);
+    P.icp = 0;
+    P.rgb = 1;
+    launch_step_b(step_launch(c, 1, level), level, 0, -1, P, 1, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    if ((rc = read_state(c))) return rc;
+    memcpy(out, c->h_st.rgb_sums, sizeof(double) * kSE3);
+    return ODOM_OK;
+}
+
+int odom_so3_step(odom_handle c, const float H[9], const float Ki[9], const float KR[9], double out[11],
+                  uint8_t* mask)
+{
+    if (!c || !H || !Ki || !KR || !out) return ODOM_ERR_ARG;
+    float c27[27];
+    memcpy(c27, H, 36);
+    memcpy(c27 + 9, Ki, 36);
+    memcpy(c27 + 18, KR, 36);
+    HIPCHK(c, hipMemcpyAsync(c->d_c27, c27, sizeof c27, hipMemcpyHostToDevice, c->stream));
+    launch_so3_step_dbg(c->d_pair, c->d_c27, c->d_out11, mask ? c->dbg_mask : nullptr, c->stream);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_out11, sizeof(double) * kSO3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mask) HIPCHK(c, hipMemcpy(mask, c->dbg_mask, (size_t)c->n[2], hipMemcpyDeviceToHost));
+    return ODOM_OK;
+}
+
+int odom_read_buffer(odom_handle c, const char* name, int level, void* dst, size_t bytes)
+{
+    if (!c || !name || !dst || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
+    const std::string s(name);
+    const size_t n = (size_t)c->n[level];
+    const void* src = nullptr;
+    size_t need = 0;
+    if (s == "depth") { src = c->depth[level]; need = n * 2; }
+    else if (s == "vmap_curr") { src = c->vcur[level]; need = n * 12; }
+    else if (s == "nmap_curr") { src = c->ncur[level]; need = n * 12; }
+    else if (s == "vmap_g_prev") { src = c->vprev[level]; need = n * 12; }
+    else if (s == "nmap_g_prev") { src = c->nprev[level]; need = n * 12; }
+    else if (s == "last_depth") { src = c->lastDepth[level]; need = n * 4; }
+    else if (s == "next_depth") { src = c->nextDepth[level]; need = n * 4; }
+    else if (s == "last_image") { src = c->lastImage[level]; need = n; }
+    else if (s == "next_image") { src = c->nextImage[level]; need = n; }
+    else if (s == "last_next_image") { src = c->lastNextImage[level]; need = n; }
+    else if (s == "dxy") { src = c->dxy[level]; need = n * 4; }
+    else if (s == "zsrc") { src = c->zsrc[level]; need = n * 4; }
+    else return ODOM_ERR_ARG;
+    if (bytes < need) return ODOM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+    return ODOM_OK;
+}
+
+int odom_set_step_shape(odom_handle c, int kernel, int level, int threads, int ppt)
+{
+    if (!c || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
+    const StepShape sh{threads, ppt};
+    if (!step_shape_supported(sh)) return ODOM_ERR_ARG;
+    if (kernel == 0) c->shape_a[level] = sh;
+    else c->shape_b[level] = sh;
+    return upload_pair(c);
+}
+
+int odom_get_step_shape(odom_handle c, int kernel, int level, int* threads, int* ppt)
+{
+    if (!c || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels || !threads || !ppt) return ODOM_ERR_ARG;
+    const StepShape sh = kernel == 0 ? c->shape_a[level] : c->shape_b[level];
+    *threads = sh.threads;
+    *ppt = sh.ppt;
+    return ODOM_OK;
+}
+
+int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
+{
+    if (!c) return ODOM_ERR_ARG;
+    TrackParams P = make_params(c, rgbOnly, icpWeight, 1, 0, 0);
+    const int threads[] = {128, 256, 512};
+    const int ppts[] = {1, 2, 4, 8};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int rc = odom_compute_derivatives(c);
+    if (rc) return rc;
+    for (int kernel = 0; kernel < 2; ++kernel) {
+        if (kernel == 1 && !P.rgb) break;
+        for (int level = 0; level < kLevels; ++level) {
+            float best = 1e30f;
+            StepShape best_sh = kernel == 0 ? c->shape_a[level] : c->shape_b[level];
+            for (int t : threads)
+                for (int q : ppts) {
+                    if ((rc = odom_set_step_shape(c, kernel, level, t, q))) return rc;
+                    const int reps = 20;
+                    for (int r = 0; r < reps + 3; ++r) {
+                        if (r == 3) HIPCHK(c, hipEventRecord(e0, c->stream));
+                        if (kernel == 0) launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, nullptr, c->stream);
+                        else launch_step_b(step_launch(c, 1, level), level, 0, -1, P, 1, c->stream);
+                    }
+                    HIPCHK(c, hipEventRecord(e1, c->stream));
+                    HIPCHK(c, hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+                    if ((rc = check_launch(c))) return rc;
+                    if (ms < best) { best = ms; best_sh = StepShape{t, q}; }
+                }
+            if ((rc = odom_set_step_shape(c, kernel, level, best_sh.threads, best_sh.ppt))) return rc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ODOM_OK;
+}
+
+}  // extern "C"

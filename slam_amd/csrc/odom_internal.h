@@ -1,0 +1,93 @@
+// odom_internal.h -- launchers of the HIP kernels (odom_kernels.hip), called by
+// the host class (odom_host.cpp).  All launchers are asynchronous on `s`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "odom_device.h"
+
+namespace slam {
+
+// Launch shape of a step kernel at one level (replaces GPUConfig.h's
+// threads/blocks table; chosen by the autotuner).
+struct StepShape {
+    int threads;    // 64, 128, 256 or 512
+    int ppt;        // pixels per thread: 1, 2, 4, 8 or 16
+};
+
+// Kernel names and the shapes they support, for the autotuner.
+int step_shape_supported(StepShape sh);
+
+// ---- preprocessing (a1-a13) ----
+void launch_bilateral(const uint16_t* src, int src_pitch, uint16_t* dst, int w, int h, float maxD,
+                      hipStream_t s);
+void launch_pyrdown_u16(const uint16_t* src, int w, int h, uint16_t* dst, hipStream_t s);
+// depth pyramid -> vertex+normal maps of all levels in one launch
+struct VNLevels {
+    const uint16_t* depth[kLevels];
+    float* vmap[kLevels];
+    float* nmap[kLevels];
+    int w[kLevels], h[kLevels];
+    float fx_inv[kLevels], fy_inv[kLevels], cx[kLevels], cy[kLevels];
+    float cutoff;
+};
+void launch_vmap_nmap(const VNLevels& a, hipStream_t s);
+// RGBA32F vertex/normal maps -> planar pyramids (copyMaps + resize, optional
+// rigid transform), plus the vertex z plane kept for verticesToDepth.
+struct ModelLevels {
+    const float4* v4;
+    const float4* n4;
+    int v_pitch, n_pitch;           // in float4 elements
+    float* vmap[kLevels];
+    float* nmap[kLevels];
+    int w[kLevels], h[kLevels];
+    float* vz;                      // w0*h0 z values (may be null)
+    int transform;
+    float R[9], t[3];
+};
+void launch_model_maps(const ModelLevels& a, hipStream_t s);
+void launch_vertices_to_depth(const float* vz, int w, int h, float cutoff, float* depth, hipStream_t s);
+void launch_pyrdown_f32(const float* src, int w, int h, float* dst, hipStream_t s);
+void launch_intensity(const uint8_t* rgb, int pitch_bytes, int channels, int w, int h, uint8_t* dst,
+                      hipStream_t s);
+void launch_pyrdown_u8(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
+// derivatives (a12) + static photometric source mask, all levels of all pairs
+struct PrepLevels {
+    const uint8_t* img[kLevels];
+    const float* depth[kLevels];
+    short2* dxy[kLevels];
+    float* zsrc[kLevels];
+    int w[kLevels], h[kLevels];
+    float minScale[kLevels];
+};
+void launch_rgb_prep(const PrepLevels& a, hipStream_t s);
+
+// ---- tracking (a14-a21) ----
+struct StepLaunch {
+    const PairDesc* pairs;          // device array
+    int npairs;
+    const int* blk_pair;            // device table, per level (null when npairs == 1)
+    int total_blocks;               // blocks of this level over all pairs
+    StepShape shape;
+};
+void launch_so3(const PairDesc* pairs, int npairs, int mode, hipStream_t s);
+void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, int first_level,
+                        hipStream_t s);
+// mode 0: tracking iteration; mode 1: parity pass (sums only, constants from state).
+// next_level: level of the following iteration (-1: none); break_next: first
+// level below `level` that has iterations (where an rgbOnly early exit goes).
+void launch_step_a(const StepLaunch& L, int level, int iter, int next_level, int break_next,
+                   const TrackParams& P, int mode, uint8_t* dbg_mask, hipStream_t s);
+void launch_step_b(const StepLaunch& L, int level, int iter, int next_level, const TrackParams& P,
+                   int mode, hipStream_t s);
+void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
+void launch_so3_step_dbg(const PairDesc* pairs, const float* consts27, double* out11, uint8_t* mask,
+                         hipStream_t s);
+void launch_copy_u8(const uint8_t* src, uint8_t* dst, size_t n, hipStream_t s);
+void launch_swap_u8(uint8_t* a, uint8_t* b, size_t n, hipStream_t s);
+
+// Host copy of the device exp (same operation sequence): tables and tests.
+float bexpf_host(float x);
+
+}  // namespace slam

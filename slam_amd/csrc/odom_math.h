@@ -1,0 +1,188 @@
+// odom_math.h -- small fixed-size float/double linear algebra used by the
+// device-side solve (the last workgroup of each step) and by the host class.
+// Every routine is written out operation by operation; the library is built
+// with -ffp-contract=off, so host and device evaluate them identically.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+
+namespace slam {
+
+#define ODOM_HD __host__ __device__ __forceinline__
+
+// exp(x) for x <= 0: Cody-Waite reduction by ln2 (high part with 9 trailing
+// zero bits) and a degree-7 Taylor polynomial in Horner form; 0 below -87.
+ODOM_HD float bexpf(float x)
+{
+    if (!(x >= -87.0f)) return 0.0f;
+    if (x > 0.0f) x = 0.0f;
+    const float kf = __builtin_rintf(x * 1.44269504f);
+    float r = x - kf * 0.693145751953125f;
+    r = r - kf * 1.42860677e-06f;
+    float p = 1.98412698e-04f;
+    p = p * r + 1.38888889e-03f;
+    p = p * r + 8.33333333e-03f;
+    p = p * r + 4.16666667e-02f;
+    p = p * r + 1.66666667e-01f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    const int kk = (int)kf;
+    const unsigned int bits = (unsigned int)(kk + 127) << 23;
+    return p * __builtin_bit_cast(float, bits);
+}
+
+// __float2int_rn: round half to even, saturating, NaN -> 0.  The saturation
+// bound only has to exceed any image dimension.
+ODOM_HD int f2i_rn(float f)
+{
+    float r = __builtin_rintf(f);
+    if (!(r == r)) r = 0.0f;
+    if (r > 1.0e9f) r = 1.0e9f;
+    if (r < -1.0e9f) r = -1.0e9f;
+    return (int)r;
+}
+
+ODOM_HD void mm3d(const double* a, const double* b, double* o)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            o[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+}
+
+ODOM_HD void mm3f(const float* a, const float* b, float* o)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            o[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+}
+
+ODOM_HD void mm4d(const double* a, const double* b, double* o)
+{
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            o[i * 4 + j] = a[i * 4] * b[j] + a[i * 4 + 1] * b[4 + j] + a[i * 4 + 2] * b[8 + j] +
+                           a[i * 4 + 3] * b[12 + j];
+}
+
+// 3x3 inverse by cofactors in float (Rprev.inverse(), RGBDOdometryef.cpp:386)
+ODOM_HD void inv3f(const float* m, float* o)
+{
+    const float c0 = m[4] * m[8] - m[5] * m[7];
+    const float c1 = m[5] * m[6] - m[3] * m[8];
+    const float c2 = m[3] * m[7] - m[4] * m[6];
+    const float det = m[0] * c0 + m[1] * c1 + m[2] * c2;
+    const float id = 1.0f / det;
+    o[0] = c0 * id;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    o[3] = c1 * id;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    o[6] = c2 * id;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+
+// K and its closed-form inverse in double, from float level intrinsics
+ODOM_HD void kmat(float fx, float fy, float cx, float cy, double* K, double* Ki)
+{
+    const double dfx = fx, dfy = fy, dcx = cx, dcy = cy;
+    K[0] = dfx; K[1] = 0; K[2] = dcx; K[3] = 0; K[4] = dfy; K[5] = dcy; K[6] = 0; K[7] = 0; K[8] = 1;
+    Ki[0] = 1.0 / dfx; Ki[1] = 0; Ki[2] = -dcx / dfx;
+    Ki[3] = 0; Ki[4] = 1.0 / dfy; Ki[5] = -dcy / dfy;
+    Ki[6] = 0; Ki[7] = 0; Ki[8] = 1;
+}
+
+// Symmetric solve, LDL^T with diagonal pivoting on the largest remaining
+// |diagonal| and a pseudo-inverse of D (|d| <= DBL_MIN gives a zero
+// component): the semantics of Eigen's LDLT::solve used by the reference
+// (RGBDOdometryef.cpp:366, 544).  n <= 6.  Returns the number of zero pivots.
+ODOM_HD int ldlt_solve(int n, const double* Ain, const double* b, double* x)
+{
+    double A[36];
+    int perm[6];
+    double y[6];
+    int zeros = 0;
+    for (int i = 0; i < n * n; ++i) A[i] = Ain[i];
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i * n + i]) > best) { best = fabs(A[i * n + i]); p = i; }
+        if (p != k) {
+            for (int j = 0; j < n; ++j) { double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
+            for (int i = 0; i < n; ++i) { double t = A[i * n + k]; A[i * n + k] = A[i * n + p]; A[i * n + p] = t; }
+            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+        }
+        const double d = A[k * n + k];
+        if (fabs(d) <= DBL_MIN) {
+            for (int i = k + 1; i < n; ++i) A[i * n + k] = 0.0;
+            continue;
+        }
+        for (int i = k + 1; i < n; ++i) A[i * n + k] = A[i * n + k] / d;
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j <= i; ++j) {
+                A[i * n + j] = A[i * n + j] - A[i * n + k] * A[j * n + k] * d;
+                A[j * n + i] = A[i * n + j];
+            }
+    }
+    for (int i = 0; i < n; ++i) y[i] = b[perm[i]];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) y[i] = y[i] - A[i * n + j] * y[j];
+    for (int i = 0; i < n; ++i) {
+        const double d = A[i * n + i];
+        if (fabs(d) > DBL_MIN) y[i] = y[i] / d;
+        else { y[i] = 0.0; ++zeros; }
+    }
+    for (int i = n - 1; i >= 0; --i)
+        for (int j = i + 1; j < n; ++j) y[i] = y[i] - A[j * n + i] * y[j];
+    for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
+    return zeros;
+}
+
+// OdometryProvider::rodrigues, utils.h:16-52
+ODOM_HD void rodrigues(const double* src, double* R)
+{
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    double rx = src[0], ry = src[1], rz = src[2];
+    const double theta = sqrt(rx * rx + ry * ry + rz * rz);
+    if (theta >= DBL_EPSILON) {
+        const double c = cos(theta), s = sin(theta), c1 = 1. - c;
+        const double itheta = theta ? 1. / theta : 0.;
+        rx *= itheta; ry *= itheta; rz *= itheta;
+        const double rrt[9] = {rx * rx, rx * ry, rx * rz, rx * ry, ry * ry, ry * rz, rx * rz, ry * rz, rz * rz};
+        const double rx_[9] = {0, -rz, ry, rz, 0, -rx, -ry, rx, 0};
+        const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        for (int k = 0; k < 9; ++k) R[k] = c * I[k] + c1 * rrt[k] + s * rx_[k];
+    }
+}
+
+// Unpack of the 29 packed sums into symmetric A and b (reduce.cu:475-489)
+ODOM_HD void unpack_se3(const double* in, double* A, double* b)
+{
+    int s = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) {
+            const double v = in[s++];
+            if (j == 6) b[i] = v;
+            else { A[j * 6 + i] = v; A[i * 6 + j] = v; }
+        }
+}
+
+// reduce.cu:1125-1139
+ODOM_HD void unpack_so3(const double* in, double* A, double* b)
+{
+    int s = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 4; ++j) {
+            const double v = in[s++];
+            if (j == 3) b[i] = v;
+            else { A[j * 3 + i] = v; A[i * 3 + j] = v; }
+        }
+}
+
+}  // namespace slam
