@@ -96,6 +96,11 @@ int odom_init_icp_maps(odom_handle h, const odom_frame_view *vertices, const odo
  * 4x4 camera-to-world of the model prediction. */
 int odom_init_icp_model(odom_handle h, const odom_frame_view *vertices, const odom_frame_view *normals,
                         float depth_cutoff, const float pose[16]);
+/* initICPModel with the handle's current pose estimate as modelPose (the
+ * currPose the reference's frame loop passes, elastic_fusion_file.cpp:366-368),
+ * read on the device: lets a sequence be enqueued with no host round trip. */
+int odom_init_icp_model_current(odom_handle h, const odom_frame_view *vertices, const odom_frame_view *normals,
+                                float depth_cutoff);
 /* RGBDOdometryef::initRGB, RGBDOdometryef.cpp:243-247 */
 int odom_init_rgb(odom_handle h, const odom_frame_view *rgb);
 /* RGBDOdometryef::initRGBModel, RGBDOdometryef.cpp:237-241 */
@@ -114,6 +119,10 @@ int odom_track_async(odom_handle h, int rgb_only, float icp_weight, int pyramid,
 int odom_set_pose(odom_handle h, const float trans[3], const float rot[9]);
 int odom_get_pose(odom_handle h, float trans[3], float rot[9]);
 int odom_synchronize(odom_handle h);
+/* Record every call's resulting pose ([R|t], 12 floats row-major) into a
+ * device ring of `capacity` entries, slot = call index % capacity; the call
+ * index restarts at 0.  capacity 0 turns the log off. */
+int odom_set_pose_log(odom_handle h, float *dev_ptr, int capacity);
 
 /* RGBDOdometryef::getCovariance, RGBDOdometryef.cpp:597-600: lastA^-1 */
 int odom_get_covariance(odom_handle h, double cov[36]);
@@ -153,6 +162,13 @@ int odom_read_buffer(odom_handle h, const char *name, int level, void *dst, size
  * association pass, 1 = photometric pass. */
 int odom_set_step_shape(odom_handle h, int kernel, int level, int threads, int pixels_per_thread);
 int odom_get_step_shape(odom_handle h, int kernel, int level, int *threads, int *pixels_per_thread);
+/* One getIncrementalTransformation on the current inputs and pose with a
+ * HIP event pair around every Gauss-Newton step launch on the handle's
+ * stream: mean microseconds of pass A (ICP + photometric association) and
+ * pass B (photometric products) per pyramid level, the number of launches
+ * per level, and the whole call's duration. */
+int odom_profile_track(odom_handle h, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3,
+                       float us_a[3], float us_b[3], int n_a[3], int n_b[3], float *us_total);
 /* Times the supported shapes on the handle's current inputs and keeps the
  * fastest per kernel and level. */
 int odom_autotune(odom_handle h, int rgb_only, float icp_weight);

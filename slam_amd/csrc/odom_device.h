@@ -16,6 +16,8 @@ constexpr int kLevels = 3;
 constexpr int kSE3 = 29;   // 27 upper-triangular products of [J|r], r^2, inliers
 constexpr int kSO3 = 11;   // 9 products of [J|r] (3x3+3), r^2, count
 constexpr int kPartStride = 32;  // floats per block partial record
+constexpr int kMaxBlk = 256;     // block partials per pair, level and pass: the step
+                                 // grids stride over their tiles with at most this many blocks
 
 // Maps of one pyramid level of one pair.  Vertex/normal maps are planar
 // (x plane, y plane, z plane of w*h floats each); a pixel is invalid iff its x
@@ -36,38 +38,60 @@ struct LevelDesc {
     const uint8_t* ilast;            // model (last) intensity
     float minScale;                  // (minGrad / sobelScale)^2
     int pad_;
+    double K[9], Ki[9];              // level intrinsics and closed-form inverse, in double
+};
+
+// State of the Gauss-Newton loop between two iterations.  Two slots
+// alternate (iteration k lives in slot k & 1): the ICP/association pass A_k
+// reads slot k-1 plus the photometric partials of iteration k-1, solves, and
+// its block 0 writes slot k; the photometric pass B_k reads the pose fields of
+// slot k and its block 0 writes the association fields of slot k.  No launch
+// reads a field that a block of the same launch writes.
+struct IterState {
+    // written by A_k
+    double resultRt[16];             // accumulated increment (row-major 4x4)
+    double lastA[36], lastb[6];      // system of the last solve
+    float Rcurr[9], tcurr[3];        // current estimate (float, as the reference)
+    float krkinv[9], kt[3];          // photometric warp constants of iteration k
+    float rgb_err_cmp;               // lastRGBError the rgbOnly test of iteration k compares to
+    int skip;                        // iteration k skipped (its level already stopped)
+    int break_level;                 // level stopped by the rgbOnly test, else -1
+    int degenerate;                  // last solve had zero information
+    int iters_done;
+    // ICP statistics of iteration k-1, written by A_k from the fused
+    // reduction of its prologue (carried otherwise)
+    double icp_sums[kSE3];
+    float res0, res1;                // residual[2] of the ICP step
+    float lastICPError, lastICPCount;
+    // written by B_k (by A_{k+1} when there is no photometric pass)
+    float lastRGBError, lastRGBCount;
+    float sigmaVal;
+    int rgb_count, rgb_sigma;
+    int pending;                     // a solve of iteration k is due
+    int broke;                       // the rgbOnly test fired at iteration k
+    int pad_;
 };
 
 // Device-resident tracking state of one pair (one getIncrementalTransformation
-// call).  Written by single-thread begin/finalize code, read uniformly by the
-// step kernels.
+// call).
 struct TrackState {
-    // pose prior and running estimate (float, as the reference's Eigen state)
     float Rprev[9], tprev[3], Rprev_inv[9];
-    float Rcurr[9], tcurr[3];
     float pose_out[12];              // [R | t] row-major result of the call
-    // accumulated increment (double 4x4 row-major) and SO3 pre-alignment
-    double resultRt[16];
-    double so3R[9];
-    // per-iteration constants of the photometric warp
-    float krkinv[9], kt[3];
-    float sigmaVal;
-    int rgb_count, rgb_sigma;
-    // FP64 totals of the current iteration
-    double icp_sums[kSE3];
-    double rgb_sums[kSE3];
+    double so3R[9];                  // SO3 pre-alignment
+    IterState it[2];
+    // results of the call, copied by the end kernel (RGBDOdometryef.h:62-70)
     double lastA[36], lastb[6];
-    double lastx[6];
-    // reported errors (RGBDOdometryef.h:62-67)
     float lastICPError, lastICPCount, lastRGBError, lastRGBCount, lastSO3Error, lastSO3Count;
-    float res0, res1;                // residual[2] of the ICP step
     int so3_iters;
-    int break_level;                 // rgbOnly early exit: level that stopped, else -1
     int iters_done;
-    int degenerate;                  // last solve had zero information
+    int degenerate;
     int reverted;                    // 0.3 m revert taken
-    unsigned int ticket_a, ticket_b; // last-block-done counters (reset by the last block)
-    int pad_[2];
+    // kernel-level entry points (parity checks)
+    double dbg_sums[kSE3];
+    int dbg_count, dbg_sigma;
+    float* pose_log;                 // optional device ring of [R|t] results
+    int pose_log_cap;
+    int frame_count;                 // track calls completed
 };
 
 // Uniform options of one track call.
@@ -84,9 +108,9 @@ struct TrackParams {
 struct PairDesc {
     LevelDesc lv[kLevels];
     TrackState* st;
-    float* part_a;                   // block partials of the ICP/association pass
-    float* part_b;                   // block partials of the photometric pass
-    unsigned int* part_ai;           // integer partials (count, sum diff^2)
+    float* part_a[2];                // block partials of the ICP/association pass, by iteration parity
+    float* part_b[2];                // block partials of the photometric pass
+    unsigned int* part_ai[2];        // integer partials (count, sum diff^2)
     int nblk_a[kLevels];             // blocks of this pair per level, ICP/association pass
     int blk0_a[kLevels];             // first global block of this pair per level
     int nblk_b[kLevels];             // same for the photometric pass

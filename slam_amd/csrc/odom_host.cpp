@@ -25,7 +25,6 @@ using namespace slam;
 
 namespace {
 
-constexpr int kMinThreads = 128;
 
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -55,7 +54,9 @@ struct odom_ctx {
     uint16_t* depth[kLevels];
     float *vcur[kLevels], *ncur[kLevels], *vprev[kLevels], *nprev[kLevels];
     float *lastDepth[kLevels], *nextDepth[kLevels];
-    uint8_t *lastImage[kLevels], *nextImage[kLevels], *lastNextImage[kLevels];
+    uint8_t* lastImage[kLevels];
+    uint8_t* img[2][kLevels];         // nextImage = img[parity], lastNextImage = img[parity ^ 1]
+    int parity = 0;                   // flipped by the swap of RGBDOdometryef.cpp:585-591
     short2* dxy[kLevels];
     float* zsrc[kLevels];
     float* vz = nullptr;              // z plane of the last vertex input (vmaps_tmp)
@@ -65,15 +66,15 @@ struct odom_ctx {
     float4* stage_n4 = nullptr;
     uint8_t* dbg_mask = nullptr;
     TrackState* st = nullptr;
-    PairDesc* d_pair = nullptr;
-    float* part_a = nullptr;
-    float* part_b = nullptr;
-    unsigned int* part_ai = nullptr;
+    PairDesc* d_pair[2] = {nullptr, nullptr};  // one descriptor per image parity
+    float* part_a[2] = {nullptr, nullptr};
+    float* part_b[2] = {nullptr, nullptr};
+    unsigned int* part_ai[2] = {nullptr, nullptr};
     double* d_out11 = nullptr;
     float* d_c27 = nullptr;
     int maxblk = 0;
 
-    PairDesc h_pair{};
+    PairDesc h_pair[2]{};
     StepShape shape_a[kLevels], shape_b[kLevels];
     TrackState h_st{};
     bool have_state = false;
@@ -105,35 +106,43 @@ int check_launch(odom_ctx* c)
 }
 
 // Rebuild the host descriptor from the buffers and shapes, and upload it.
+uint8_t* next_img(odom_ctx* c, int l) { return c->img[c->parity][l]; }
+uint8_t* last_next_img(odom_ctx* c, int l) { return c->img[c->parity ^ 1][l]; }
+
 int upload_pair(odom_ctx* c)
 {
-    PairDesc& D = c->h_pair;
-    for (int l = 0; l < kLevels; ++l) {
-        LevelDesc& L = D.lv[l];
-        L.w = c->w[l]; L.h = c->h[l];
-        L.fx = c->fx[l]; L.fy = c->fy[l]; L.cx = c->cx[l]; L.cy = c->cy[l];
-        L.invFx = 1.0f / c->fx[l];
-        L.invFy = 1.0f / c->fy[l];
-        L.vcur = c->vcur[l]; L.ncur = c->ncur[l]; L.vprev = c->vprev[l]; L.nprev = c->nprev[l];
-        L.zsrc = c->zsrc[l]; L.dxy = c->dxy[l]; L.inext = c->nextImage[l];
-        L.zlast = c->lastDepth[l]; L.ilast = c->lastImage[l];
-        const float sobelScale = (float)(1.0 / pow(2.0, 3));
-        const double minGrad[kLevels] = {5, 3, 1};
-        L.minScale = (float)(pow(minGrad[l], 2.0) / pow((double)sobelScale, 2.0));
-        const StepShape& sa = c->shape_a[l];
-        const StepShape& sb = c->shape_b[l];
-        D.nblk_a[l] = cdiv(c->n[l], sa.threads * sa.ppt);
-        D.nblk_b[l] = cdiv(c->n[l], sb.threads * sb.ppt);
-        D.blk0_a[l] = 0;
-        D.blk0_b[l] = 0;
+    for (int q = 0; q < 2; ++q) {
+        PairDesc& D = c->h_pair[q];
+        for (int l = 0; l < kLevels; ++l) {
+            LevelDesc& L = D.lv[l];
+            L.w = c->w[l]; L.h = c->h[l];
+            L.fx = c->fx[l]; L.fy = c->fy[l]; L.cx = c->cx[l]; L.cy = c->cy[l];
+            L.invFx = 1.0f / c->fx[l];
+            L.invFy = 1.0f / c->fy[l];
+            L.vcur = c->vcur[l]; L.ncur = c->ncur[l]; L.vprev = c->vprev[l]; L.nprev = c->nprev[l];
+            L.zsrc = c->zsrc[l]; L.dxy = c->dxy[l]; L.inext = c->img[q][l];
+            L.zlast = c->lastDepth[l]; L.ilast = c->lastImage[l];
+            const float sobelScale = (float)(1.0 / pow(2.0, 3));
+            const double minGrad[kLevels] = {5, 3, 1};
+            L.minScale = (float)(pow(minGrad[l], 2.0) / pow((double)sobelScale, 2.0));
+            kmat(L.fx, L.fy, L.cx, L.cy, L.K, L.Ki);
+            const StepShape& sa = c->shape_a[l];
+            const StepShape& sb = c->shape_b[l];
+            D.nblk_a[l] = std::min(kMaxBlk, cdiv(c->n[l], sa.threads * sa.ppt));
+            D.nblk_b[l] = std::min(kMaxBlk, cdiv(c->n[l], sb.threads * sb.ppt));
+            D.blk0_a[l] = 0;
+            D.blk0_b[l] = 0;
+        }
+        D.st = c->st;
+        for (int i = 0; i < 2; ++i) {
+            D.part_a[i] = c->part_a[i];
+            D.part_b[i] = c->part_b[i];
+            D.part_ai[i] = c->part_ai[i];
+        }
+        D.so3_last = c->img[q ^ 1][2];
+        D.so3_next = c->img[q][2];
+        HIPCHK(c, hipMemcpyAsync(c->d_pair[q], &D, sizeof(PairDesc), hipMemcpyHostToDevice, c->stream));
     }
-    D.st = c->st;
-    D.part_a = c->part_a;
-    D.part_b = c->part_b;
-    D.part_ai = c->part_ai;
-    D.so3_last = c->lastNextImage[2];
-    D.so3_next = c->nextImage[2];
-    HIPCHK(c, hipMemcpyAsync(c->d_pair, &D, sizeof(PairDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ODOM_OK;
 }
@@ -169,7 +178,15 @@ int resolve(odom_ctx* c, const odom_frame_view* v, int fmt_a, int fmt_b, void* s
 
 int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
 {
-    for (int l = 1; l < kLevels; ++l) launch_pyrdown_u16(c->depth[l - 1], c->w[l - 1], c->h[l - 1], c->depth[l], c->stream);
+    Pyr12 py{};
+    py.count = 1;
+    py.kind[0] = 2;
+    py.src[0] = c->depth[0];
+    py.dst1[0] = c->depth[1];
+    py.dst2[0] = c->depth[2];
+    py.w0 = c->w[0];
+    py.h0 = c->h[0];
+    launch_pyr12(py, c->stream);
     VNLevels a{};
     for (int l = 0; l < kLevels; ++l) {
         a.depth[l] = c->depth[l];
@@ -187,7 +204,8 @@ int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
     return check_launch(c);
 }
 
-int enqueue_model(odom_ctx* c, const odom_frame_view* v, const odom_frame_view* n, const float* pose, bool model)
+int enqueue_model(odom_ctx* c, const odom_frame_view* v, const odom_frame_view* n, const float* pose, bool model,
+                  bool pose_on_device = false)
 {
     const void *pv, *pn;
     int pitch_v, pitch_n, f;
@@ -208,7 +226,8 @@ int enqueue_model(odom_ctx* c, const odom_frame_view* v, const odom_frame_view* 
     }
     a.vz = c->vz;
     a.transform = model ? 1 : 0;
-    if (model) {
+    a.pose_dev = pose_on_device ? c->st->pose_out : nullptr;
+    if (model && !pose_on_device) {
         // Rcam = modelPose.topLeftCorner(3,3), tcam = topRightCorner (RGBDOdometryef.cpp:194-198)
         const float R[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
         const float t[3] = {pose[3], pose[7], pose[11]};
@@ -226,13 +245,25 @@ int enqueue_populate(odom_ctx* c, const odom_frame_view* rgb, float** dstDepth, 
     int pitch, fmt;
     int rc = resolve(c, rgb, ODOM_FMT_RGB8, ODOM_FMT_RGBA8, c->stage_rgb, &p, &pitch, &fmt);
     if (rc) return rc;
+    launch_populate_l0(c->vz, dstDepth ? dstDepth[0] : nullptr, 6.0f, (const uint8_t*)p, pitch,
+                       fmt == ODOM_FMT_RGB8 ? 3 : 4, dstImage[0], c->w[0], c->h[0], c->stream);
+    Pyr12 py{};
+    py.w0 = c->w[0];
+    py.h0 = c->h[0];
+    py.count = 0;
     if (dstDepth) {
-        launch_vertices_to_depth(c->vz, c->w[0], c->h[0], 6.0f, dstDepth[0], c->stream);
-        for (int l = 0; l + 1 < kLevels; ++l)
-            launch_pyrdown_f32(dstDepth[l], c->w[l], c->h[l], dstDepth[l + 1], c->stream);
+        py.kind[py.count] = 0;
+        py.src[py.count] = dstDepth[0];
+        py.dst1[py.count] = dstDepth[1];
+        py.dst2[py.count] = dstDepth[2];
+        py.count++;
     }
-    launch_intensity((const uint8_t*)p, pitch, fmt == ODOM_FMT_RGB8 ? 3 : 4, c->w[0], c->h[0], dstImage[0], c->stream);
-    for (int l = 0; l + 1 < kLevels; ++l) launch_pyrdown_u8(dstImage[l], c->w[l], c->h[l], dstImage[l + 1], c->stream);
+    py.kind[py.count] = 1;
+    py.src[py.count] = dstImage[0];
+    py.dst1[py.count] = dstImage[1];
+    py.dst2[py.count] = dstImage[2];
+    py.count++;
+    launch_pyr12(py, c->stream);
     return check_launch(c);
 }
 
@@ -257,55 +288,63 @@ TrackParams make_params(odom_ctx* c, int rgbOnly, float icpWeight, int pyramid, 
 StepLaunch step_launch(odom_ctx* c, int kernel, int level)
 {
     StepLaunch L;
-    L.pairs = c->d_pair;
+    L.pairs = c->d_pair[c->parity];
     L.npairs = 1;
     L.blk_pair = nullptr;
     L.shape = kernel == 0 ? c->shape_a[level] : c->shape_b[level];
-    L.total_blocks = kernel == 0 ? c->h_pair.nblk_a[level] : c->h_pair.nblk_b[level];
+    L.total_blocks = kernel == 0 ? c->h_pair[0].nblk_a[level] : c->h_pair[0].nblk_b[level];
     return L;
 }
 
 void prep_levels(odom_ctx* c, PrepLevels& a)
 {
     for (int l = 0; l < kLevels; ++l) {
-        a.img[l] = c->nextImage[l];
+        a.img[l] = next_img(c, l);
         a.depth[l] = c->nextDepth[l];
         a.dxy[l] = c->dxy[l];
         a.zsrc[l] = c->zsrc[l];
         a.w[l] = c->w[l];
         a.h[l] = c->h[l];
-        a.minScale[l] = c->h_pair.lv[l].minScale;
+        a.minScale[l] = c->h_pair[0].lv[l].minScale;
     }
 }
 
 // getIncrementalTransformation (RGBDOdometryef.cpp:267-595) without the
 // host round trips: derivatives, SO3 loop, begin, 2 passes per Gauss-Newton
-// iteration, end, image swap.
-int enqueue_track(odom_ctx* c, const TrackParams& P)
+// iteration (the solve of iteration k runs in the prologue of pass A of
+// iteration k+1), end, image swap.  With `ev`, an event pair brackets every
+// step launch (ev[2*i], ev[2*i+1] for launch i, A and B alternating).
+int enqueue_track(odom_ctx* c, const TrackParams& P, hipEvent_t* ev = nullptr)
 {
     if (P.rgb) {
         PrepLevels a{};
         prep_levels(c, a);
         launch_rgb_prep(a, c->stream);
     }
-    if (P.so3) launch_so3(c->d_pair, 1, 0, c->stream);
+    PairDesc* dp = c->d_pair[c->parity];
+    if (P.so3) launch_so3(dp, 1, c->n[2], c->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
-    const int first = sched.empty() ? -1 : sched[0] / 64;
-    launch_track_begin(c->d_pair, 1, P, first, c->stream);
+    launch_track_begin(dp, 1, P, c->stream);
+    int e = 0;
     for (size_t k = 0; k < sched.size(); ++k) {
         const int level = sched[k] / 64, iter = sched[k] % 64;
-        const int next = (k + 1 < sched.size()) ? sched[k + 1] / 64 : -1;
-        int brk = -1;
-        for (int l = level - 1; l >= 0 && brk < 0; --l)
-            if (P.iters[l] > 0) brk = l;
-        launch_step_a(step_launch(c, 0, level), level, iter, next, brk, P, 0, nullptr, c->stream);
-        if (P.rgb) launch_step_b(step_launch(c, 1, level), level, iter, next, P, 0, c->stream);
+        const int prev = k > 0 ? sched[k - 1] / 64 : -1;
+        if (ev) (void)hipEventRecord(ev[e++], c->stream);
+        launch_step_a(step_launch(c, 0, level), level, iter, (int)k, prev, P, 0, nullptr, c->stream);
+        if (ev) (void)hipEventRecord(ev[e++], c->stream);
+        if (P.rgb) {
+            if (ev) (void)hipEventRecord(ev[e++], c->stream);
+            launch_step_b(step_launch(c, 1, level), level, iter, (int)k, P, 0, c->stream);
+            if (ev) (void)hipEventRecord(ev[e++], c->stream);
+        }
     }
-    launch_track_end(c->d_pair, 1, P, c->stream);
-    if (P.so3)
-        for (int l = 0; l < kLevels; ++l) launch_swap_u8(c->lastNextImage[l], c->nextImage[l], (size_t)c->n[l], c->stream);
+    const int K = (int)sched.size();
+    launch_track_end(dp, 1, P, K, K > 0 ? sched[K - 1] / 64 : 0, c->stream);
+    // std::swap(lastNextImage[i], nextImage[i]) (RGBDOdometryef.cpp:585-591):
+    // later launches take the other descriptor
+    if (P.so3) c->parity ^= 1;
     return check_launch(c);
 }
 
@@ -363,6 +402,7 @@ int odom_create(const odom_params* p, odom_handle* out)
         fprintf(stderr, "slam_odom: device is %s; this library is built for gfx950 only\n", arch);
         return ODOM_ERR_NODEV;
     }
+    init_device_tables();
     odom_ctx* c = new odom_ctx();
     c->prm = *p;
     for (int l = 0; l < kLevels; ++l) {
@@ -375,10 +415,10 @@ int odom_create(const odom_params* p, odom_handle* out)
         c->fy[l] = p->fy / (float)div;
         c->cx[l] = p->cx / (float)div;
         c->cy[l] = p->cy / (float)div;
-        c->shape_a[l] = StepShape{256, 4};
-        c->shape_b[l] = StepShape{256, 4};
+        c->shape_a[l] = StepShape{256, 2};
+        c->shape_b[l] = StepShape{256, 2};
     }
-    c->maxblk = cdiv(c->n[0], kMinThreads);
+    c->maxblk = kMaxBlk;
     // one arena for every device buffer
     const size_t N0 = (size_t)c->n[0];
     struct Req { void** dst; size_t bytes; };
@@ -393,8 +433,8 @@ int odom_create(const odom_params* p, odom_handle* out)
         reqs.push_back({(void**)&c->lastDepth[l], n * 4});
         reqs.push_back({(void**)&c->nextDepth[l], n * 4});
         reqs.push_back({(void**)&c->lastImage[l], n});
-        reqs.push_back({(void**)&c->nextImage[l], n});
-        reqs.push_back({(void**)&c->lastNextImage[l], n});
+        reqs.push_back({(void**)&c->img[0][l], n});
+        reqs.push_back({(void**)&c->img[1][l], n});
         reqs.push_back({(void**)&c->dxy[l], n * 4});
         reqs.push_back({(void**)&c->zsrc[l], n * 4});
     }
@@ -405,10 +445,13 @@ int odom_create(const odom_params* p, odom_handle* out)
     reqs.push_back({(void**)&c->stage_n4, N0 * 16});
     reqs.push_back({(void**)&c->dbg_mask, N0});
     reqs.push_back({(void**)&c->st, sizeof(TrackState)});
-    reqs.push_back({(void**)&c->d_pair, sizeof(PairDesc)});
-    reqs.push_back({(void**)&c->part_a, (size_t)c->maxblk * kPartStride * 4});
-    reqs.push_back({(void**)&c->part_b, (size_t)c->maxblk * kPartStride * 4});
-    reqs.push_back({(void**)&c->part_ai, (size_t)c->maxblk * 2 * 4});
+    reqs.push_back({(void**)&c->d_pair[0], sizeof(PairDesc)});
+    reqs.push_back({(void**)&c->d_pair[1], sizeof(PairDesc)});
+    for (int i = 0; i < 2; ++i) {
+        reqs.push_back({(void**)&c->part_a[i], (size_t)c->maxblk * kPartStride * 4});
+        reqs.push_back({(void**)&c->part_b[i], (size_t)c->maxblk * kPartStride * 4});
+        reqs.push_back({(void**)&c->part_ai[i], (size_t)c->maxblk * 2 * 4});
+    }
     reqs.push_back({(void**)&c->d_out11, 16 * 8});
     reqs.push_back({(void**)&c->d_c27, 32 * 4});
     size_t total = 0;
@@ -437,7 +480,6 @@ int odom_create(const odom_params* p, odom_handle* out)
     s.lastRGBCount = (float)(p->width * p->height);
     s.lastSO3Count = (float)(p->width * p->height);
     s.pose_out[0] = s.pose_out[5] = s.pose_out[10] = 1.0f;
-    s.break_level = -1;
     for (int k = 0; k < 9; ++k) s.so3R[k] = (k % 4 == 0) ? 1.0 : 0.0;
     if (hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         upload_pair(c) != ODOM_OK) {
@@ -518,10 +560,18 @@ int odom_init_icp_model(odom_handle c, const odom_frame_view* v, const odom_fram
     return enqueue_model(c, v, n, pose, true);
 }
 
+int odom_init_icp_model_current(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff)
+{
+    (void)cutoff;
+    if (!c) return ODOM_ERR_ARG;
+    return enqueue_model(c, v, n, nullptr, true, true);
+}
+
 int odom_init_rgb(odom_handle c, const odom_frame_view* rgb)
 {
     if (!c) return ODOM_ERR_ARG;
-    return enqueue_populate(c, rgb, c->nextDepth, c->nextImage);
+    uint8_t* dst[kLevels] = {next_img(c, 0), next_img(c, 1), next_img(c, 2)};
+    return enqueue_populate(c, rgb, c->nextDepth, dst);
 }
 
 int odom_init_rgb_model(odom_handle c, const odom_frame_view* rgb)
@@ -533,7 +583,8 @@ int odom_init_rgb_model(odom_handle c, const odom_frame_view* rgb)
 int odom_init_first_rgb(odom_handle c, const odom_frame_view* rgb)
 {
     if (!c) return ODOM_ERR_ARG;
-    return enqueue_populate(c, rgb, nullptr, c->lastNextImage);
+    uint8_t* dst[kLevels] = {last_next_img(c, 0), last_next_img(c, 1), last_next_img(c, 2)};
+    return enqueue_populate(c, rgb, nullptr, dst);
 }
 
 int odom_set_pose(odom_handle c, const float trans[3], const float rot[9])
@@ -556,6 +607,18 @@ int odom_get_pose(odom_handle c, float trans[3], float rot[9])
         for (int b = 0; b < 3; ++b) rot[a * 3 + b] = c->h_st.pose_out[a * 4 + b];
         trans[a] = c->h_st.pose_out[a * 4 + 3];
     }
+    return ODOM_OK;
+}
+
+int odom_set_pose_log(odom_handle c, float* dev_ptr, int capacity)
+{
+    if (!c || capacity < 0 || (capacity > 0 && !dev_ptr)) return ODOM_ERR_ARG;
+    int rc;
+    const int zero = 0;
+    if ((rc = write_field(c, &c->st->pose_log, &dev_ptr, 1)) || (rc = write_field(c, &c->st->pose_log_cap, &capacity, 1)) ||
+        (rc = write_field(c, &c->st->frame_count, &zero, 1)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ODOM_OK;
 }
 
@@ -653,16 +716,18 @@ int odom_icp_step(odom_handle c, int level, const float Rc[9], const float tc[3]
 {
     if (!c || level < 0 || level >= kLevels || !Rc || !tc || !Rpi || !tp || !out) return ODOM_ERR_ARG;
     int rc;
-    if ((rc = write_field(c, c->st->Rcurr, Rc, 9)) || (rc = write_field(c, c->st->tcurr, tc, 3)) ||
+    IterState* S0 = &c->st->it[0];
+    if ((rc = write_field(c, S0->Rcurr, Rc, 9)) || (rc = write_field(c, S0->tcurr, tc, 3)) ||
         (rc = write_field(c, c->st->Rprev_inv, Rpi, 9)) || (rc = write_field(c, c->st->tprev, tp, 3)))
         return rc;
     TrackParams P = make_params(c, 0, 100.0f, 1, 0, 0);
     P.icp = 1;
     P.rgb = 0;
-    launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    launch_step_a(step_launch(c, 0, level), level, 0, 0, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    launch_reduce_dbg(c->d_pair[c->parity], 0, level, c->stream);
     if ((rc = check_launch(c))) return rc;
     if ((rc = read_state(c))) return rc;
-    memcpy(out, c->h_st.icp_sums, sizeof(double) * kSE3);
+    memcpy(out, c->h_st.dbg_sums, sizeof(double) * kSE3);
     if (mask) {
         HIPCHK(c, hipMemcpy(mask, c->dbg_mask, (size_t)c->n[level], hipMemcpyDeviceToHost));
         for (int i = 0; i < c->n[level]; ++i) mask[i] = mask[i] & 1u;
@@ -684,15 +749,17 @@ int odom_rgb_residual(odom_handle c, int level, const float K[9], const float kt
 {
     if (!c || level < 0 || level >= kLevels || !K || !kt || !count || !sigma) return ODOM_ERR_ARG;
     int rc;
-    if ((rc = write_field(c, c->st->krkinv, K, 9)) || (rc = write_field(c, c->st->kt, kt, 3))) return rc;
+    IterState* S0 = &c->st->it[0];
+    if ((rc = write_field(c, S0->krkinv, K, 9)) || (rc = write_field(c, S0->kt, kt, 3))) return rc;
     TrackParams P = make_params(c, 0, 10.0f, 1, 0, 0);
     P.icp = 0;
     P.rgb = 1;
-    launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    launch_step_a(step_launch(c, 0, level), level, 0, 0, -1, P, 1, mask ? c->dbg_mask : nullptr, c->stream);
+    launch_reduce_dbg(c->d_pair[c->parity], 0, level, c->stream);
     if ((rc = check_launch(c))) return rc;
     if ((rc = read_state(c))) return rc;
-    *count = c->h_st.rgb_count;
-    *sigma = c->h_st.rgb_sigma;
+    *count = c->h_st.dbg_count;
+    *sigma = c->h_st.dbg_sigma;
     if (mask) {
         HIPCHK(c, hipMemcpy(mask, c->dbg_mask, (size_t)c->n[level], hipMemcpyDeviceToHost));
         for (int i = 0; i < c->n[level]; ++i) mask[i] = (mask[i] >> 1) & 1u;
@@ -704,16 +771,18 @@ int odom_rgb_step(odom_handle c, int level, float sigma, const float K[9], const
 {
     if (!c || level < 0 || level >= kLevels || !K || !kt || !out) return ODOM_ERR_ARG;
     int rc;
-    if ((rc = write_field(c, c->st->krkinv, K, 9)) || (rc = write_field(c, c->st->kt, kt, 3)) ||
-        (rc = write_field(c, &c->st->sigmaVal, &sigma, 1)))
+    IterState* S0 = &c->st->it[0];
+    if ((rc = write_field(c, S0->krkinv, K, 9)) || (rc = write_field(c, S0->kt, kt, 3)) ||
+        (rc = write_field(c, &S0->sigmaVal, &sigma, 1)))
         return rc;
     TrackParams P = make_params(c, 0, 10.0f, 1, 0, 0);
     P.icp = 0;
     P.rgb = 1;
-    launch_step_b(step_launch(c, 1, level), level, 0, -1, P, 1, c->stream);
+    launch_step_b(step_launch(c, 1, level), level, 0, 0, P, 1, c->stream);
+    launch_reduce_dbg(c->d_pair[c->parity], 1, level, c->stream);
     if ((rc = check_launch(c))) return rc;
     if ((rc = read_state(c))) return rc;
-    memcpy(out, c->h_st.rgb_sums, sizeof(double) * kSE3);
+    memcpy(out, c->h_st.dbg_sums, sizeof(double) * kSE3);
     return ODOM_OK;
 }
 
@@ -726,7 +795,7 @@ int odom_so3_step(odom_handle c, const float H[9], const float Ki[9], const floa
     memcpy(c27 + 9, Ki, 36);
     memcpy(c27 + 18, KR, 36);
     HIPCHK(c, hipMemcpyAsync(c->d_c27, c27, sizeof c27, hipMemcpyHostToDevice, c->stream));
-    launch_so3_step_dbg(c->d_pair, c->d_c27, c->d_out11, mask ? c->dbg_mask : nullptr, c->stream);
+    launch_so3_step_dbg(c->d_pair[c->parity], c->d_c27, c->d_out11, mask ? c->dbg_mask : nullptr, c->stream);
     int rc = check_launch(c);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->d_out11, sizeof(double) * kSO3, hipMemcpyDeviceToHost, c->stream));
@@ -750,8 +819,8 @@ int odom_read_buffer(odom_handle c, const char* name, int level, void* dst, size
     else if (s == "last_depth") { src = c->lastDepth[level]; need = n * 4; }
     else if (s == "next_depth") { src = c->nextDepth[level]; need = n * 4; }
     else if (s == "last_image") { src = c->lastImage[level]; need = n; }
-    else if (s == "next_image") { src = c->nextImage[level]; need = n; }
-    else if (s == "last_next_image") { src = c->lastNextImage[level]; need = n; }
+    else if (s == "next_image") { src = next_img(c, level); need = n; }
+    else if (s == "last_next_image") { src = last_next_img(c, level); need = n; }
     else if (s == "dxy") { src = c->dxy[level]; need = n * 4; }
     else if (s == "zsrc") { src = c->zsrc[level]; need = n * 4; }
     else return ODOM_ERR_ARG;
@@ -780,12 +849,60 @@ int odom_get_step_shape(odom_handle c, int kernel, int level, int* threads, int*
     return ODOM_OK;
 }
 
+int odom_profile_track(odom_handle c, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3,
+                       float us_a[3], float us_b[3], int n_a[3], int n_b[3], float* us_total)
+{
+    if (!c || !us_a || !us_b || !n_a || !n_b || !us_total) return ODOM_ERR_ARG;
+    const TrackParams P = make_params(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    std::vector<int> sched;
+    for (int l = kLevels - 1; l >= 0; --l)
+        for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
+    const int nev = (int)sched.size() * (P.rgb ? 4 : 2);
+    std::vector<hipEvent_t> ev(nev);
+    hipEvent_t t0, t1;
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    HIPCHK(c, hipEventCreate(&t0));
+    HIPCHK(c, hipEventCreate(&t1));
+    HIPCHK(c, hipEventRecord(t0, c->stream));
+    int rc = enqueue_track(c, P, ev.data());
+    HIPCHK(c, hipEventRecord(t1, c->stream));
+    HIPCHK(c, hipEventSynchronize(t1));
+    for (int l = 0; l < kLevels; ++l) { us_a[l] = 0; us_b[l] = 0; n_a[l] = 0; n_b[l] = 0; }
+    int e = 0;
+    for (size_t k = 0; k < sched.size() && rc == ODOM_OK; ++k) {
+        const int l = sched[k];
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[e], ev[e + 1]));
+        us_a[l] += ms * 1000.0f;
+        n_a[l]++;
+        e += 2;
+        if (P.rgb) {
+            HIPCHK(c, hipEventElapsedTime(&ms, ev[e], ev[e + 1]));
+            us_b[l] += ms * 1000.0f;
+            n_b[l]++;
+            e += 2;
+        }
+    }
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+    *us_total = ms * 1000.0f;
+    for (int l = 0; l < kLevels; ++l) {
+        if (n_a[l]) us_a[l] /= (float)n_a[l];
+        if (n_b[l]) us_b[l] /= (float)n_b[l];
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
+}
+
 int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
 {
     if (!c) return ODOM_ERR_ARG;
     TrackParams P = make_params(c, rgbOnly, icpWeight, 1, 0, 0);
-    const int threads[] = {128, 256, 512};
-    const int ppts[] = {1, 2, 4, 8};
+    const int threads[] = {256, 512};
+    const int ppts[] = {1, 2, 4};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -802,8 +919,8 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
                     const int reps = 20;
                     for (int r = 0; r < reps + 3; ++r) {
                         if (r == 3) HIPCHK(c, hipEventRecord(e0, c->stream));
-                        if (kernel == 0) launch_step_a(step_launch(c, 0, level), level, 0, -1, -1, P, 1, nullptr, c->stream);
-                        else launch_step_b(step_launch(c, 1, level), level, 0, -1, P, 1, c->stream);
+                        if (kernel == 0) launch_step_a(step_launch(c, 0, level), level, 0, 0, -1, P, 1, nullptr, c->stream);
+                        else launch_step_b(step_launch(c, 1, level), level, 0, 0, P, 1, c->stream);
                     }
                     HIPCHK(c, hipEventRecord(e1, c->stream));
                     HIPCHK(c, hipEventSynchronize(e1));

@@ -45,6 +45,7 @@ struct ModelLevels {
     float* vz;                      // w0*h0 z values (may be null)
     int transform;
     float R[9], t[3];
+    const float* pose_dev;          // if set: [R|t] (3x4 row-major) read on the device
 };
 void launch_model_maps(const ModelLevels& a, hipStream_t s);
 void launch_vertices_to_depth(const float* vz, int w, int h, float cutoff, float* depth, hipStream_t s);
@@ -52,6 +53,21 @@ void launch_pyrdown_f32(const float* src, int w, int h, float* dst, hipStream_t 
 void launch_intensity(const uint8_t* rgb, int pitch_bytes, int channels, int w, int h, uint8_t* dst,
                       hipStream_t s);
 void launch_pyrdown_u8(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
+// Levels 1 and 2 of up to two pyramids in one launch.  kind: 0 = float
+// Gaussian (a9), 1 = uint8 Gaussian (a11), 2 = uint16 bilateral-gated (a2).
+struct Pyr12 {
+    int kind[2];
+    const void* src[2];
+    void* dst1[2];
+    void* dst2[2];
+    int w0, h0;
+    int count;
+};
+void launch_pyr12(const Pyr12& a, hipStream_t s);
+// level 0 of populateRGBDData: depth from the vertex z plane (skipped when
+// depth0 is null) and intensity
+void launch_populate_l0(const float* vz, float* depth0, float cutoff, const uint8_t* rgb, int pitch, int ch,
+                        uint8_t* img0, int w, int h, hipStream_t s);
 // derivatives (a12) + static photometric source mask, all levels of all pairs
 struct PrepLevels {
     const uint8_t* img[kLevels];
@@ -71,21 +87,27 @@ struct StepLaunch {
     int total_blocks;               // blocks of this level over all pairs
     StepShape shape;
 };
-void launch_so3(const PairDesc* pairs, int npairs, int mode, hipStream_t s);
-void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, int first_level,
-                        hipStream_t s);
-// mode 0: tracking iteration; mode 1: parity pass (sums only, constants from state).
-// next_level: level of the following iteration (-1: none); break_next: first
-// level below `level` that has iterations (where an rgbOnly early exit goes).
-void launch_step_a(const StepLaunch& L, int level, int iter, int next_level, int break_next,
-                   const TrackParams& P, int mode, uint8_t* dbg_mask, hipStream_t s);
-void launch_step_b(const StepLaunch& L, int level, int iter, int next_level, const TrackParams& P,
-                   int mode, hipStream_t s);
-void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
+// SO3 pre-alignment; max_px_l2: largest level-2 pixel count over the pairs
+void launch_so3(const PairDesc* pairs, int npairs, int max_px_l2, hipStream_t s);
+void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
+// Gauss-Newton iteration k (global index over the call) at (level, iter).
+// mode 0: tracking (the prologue finishes iteration k-1, which ran at
+// prev_level); mode 1: parity pass with the constants of state slot 0,
+// partials into parity 0 (totals by launch_reduce_dbg).
+void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_level, const TrackParams& P, int mode,
+                   uint8_t* dbg_mask, hipStream_t s);
+void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s);
+// finishes the last iteration (K-1, at last_level), revert, result
+void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s);
+// FP64 totals of a parity pass: which 0 = pass A (ICP sums, count, sigma), 1 = pass B
+void launch_reduce_dbg(const PairDesc* pairs, int which, int level, hipStream_t s);
 void launch_so3_step_dbg(const PairDesc* pairs, const float* consts27, double* out11, uint8_t* mask,
                          hipStream_t s);
 void launch_copy_u8(const uint8_t* src, uint8_t* dst, size_t n, hipStream_t s);
 void launch_swap_u8(uint8_t* a, uint8_t* b, size_t n, hipStream_t s);
+
+// Constant tables of the kernels (the bilateral's spatial factors), per device.
+void init_device_tables();
 
 // Host copy of the device exp (same operation sequence): tables and tests.
 float bexpf_host(float x);

@@ -27,67 +27,96 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 float bexpf_host(float x) { return bexpf(x); }
 
 // =============================================================== a1 bilateral
-// depth_bilateral.frag:30-76.  Output tile 32x32 per 256-thread block (4
-// pixels per thread), input tile with a 6-pixel halo in LDS.  The weight is
+// depth_bilateral.frag:30-76.  Output tile 32x8 per 256-thread block (one
+// pixel per thread), input tile with a 6-pixel halo in LDS.  The weight is
 // the product of a 13x13 spatial table and a range table over |value - tmp|
 // (factors of exp(-(space2*0.024691358 + color2*0.000555556))), both built
 // per block with bexpf.  Range factors for |value - tmp| >= 380 are 0.
 constexpr int kBilatRange = 380;
-constexpr int kBT = 32;                 // output tile edge
-constexpr int kBL = kBT + 12;           // LDS tile edge
+constexpr int kBT = 32;                 // output tile width
+constexpr int kBTY = 8;                 // output tile height (one pixel per thread)
+constexpr int kBL = kBT + 12;           // LDS tile width
+constexpr int kBLY = kBTY + 12;         // LDS tile height
+
+// The 13x13 spatial factors, set from the host copy of bexpf (the same
+// operation sequence) so the unrolled interior path reads them as scalars.
+__constant__ float c_bilat_es[169];
+static bool g_bilat_es_set[64];
+
+void init_device_tables()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_bilat_es_set[dev]) return;
+    float es[169];
+    for (int i = 0; i < 169; ++i) {
+        const int dy = i / 13 - 6, dx = i % 13 - 6;
+        es[i] = bexpf(-((float)(dx * dx + dy * dy) * 0.024691358f));
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_bilat_es), es, sizeof es) == hipSuccess) g_bilat_es_set[dev] = true;
+}
 
 __global__ __launch_bounds__(256) void k_bilateral(const uint16_t* __restrict__ src, int pitch,
                                                    uint16_t* __restrict__ dst, int w, int h,
                                                    unsigned int maxv)
 {
-    __shared__ uint16_t tile[kBL * kBL];
-    __shared__ float sES[169];
+    __shared__ uint16_t tile[kBLY * kBL];
     __shared__ float sEC[kBilatRange + 1];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * kBT, y0 = blockIdx.y * kBT;
-    for (int i = tid; i < kBL * kBL; i += 256) {
+    const int x0 = blockIdx.x * kBT, y0 = blockIdx.y * kBTY;
+    for (int i = tid; i < kBLY * kBL; i += 256) {
         const int gx = x0 - 6 + i % kBL, gy = y0 - 6 + i / kBL;
         tile[i] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? src[(size_t)gy * pitch + gx] : (uint16_t)0;
-    }
-    for (int i = tid; i < 169; i += 256) {
-        const int dy = i / 13 - 6, dx = i % 13 - 6;
-        sES[i] = bexpf(-((float)(dx * dx + dy * dy) * 0.024691358f));
     }
     for (int i = tid; i <= kBilatRange; i += 256)
         sEC[i] = (i < kBilatRange) ? bexpf(-((float)(i * i) * 0.000555556f)) : 0.0f;
     __syncthreads();
-    const int lx = tid & 31;
-    for (int r = 0; r < 4; ++r) {
-        const int ly = (tid >> 5) + 8 * r;
-        const int x = x0 + lx, y = y0 + ly;
-        if (x >= w || y >= h) continue;
-        const int value = tile[(ly + 6) * kBL + lx + 6];
-        if ((unsigned int)value > maxv || value < 300) {
-            dst[(size_t)y * w + x] = 0;
-            continue;
+    const int lx = tid & 31, ly = tid >> 5;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const int value = tile[(ly + 6) * kBL + lx + 6];
+    if ((unsigned int)value > maxv || value < 300) {
+        dst[(size_t)y * w + x] = 0;
+        return;
+    }
+    float sum1 = 0.0f, sum2 = 0.0f;
+    if (x0 >= 6 && y0 >= 6 && x0 + kBT + 6 <= w && y0 + kBTY + 6 <= h) {
+        // interior tile: no window clipping, 13 columns unrolled per row
+#pragma unroll 1
+        for (int dy = 0; dy < 13; ++dy) {
+            const int trow = (ly + dy) * kBL + lx;
+            const float* es = c_bilat_es + dy * 13;
+#pragma unroll
+            for (int dx = 0; dx < 13; ++dx) {
+                const int t = tile[trow + dx];
+                const int a = min(abs(value - t), kBilatRange);
+                const float wgt = es[dx] * sEC[a];
+                sum1 = sum1 + (float)t * wgt;
+                sum2 = sum2 + wgt;
+            }
         }
+    } else {
         const int cy0 = max(y - 6, 0), cy1 = min(y + 7, h);
         const int cx0 = max(x - 6, 0), cx1 = min(x + 7, w);
-        float sum1 = 0.0f, sum2 = 0.0f;
         for (int cy = cy0; cy < cy1; ++cy) {
             const int trow = (cy - y0 + 6) * kBL - x0 + 6;
             const int erow = (cy - y + 6) * 13 + 6 - x;
             for (int cx = cx0; cx < cx1; ++cx) {
                 const int t = tile[trow + cx];
                 const int a = min(abs(value - t), kBilatRange);
-                const float wgt = sES[erow + cx] * sEC[a];
+                const float wgt = c_bilat_es[erow + cx] * sEC[a];
                 sum1 = sum1 + (float)t * wgt;
                 sum2 = sum2 + wgt;
             }
         }
-        dst[(size_t)y * w + x] = (uint16_t)(unsigned int)roundf(sum1 / sum2);
     }
+    dst[(size_t)y * w + x] = (uint16_t)(unsigned int)roundf(sum1 / sum2);
 }
 
 void launch_bilateral(const uint16_t* src, int pitch, uint16_t* dst, int w, int h, float maxD, hipStream_t s)
 {
     const unsigned int maxv = (unsigned int)(maxD * 1000.0f);
-    dim3 grid(cdiv(w, kBT), cdiv(h, kBT));
+    init_device_tables();
+    dim3 grid(cdiv(w, kBT), cdiv(h, kBTY));
     hipLaunchKernelGGL(k_bilateral, grid, dim3(256), 0, s, src, pitch, dst, w, h, maxv);
 }
 
@@ -268,10 +297,19 @@ __global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, Seg3 g)
     Cam6 c = (l == 0) ? model_l0(a, x, y) : (l == 1 ? model_l1(a, x, y) : model_l2(a, x, y));
     if (l == 0 && a.vz) a.vz[i] = a.v4[(size_t)y * a.v_pitch + x].z;
     if (a.transform) {
-        const float* R = a.R;
+        float R[9], t[3];
+        if (a.pose_dev) {
+            for (int r = 0; r < 3; ++r) {
+                for (int k = 0; k < 3; ++k) R[3 * r + k] = a.pose_dev[4 * r + k];
+                t[r] = a.pose_dev[4 * r + 3];
+            }
+        } else {
+            for (int k = 0; k < 9; ++k) R[k] = a.R[k];
+            for (int k = 0; k < 3; ++k) t[k] = a.t[k];
+        }
         if (!isnan(c.v[0])) {
             const float v0 = c.v[0], v1 = c.v[1], v2 = c.v[2];
-            for (int r = 0; r < 3; ++r) c.v[r] = R[3 * r] * v0 + R[3 * r + 1] * v1 + R[3 * r + 2] * v2 + a.t[r];
+            for (int r = 0; r < 3; ++r) c.v[r] = R[3 * r] * v0 + R[3 * r + 1] * v1 + R[3 * r + 2] * v2 + t[r];
         } else {
             c.v[0] = c.v[1] = c.v[2] = QNAN;
         }
@@ -317,8 +355,12 @@ void launch_vertices_to_depth(const float* vz, int w, int h, float cutoff, float
 // ============================================== a9 + a11 Gaussian pyrDown
 // utils.cu:332-363 and 470-500: loop bound min(2x+3, cols-1) and the
 // reversed weight index (ty-cy-1)*5+(tx-cx-1) are kept.
-__constant__ float c_gauss25[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6,
-                                    4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
+// The 5x5 kernel {1 4 6 4 1} x {1 4 6 4 1} of utils.cu:453-457, entry
+// (r, c) computed as an exact product instead of a per-lane table lookup.
+__device__ __forceinline__ float gauss5(int k)
+{
+    return k == 2 ? 6.0f : ((k & 1) ? 4.0f : 1.0f);
+}
 
 __global__ __launch_bounds__(256) void k_pyrdown_f32(const float* __restrict__ src, int sw, int sh,
                                                      float* __restrict__ dst, int dw, int dh)
@@ -333,7 +375,7 @@ __global__ __launch_bounds__(256) void k_pyrdown_f32(const float* __restrict__ s
         for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
             const float v = src[(size_t)cy * sw + cx];
             if (!isnan(v)) {
-                const float gk = c_gauss25[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
                 sum = sum + v * gk;
                 count = (int)((float)count + gk);
             }
@@ -360,7 +402,7 @@ __global__ __launch_bounds__(256) void k_pyrdown_u8(const uint8_t* __restrict__ 
         for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
             const uint8_t v = src[(size_t)cy * sw + cx];
             if (v > 0) {
-                const float gk = c_gauss25[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
                 sum = sum + (float)v * gk;
                 count = (int)((float)count + gk);
             }
@@ -393,6 +435,165 @@ void launch_intensity(const uint8_t* rgb, int pitch, int ch, int w, int h, uint8
     hipLaunchKernelGGL(k_intensity, dim3(cdiv(w, 64), cdiv(h, 4)), dim3(256), 0, s, rgb, pitch, ch, w, h, dst);
 }
 
+// ===================================================== pyramid launches
+// One pyramid level of up to two pyramids per launch.
+template <class Get>
+__device__ __forceinline__ float gauss_f32_at(Get get, int sw, int sh, int x, int y)
+{
+    const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
+    float sum = 0.0f;
+    int count = 0;
+    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
+        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+            const float v = get(cx, cy);
+            if (!isnan(v)) {
+                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
+                sum = sum + v * gk;
+                count = (int)((float)count + gk);
+            }
+        }
+    return sum / (float)count;
+}
+
+template <class Get>
+__device__ __forceinline__ uint8_t gauss_u8_at(Get get, int sw, int sh, int x, int y)
+{
+    const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
+    float sum = 0.0f;
+    int count = 0;
+    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
+        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+            const uint8_t v = get(cx, cy);
+            if (v > 0) {
+                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
+                sum = sum + (float)v * gk;
+                count = (int)((float)count + gk);
+            }
+        }
+    const float q = sum / (float)count;
+    return (count == 0) ? (uint8_t)0 : (uint8_t)q;
+}
+
+// utils.cu:57-94 (sigma_color 30)
+template <class Get>
+__device__ __forceinline__ uint16_t pyr_u16_at(Get get, int sw, int sh, int x, int y)
+{
+    const float wts[3] = {0.375f, 0.25f, 0.0625f};
+    const int center = get(2 * x, 2 * y);
+    const int x_mi = max(0, 2 * x - 2) - 2 * x, y_mi = max(0, 2 * y - 2) - 2 * y;
+    const int x_ma = min(sw, 2 * x + 3) - 2 * x, y_ma = min(sh, 2 * y + 3) - 2 * y;
+    float sum = 0.0f, wall = 0.0f;
+    for (int yi = y_mi; yi < y_ma; ++yi)
+        for (int xi = x_mi; xi < x_ma; ++xi) {
+            const int val = get(2 * x + xi, 2 * y + yi);
+            if ((float)abs(val - center) < 90.0f) {
+                sum = sum + (float)val * wts[abs(xi)] * wts[abs(yi)];
+                wall = wall + wts[abs(xi)] * wts[abs(yi)];
+            }
+        }
+    return (uint16_t)(int)(sum / wall);
+}
+
+// Segments of a one-dimensional grid: blocks [b0[s], b0[s+1]) run segment s.
+struct SegN {
+    int b0[9];
+    int n;
+};
+
+__device__ __forceinline__ int seg_index(const SegN& g, int b)
+{
+    int s = 0;
+    while (s + 1 < g.n && b >= g.b0[s + 1]) ++s;
+    return s;
+}
+
+static SegN make_segn(const int* n, int count, int per_block)
+{
+    SegN g;
+    g.n = count;
+    g.b0[0] = 0;
+    for (int s = 0; s < count; ++s) g.b0[s + 1] = g.b0[s] + cdiv(n[s], per_block);
+    return g;
+}
+
+__global__ __launch_bounds__(256) void k_pyr_level(Pyr12 a, SegN g)
+{
+    const int pyr = seg_index(g, blockIdx.x);
+    const int i = (blockIdx.x - g.b0[pyr]) * 256 + threadIdx.x;
+    const int sw = a.w0, sh = a.h0, dw = sw / 2, dh = sh / 2;
+    if (i >= dw * dh) return;
+    const int y = i / dw, x = i - y * dw;
+    const int kind = a.kind[pyr];
+    if (kind == 0) {
+        const float* s0 = (const float*)a.src[pyr];
+        auto g0 = [&](int cx, int cy) { return s0[(size_t)cy * sw + cx]; };
+        ((float*)a.dst1[pyr])[i] = gauss_f32_at(g0, sw, sh, x, y);
+    } else if (kind == 1) {
+        const uint8_t* s0 = (const uint8_t*)a.src[pyr];
+        auto g0 = [&](int cx, int cy) { return s0[(size_t)cy * sw + cx]; };
+        ((uint8_t*)a.dst1[pyr])[i] = gauss_u8_at(g0, sw, sh, x, y);
+    } else {
+        const uint16_t* s0 = (const uint16_t*)a.src[pyr];
+        auto g0 = [&](int cx, int cy) { return (int)s0[(size_t)cy * sw + cx]; };
+        ((uint16_t*)a.dst1[pyr])[i] = pyr_u16_at(g0, sw, sh, x, y);
+    }
+}
+
+void launch_pyr12(const Pyr12& a, hipStream_t s)
+{
+    // level 1 from level 0, then level 2 from level 1, each for every pyramid
+    for (int lvl = 1; lvl <= 2; ++lvl) {
+        Pyr12 b = a;
+        b.w0 = a.w0 >> (lvl - 1);
+        b.h0 = a.h0 >> (lvl - 1);
+        for (int p = 0; p < a.count; ++p) {
+            b.src[p] = lvl == 1 ? a.src[p] : a.dst1[p];
+            b.dst1[p] = lvl == 1 ? a.dst1[p] : a.dst2[p];
+        }
+        int n[2];
+        for (int p = 0; p < a.count; ++p) n[p] = (b.w0 / 2) * (b.h0 / 2);
+        const SegN g = make_segn(n, a.count, 256);
+        hipLaunchKernelGGL(k_pyr_level, dim3(g.b0[g.n]), dim3(256), 0, s, b, g);
+    }
+}
+
+// Level 0 of populateRGBDData (RGBDOdometryef.cpp:212, 225): verticesToDepth
+// (utils.cu:526-537) and the intensity conversion (utils.cu:550-563) in one launch.
+__global__ __launch_bounds__(256) void k_populate_l0(const float* __restrict__ vz, float* __restrict__ depth0,
+                                                     float cutoff, const uint8_t* __restrict__ rgb, int pitch,
+                                                     int ch, uint8_t* __restrict__ img0, int w, int h, SegN g)
+{
+    const int sgi = seg_index(g, blockIdx.x);
+    const int i = (blockIdx.x - g.b0[sgi]) * 256 + threadIdx.x;
+    if (i >= w * h) return;
+    if (sgi == 0 && depth0) {
+        const float z = vz[i];
+        depth0[i] = (z > cutoff || z <= 0) ? QNAN : z;
+    } else {
+        const int y = i / w, x = i - y * w;
+        const uint8_t* p = rgb + (size_t)y * pitch + (size_t)x * ch;
+        const int value = (int)((float)p[0] * 0.114f + (float)p[1] * 0.299f + (float)p[2] * 0.587f);
+        img0[i] = (uint8_t)value;
+    }
+}
+
+void launch_populate_l0(const float* vz, float* depth0, float cutoff, const uint8_t* rgb, int pitch, int ch,
+                        uint8_t* img0, int w, int h, hipStream_t s)
+{
+    const int n[2] = {w * h, w * h};
+    SegN g;
+    if (depth0) {
+        g = make_segn(n, 2, 256);
+    } else {
+        // intensity only: a single segment indexed as segment 1
+        g.n = 2;
+        g.b0[0] = 0;
+        g.b0[1] = 0;
+        g.b0[2] = cdiv(w * h, 256);
+    }
+    hipLaunchKernelGGL(k_populate_l0, dim3(g.b0[2]), dim3(256), 0, s, vz, depth0, cutoff, rgb, pitch, ch, img0, w, h, g);
+}
+
 // ================================ a12 derivatives + photometric source mask
 // computeDerivativeImages (utils.cu:582-606: kernelIndex counts down over the
 // taps visited, so border pixels shift the kernel) and the pixel tests of
@@ -414,14 +615,32 @@ __global__ __launch_bounds__(256) void k_rgb_prep(PrepLevels a, Seg3 g)
     const int y = k / w, x = k - y * w;
     const uint8_t* img = a.img[l];
     float dxv = 0.0f, dyv = 0.0f;
-    int ki = 8;
-    for (int j = max(y - 1, 0); j <= min(y + 1, h - 1); j++)
-        for (int i = max(x - 1, 0); i <= min(x + 1, w - 1); i++) {
-            const float sv = (float)img[(size_t)j * w + i];
-            dxv = dxv + sv * c_gsx[ki];
-            dyv = dyv + sv * c_gsy[ki];
-            --ki;
-        }
+    if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
+        // interior: the nine taps in the same order with fixed weights
+        const float kx[9] = {0.52201f, 0.00000f, -0.52201f, 0.79451f, -0.00000f, -0.79451f,
+                             0.52201f, 0.00000f, -0.52201f};
+        const float ky[9] = {0.52201f, 0.79451f, 0.52201f, 0.00000f, 0.00000f, 0.00000f,
+                             -0.52201f, -0.79451f, -0.52201f};
+        int ki = 8;
+#pragma unroll
+        for (int j = -1; j <= 1; j++)
+#pragma unroll
+            for (int i = -1; i <= 1; i++) {
+                const float sv = (float)img[(size_t)(y + j) * w + x + i];
+                dxv = dxv + sv * kx[ki];
+                dyv = dyv + sv * ky[ki];
+                --ki;
+            }
+    } else {
+        int ki = 8;
+        for (int j = max(y - 1, 0); j <= min(y + 1, h - 1); j++)
+            for (int i = max(x - 1, 0); i <= min(x + 1, w - 1); i++) {
+                const float sv = (float)img[(size_t)j * w + i];
+                dxv = dxv + sv * c_gsx[ki];
+                dyv = dyv + sv * c_gsy[ki];
+                --ki;
+            }
+    }
     const short sx = (short)dxv, sy = (short)dyv;
     a.dxy[l][k] = make_short2(sx, sy);
     float z = QNAN;
@@ -478,29 +697,47 @@ void launch_swap_u8(uint8_t* a, uint8_t* b, size_t n, hipStream_t s)
 }
 
 // ============================================================ reductions
+// Wave64 sums by DPP: row_shr 1/2/4/8 (inclusive scans of the 16-lane rows)
+// then row_bcast 15/31 carry the rows forward; lane 63 ends with the total.
+// The order of additions is fixed, so the results are reproducible.
+#define ODOM_DPP(x, ctrl, rmask) __builtin_amdgcn_update_dpp(0, (x), (ctrl), (rmask), 0xf, false)
+
+__device__ __forceinline__ float wave_total_f(float v)
+{
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x111, 0xf));
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x112, 0xf));
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x114, 0xf));
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x118, 0xf));
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x142, 0xa));
+    v = v + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, v), 0x143, 0xc));
+    return v;
+}
+
+__device__ __forceinline__ unsigned int wave_total_u(unsigned int v)
+{
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x111, 0xf);
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x112, 0xf);
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x114, 0xf);
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x118, 0xf);
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x142, 0xa);
+    v = v + (unsigned int)ODOM_DPP((int)v, 0x143, 0xc);
+    return v;
+}
+
 template <int V>
 __device__ __forceinline__ void wave_sum(float (&a)[V])
 {
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-        float v = a[k];
-        v += __shfl_xor(v, 32);
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 8);
-        v += __shfl_xor(v, 4);
-        v += __shfl_xor(v, 2);
-        v += __shfl_xor(v, 1);
-        a[k] = v;
-    }
+    for (int k = 0; k < V; ++k) a[k] = wave_total_f(a[k]);
 }
 
-// Block partial of V floats: wave sums, then waves in order.  Deterministic.
+// Block partial of V floats: wave totals (lane 63), then the waves in order.
 template <int V, int NT>
 __device__ __forceinline__ void block_partial(float (&a)[V], float* __restrict__ out, float (*red)[V])
 {
     wave_sum<V>(a);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
+    if (lane == 63)
         for (int k = 0; k < V; ++k) red[wid][k] = a[k];
     __syncthreads();
     if (threadIdx.x < V) {
@@ -510,21 +747,58 @@ __device__ __forceinline__ void block_partial(float (&a)[V], float* __restrict__
     }
 }
 
-// FP64 total over nblk block partials (stride kPartStride), fixed order.
-template <int V, int NT>
-__device__ __forceinline__ void final_sum(const float* __restrict__ part, int nblk, double* tot,
-                                          double (*red)[NT / 32])
+// Integer block partial (count, wrapping int32 sum of diff^2).
+template <int NT>
+__device__ __forceinline__ void block_partial_int(unsigned int c0, unsigned int c1, unsigned int* __restrict__ out,
+                                                  unsigned int (*red)[2])
 {
-    constexpr int G = NT / 32;
-    const int t = threadIdx.x;
-    if (t < V * G) {
-        const int k = t / G, gi = t - k * G;
-        double s = 0.0;
-        for (int b = gi; b < nblk; b += G) s += (double)part[(size_t)b * kPartStride + k];
-        red[k][gi] = s;
+    c0 = wave_total_u(c0);
+    c1 = wave_total_u(c1);
+    if ((threadIdx.x & 63) == 63) {
+        red[threadIdx.x >> 6][0] = c0;
+        red[threadIdx.x >> 6][1] = c1;
     }
     __syncthreads();
-    if (t < V) {
+    if (threadIdx.x < 2) {
+        unsigned int s = 0;
+        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// FP64 totals of the kSE3 columns of up to two partial buffers (pa over na
+// blocks -> tot[0, kSE3), pb over nb blocks -> tot[kSE3, 2 kSE3)), in a fixed
+// order; a null buffer gives zeros.  Every thread sees tot[] afterwards.  The
+// loads are unconditional (clamped index) so each thread has all of them in
+// flight at once.
+template <int NT>
+__device__ __forceinline__ void partials_total2(const float* __restrict__ pa, int na, const float* __restrict__ pb,
+                                                int nb, double* tot, double (*red)[NT / 64])
+{
+    constexpr int G = NT / 64;
+    constexpr int R = kMaxBlk / G;
+    const int t = threadIdx.x;
+    if (t < 2 * kSE3 * G) {
+        const int col = t / G, gi = t - col * G;
+        const bool second = col >= kSE3;
+        const float* part = second ? pb : pa;
+        const int nblk = second ? nb : na;
+        const int k = second ? col - kSE3 : col;
+        double s = 0.0;
+        if (part) {
+            float v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int b = gi + r * G;
+                v[r] = part[(size_t)(b < nblk ? b : 0) * kPartStride + k];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) s += (gi + r * G < nblk) ? (double)v[r] : 0.0;
+        }
+        red[col][gi] = s;
+    }
+    __syncthreads();
+    if (t < 2 * kSE3) {
         double s = 0.0;
         for (int gi = 0; gi < G; ++gi) s += red[t][gi];
         tot[t] = s;
@@ -532,47 +806,88 @@ __device__ __forceinline__ void final_sum(const float* __restrict__ part, int nb
     __syncthreads();
 }
 
-// Last-block-done ticket.  Every thread's partial writes are released at
-// device scope before thread 0 takes a ticket; the last block acquires.
-__device__ __forceinline__ bool take_ticket(unsigned int* ticket, int nblk, int* s_last)
+// Wrapping uint32 totals of the integer partials; every thread sees tot[2].
+template <int NT>
+__device__ __forceinline__ void int_total(const unsigned int* __restrict__ part, int nblk, unsigned int* tot,
+                                          unsigned int (*red)[2])
 {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(ticket, 1u);
-        *s_last = (prev == (unsigned int)(nblk - 1));
+    unsigned int c0 = 0, c1 = 0;
+    for (int b = threadIdx.x; b < nblk; b += NT) {
+        c0 += part[2 * b];
+        c1 += part[2 * b + 1];
+    }
+    c0 = wave_total_u(c0);
+    c1 = wave_total_u(c1);
+    if ((threadIdx.x & 63) == 63) {
+        red[threadIdx.x >> 6][0] = c0;
+        red[threadIdx.x >> 6][1] = c1;
     }
     __syncthreads();
-    const bool last = *s_last != 0;
-    if (last) __threadfence();
-    return last;
+    if (threadIdx.x < 2) {
+        unsigned int s = 0;
+        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
+        tot[threadIdx.x] = s;
+    }
+    __syncthreads();
 }
 
 // ======================================================= per-pixel steps
-// a14 ICPReduction::search/getProducts, reduce.cu:282-387.  A pixel whose
-// vertex x is NaN is rejected up front: in the reference its NaN reaches
-// `dist` and fails `dist <= distThres`, so the outcome is the same.
+// a14 ICPReduction::search/getProducts, reduce.cu:282-387, split into a
+// load phase, a projection phase, a gather phase and a test/product phase so
+// one thread keeps several pixels' memory accesses in flight.
 struct IcpC {
     float Rc[9], tc[3], Rpi[9], tp[3];
     float fx, fy, cx, cy, dist, angle;
 };
 
-__device__ __forceinline__ bool icp_pixel(const LevelDesc& L, const IcpC& c, int i, size_t P, float* row)
+struct Px6 {
+    float v[3], n[3];
+};
+
+__device__ __forceinline__ void icp_load(const LevelDesc& L, int i, size_t P, Px6& s)
 {
-    const float vx = L.vcur[i];
-    if (isnan(vx)) return false;
-    const float vy = L.vcur[i + P], vz = L.vcur[i + 2 * P];
-    float vg[3], d3[3], vcp[3];
-    for (int r = 0; r < 3; ++r) vg[r] = c.Rc[3 * r] * vx + c.Rc[3 * r + 1] * vy + c.Rc[3 * r + 2] * vz + c.tc[r];
+    s.v[0] = L.vcur[i];
+    s.v[1] = L.vcur[i + P];
+    s.v[2] = L.vcur[i + 2 * P];
+    s.n[0] = L.ncur[i];
+    s.n[1] = L.ncur[i + P];
+    s.n[2] = L.ncur[i + 2 * P];
+}
+
+// Projective association (reduce.cu:290-298).  Returns the gather index or
+// -1.  A pixel whose vertex x is NaN is rejected here: in the reference its
+// NaN reaches `dist` and fails `dist <= distThres`, so the outcome is the same.
+__device__ __forceinline__ int icp_project(const LevelDesc& L, const IcpC& c, const Px6& s, float* vg, float* vcp)
+{
+    if (isnan(s.v[0])) return -1;
+    float d3[3];
+    for (int r = 0; r < 3; ++r)
+        vg[r] = c.Rc[3 * r] * s.v[0] + c.Rc[3 * r + 1] * s.v[1] + c.Rc[3 * r + 2] * s.v[2] + c.tc[r];
     for (int r = 0; r < 3; ++r) d3[r] = vg[r] - c.tp[r];
     for (int r = 0; r < 3; ++r) vcp[r] = c.Rpi[3 * r] * d3[0] + c.Rpi[3 * r + 1] * d3[1] + c.Rpi[3 * r + 2] * d3[2];
     const int ux = f2i_rn(vcp[0] * c.fx / vcp[2] + c.cx);
     const int uy = f2i_rn(vcp[1] * c.fy / vcp[2] + c.cy);
-    if (ux < 0 || uy < 0 || ux >= L.w || uy >= L.h || vcp[2] < 0) return false;
-    const size_t j = (size_t)uy * L.w + ux;
-    const float vp[3] = {L.vprev[j], L.vprev[j + P], L.vprev[j + 2 * P]};
-    const float np[3] = {L.nprev[j], L.nprev[j + P], L.nprev[j + 2 * P]};
-    const float nc[3] = {L.ncur[i], L.ncur[i + P], L.ncur[i + 2 * P]};
+    if (ux < 0 || uy < 0 || ux >= L.w || uy >= L.h || vcp[2] < 0) return -1;
+    return uy * L.w + ux;
+}
+
+__device__ __forceinline__ void icp_gather(const LevelDesc& L, int j, size_t P, Px6& d)
+{
+    d.v[0] = L.vprev[j];
+    d.v[1] = L.vprev[j + P];
+    d.v[2] = L.vprev[j + 2 * P];
+    d.n[0] = L.nprev[j];
+    d.n[1] = L.nprev[j + P];
+    d.n[2] = L.nprev[j + 2 * P];
+}
+
+// reduce.cu:305-348: normal/distance tests and the point-to-plane row.
+__device__ __forceinline__ bool icp_row(const IcpC& c, const Px6& s, const float* vg, const float* vcp,
+                                       const Px6& d, float* row)
+{
+    const float* nc = s.n;
+    const float* vp = d.v;
+    const float* np = d.n;
     float ncg[3];
     for (int r = 0; r < 3; ++r) ncg[r] = c.Rc[3 * r] * nc[0] + c.Rc[3 * r + 1] * nc[1] + c.Rc[3 * r + 2] * nc[2];
     const float df[3] = {vp[0] - vg[0], vp[1] - vg[1], vp[2] - vg[2]};
@@ -581,51 +896,49 @@ __device__ __forceinline__ bool icp_pixel(const LevelDesc& L, const IcpC& c, int
                          ncg[0] * np[1] - ncg[1] * np[0]};
     const float sine = sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
     if (!(sine < c.angle && dist <= c.dist && !isnan(nc[0]) && !isnan(np[0]))) return false;
-    float dd[3], d[3], n[3];
+    float dd[3], dv[3], n[3];
     for (int r = 0; r < 3; ++r) dd[r] = vp[r] - c.tp[r];
-    for (int r = 0; r < 3; ++r) d[r] = c.Rpi[3 * r] * dd[0] + c.Rpi[3 * r + 1] * dd[1] + c.Rpi[3 * r + 2] * dd[2];
+    for (int r = 0; r < 3; ++r) dv[r] = c.Rpi[3 * r] * dd[0] + c.Rpi[3 * r + 1] * dd[1] + c.Rpi[3 * r + 2] * dd[2];
     for (int r = 0; r < 3; ++r) n[r] = c.Rpi[3 * r] * np[0] + c.Rpi[3 * r + 1] * np[1] + c.Rpi[3 * r + 2] * np[2];
-    const float* s = vcp;  // Rprev_inv * (vcurr_g - tprev), bit for bit
+    const float* sp = vcp;  // Rprev_inv * (vcurr_g - tprev), bit for bit
     row[0] = n[0]; row[1] = n[1]; row[2] = n[2];
-    row[3] = s[1] * n[2] - s[2] * n[1];
-    row[4] = s[2] * n[0] - s[0] * n[2];
-    row[5] = s[0] * n[1] - s[1] * n[0];
-    const float sd[3] = {s[0] - d[0], s[1] - d[1], s[2] - d[2]};
+    row[3] = sp[1] * n[2] - sp[2] * n[1];
+    row[4] = sp[2] * n[0] - sp[0] * n[2];
+    row[5] = sp[0] * n[1] - sp[1] * n[0];
+    const float sd[3] = {sp[0] - dv[0], sp[1] - dv[1], sp[2] - dv[2]};
     row[6] = n[0] * sd[0] + n[1] * sd[1] + n[2] * sd[2];
     return true;
 }
 
-// a15 RGBResidual::getProducts, reduce.cu:808-830 (the pose-independent
-// tests are folded into zsrc by k_rgb_prep).
+// a15 RGBResidual::getProducts, reduce.cu:808-830; the pose-independent
+// tests are folded into zsrc by k_rgb_prep.
 struct RgbC {
     float K[9], kt[3];
     float maxDepthDelta;
 };
 
-__device__ __forceinline__ bool rgb_pixel(const LevelDesc& L, const RgbC& c, int x, int y, int idx,
-                                          float& diff, float& Z, int& u0, int& v0)
+__device__ __forceinline__ int rgb_warp(const LevelDesc& L, const RgbC& c, int x, int y, float d1, float& tz)
 {
-    const float d1 = L.zsrc[idx];
-    if (isnan(d1)) return false;
+    if (isnan(d1)) return -1;
     const float xf = (float)x, yf = (float)y;
-    const float tz = d1 * (c.K[6] * xf + c.K[7] * yf + c.K[8]) + c.kt[2];
-    u0 = f2i_rn((d1 * (c.K[0] * xf + c.K[1] * yf + c.K[2]) + c.kt[0]) / tz);
-    v0 = f2i_rn((d1 * (c.K[3] * xf + c.K[4] * yf + c.K[5]) + c.kt[1]) / tz);
-    if (!(u0 >= 0 && v0 >= 0 && u0 < L.w && v0 < L.h)) return false;
-    const size_t m = (size_t)v0 * L.w + u0;
-    const float d0 = L.zlast[m];
-    const uint8_t il = L.ilast[m];
-    if (!(d0 > 0 && fabsf(tz - d0) <= c.maxDepthDelta && il != 0)) return false;
-    diff = (float)L.inext[idx] - (float)il;
-    Z = d0;
-    return true;
+    tz = d1 * (c.K[6] * xf + c.K[7] * yf + c.K[8]) + c.kt[2];
+    const int u0 = f2i_rn((d1 * (c.K[0] * xf + c.K[1] * yf + c.K[2]) + c.kt[0]) / tz);
+    const int v0 = f2i_rn((d1 * (c.K[3] * xf + c.K[4] * yf + c.K[5]) + c.kt[1]) / tz);
+    if (!(u0 >= 0 && v0 >= 0 && u0 < L.w && v0 < L.h)) return -1;
+    return v0 * L.w + u0;
+}
+
+__device__ __forceinline__ bool rgb_test(const RgbC& c, float tz, float d0, uint8_t il)
+{
+    return d0 > 0 && fabsf(tz - d0) <= c.maxDepthDelta && il != 0;
 }
 
 // a16 RGBReduction::getProducts, reduce.cu:512-552; the cloud point is
 // projectPointsKernel's (utils.cu:655-657) recomputed from lastDepth.
-__device__ __forceinline__ void rgb_row(const LevelDesc& L, float sigma, float sobelScale, int idx, float diff,
-                                        float Z, int u0, int v0, float* row)
+__device__ __forceinline__ void rgb_row(const LevelDesc& L, float sigma, float sobelScale, short2 gr, float diff,
+                                        float Z, int m, float* row)
 {
+    const int v0 = m / L.w, u0 = m - v0 * L.w;
     float wt = sigma + fabsf(diff);
     wt = wt > FLT_EPSILON ? 1.0f / wt : 1.0f;
     if (sigma == -1) wt = 1;
@@ -633,7 +946,6 @@ __device__ __forceinline__ void rgb_row(const LevelDesc& L, float sigma, float s
     const float X = ((float)u0 - L.cx) * Z * L.invFx;
     const float Y = ((float)v0 - L.cy) * Z * L.invFy;
     const float invz = 1.0f / Z;
-    const short2 gr = L.dxy[idx];
     const float gx = wt * sobelScale * (float)gr.x;
     const float gy = wt * sobelScale * (float)gr.y;
     const float a0 = gx * L.fx * invz;
@@ -664,28 +976,31 @@ __device__ __forceinline__ void acc_row7(float (&acc)[kSE3], const float* row)
 // ===================================================== single-thread math
 // Per-iteration warp constants from resultRt (RGBDOdometryef.cpp:422-432):
 // Rt = resultRt^-1 taken as the rigid inverse [R^T | -R^T t].
-__device__ void iter_constants(TrackState* st, const LevelDesc& L)
+__device__ void iter_constants(IterState& S, const LevelDesc& L)
 {
-    const double* Rt = st->resultRt;
-    double Rinv[9], tinv[3], K[9], Ki[9], KR[9], KRKi[9];
+    const double* Rt = S.resultRt;
+    const double* K = L.K;
+    const double* Ki = L.Ki;
+    double Rinv[9], tinv[3], KR[9], KRKi[9];
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) Rinv[a * 3 + b] = Rt[b * 4 + a];
     for (int a = 0; a < 3; ++a) tinv[a] = -(Rinv[a * 3] * Rt[3] + Rinv[a * 3 + 1] * Rt[7] + Rinv[a * 3 + 2] * Rt[11]);
-    kmat(L.fx, L.fy, L.cx, L.cy, K, Ki);
     mm3d(K, Rinv, KR);
     mm3d(KR, Ki, KRKi);
-    for (int a = 0; a < 9; ++a) st->krkinv[a] = (float)KRKi[a];
-    for (int a = 0; a < 3; ++a) st->kt[a] = (float)(K[a * 3] * tinv[0] + K[a * 3 + 1] * tinv[1] + K[a * 3 + 2] * tinv[2]);
+    for (int a = 0; a < 9; ++a) S.krkinv[a] = (float)KRKi[a];
+    for (int a = 0; a < 3; ++a) S.kt[a] = (float)(K[a * 3] * tinv[0] + K[a * 3 + 1] * tinv[1] + K[a * 3 + 2] * tinv[2]);
 }
 
 // RGBDOdometryef.cpp:533-575: combine, solve, computeUpdateSE3, pose update.
-__device__ void solve_update(TrackState* st, const TrackParams& P)
+// rgb_sums: FP64 totals of the photometric pass (unused without it).
+__device__ void solve_update(IterState& S, const TrackParams& P, const double* rgb_sums, const float* Rprev,
+                             const float* tprev)
 {
     double Ai[36], bi[6], Ar[36], br[6], A[36], b[6], x[6];
     for (int k = 0; k < 36; ++k) { Ai[k] = 0; Ar[k] = 0; }
     for (int k = 0; k < 6; ++k) { bi[k] = 0; br[k] = 0; }
-    if (P.icp) unpack_se3(st->icp_sums, Ai, bi);
-    if (P.rgb) unpack_se3(st->rgb_sums, Ar, br);
+    if (P.icp) unpack_se3(S.icp_sums, Ai, bi);
+    if (P.rgb) unpack_se3(rgb_sums, Ar, br);
     if (P.icp && P.rgb) {
         const double wgt = P.icpWeight;
         for (int k = 0; k < 36; ++k) A[k] = Ar[k] + wgt * wgt * Ai[k];
@@ -697,21 +1012,20 @@ __device__ void solve_update(TrackState* st, const TrackParams& P)
         for (int k = 0; k < 36; ++k) A[k] = Ar[k];
         for (int k = 0; k < 6; ++k) b[k] = br[k];
     }
-    for (int k = 0; k < 36; ++k) st->lastA[k] = A[k];
-    for (int k = 0; k < 6; ++k) st->lastb[k] = b[k];
-    const int zeros = ldlt_solve(6, A, b, x);
-    st->degenerate = (zeros == 6);
-    for (int k = 0; k < 6; ++k) st->lastx[k] = x[k];
-    double rot[9], upd[16], nrt[16];
+    for (int k = 0; k < 36; ++k) S.lastA[k] = A[k];
+    for (int k = 0; k < 6; ++k) S.lastb[k] = b[k];
+    const int zeros = ldlt_solve<6>(A, b, x);
+    S.degenerate = (zeros == 6);
+    double rot[9], upd[16], nrt[16], old[16];
     rodrigues(x + 3, rot);
-    for (int a = 0; a < 16; ++a) upd[a] = 0.0;
+    for (int a = 0; a < 16; ++a) { upd[a] = 0.0; old[a] = S.resultRt[a]; }
     for (int a = 0; a < 3; ++a) {
         for (int c = 0; c < 3; ++c) upd[a * 4 + c] = rot[a * 3 + c];
         upd[a * 4 + 3] = x[a];
     }
     upd[15] = 1.0;
-    mm4d(upd, st->resultRt, nrt);
-    for (int a = 0; a < 16; ++a) st->resultRt[a] = nrt[a];
+    mm4d(upd, old, nrt);
+    for (int a = 0; a < 16; ++a) S.resultRt[a] = nrt[a];
     float Rf[9], tf[3], Ri[9], ti[3], Rn[9];
     for (int a = 0; a < 3; ++a) {
         for (int c = 0; c < 3; ++c) Rf[a * 3 + c] = (float)nrt[a * 4 + c];
@@ -720,22 +1034,173 @@ __device__ void solve_update(TrackState* st, const TrackParams& P)
     for (int a = 0; a < 3; ++a)
         for (int c = 0; c < 3; ++c) Ri[a * 3 + c] = Rf[c * 3 + a];
     for (int a = 0; a < 3; ++a) ti[a] = -(Ri[a * 3] * tf[0] + Ri[a * 3 + 1] * tf[1] + Ri[a * 3 + 2] * tf[2]);
-    mm3f(st->Rprev, Ri, Rn);
-    for (int a = 0; a < 9; ++a) st->Rcurr[a] = Rn[a];
+    mm3f(Rprev, Ri, Rn);
+    for (int a = 0; a < 9; ++a) S.Rcurr[a] = Rn[a];
     for (int a = 0; a < 3; ++a)
-        st->tcurr[a] = st->Rprev[a * 3] * ti[0] + st->Rprev[a * 3 + 1] * ti[1] + st->Rprev[a * 3 + 2] * ti[2] + st->tprev[a];
-    st->iters_done++;
+        S.tcurr[a] = Rprev[a * 3] * ti[0] + Rprev[a * 3 + 1] * ti[1] + Rprev[a * 3 + 2] * ti[2] + tprev[a];
+    S.iters_done++;
 }
 
-// RGBDOdometryef.cpp:434-471 and 509-510: photometric bookkeeping after the
-// association pass.  Returns false when the rgbOnly early exit fires.
-__device__ bool after_association(TrackState* st, const PairDesc& D, const TrackParams& P, int level, int iter,
-                                  int break_next, int count, int sigma)
+// The same solve and update, run by one full wave (64 lanes): lane l < 36
+// holds A[l/6][l%6], lanes 36..41 hold b.  Pivot search, the symmetric row
+// and column swaps, the column scaling and the rank-1 update of each step are
+// lane-parallel; every element sees the operations of the scalar routine in
+// the same order, so the result is the same.
+__device__ __forceinline__ int packed_index(int i, int j)
+{
+    return i * 7 - (i * (i - 1)) / 2 + (j - i);
+}
+
+__device__ void solve_update_wave(IterState& S, const TrackParams& P, const double* icp_sums,
+                                  const double* rgb_sums, const float* Rprev, const float* tprev)
+{
+    const int l = threadIdx.x & 63;
+    const int ri = l / 6, ci = l - 6 * (l / 6);
+    const bool inA = l < 36;
+    // combined system (RGBDOdometryef.cpp:539-557)
+    double v = 0.0;
+    if (l < 42) {
+        const int s = inA ? packed_index(min(ri, ci), max(ri, ci)) : packed_index(l - 36, 6);
+        const double ai = P.icp ? icp_sums[s] : 0.0;
+        const double ar = P.rgb ? rgb_sums[s] : 0.0;
+        if (P.icp && P.rgb) {
+            const double wgt = P.icpWeight;
+            v = inA ? ar + wgt * wgt * ai : ar + wgt * ai;
+        } else {
+            v = P.icp ? ai : ar;
+        }
+        if (inA) S.lastA[l] = v;
+        else S.lastb[l - 36] = v;
+    }
+    double a = inA ? v : 0.0;
+    const double bv = (l >= 36 && l < 42) ? v : 0.0;
+    int pv = l;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int p = k;
+        double best = fabs(__shfl(a, k * 7));
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i) {
+            const double di = fabs(__shfl(a, i * 7));
+            if (di > best) {
+                best = di;
+                p = i;
+            }
+        }
+        if (p != k) {
+            const int si = (ri == k) ? p : (ri == p ? k : ri);
+            const int sj = (ci == k) ? p : (ci == p ? k : ci);
+            const double an = __shfl(a, inA ? si * 6 + sj : l);
+            a = inA ? an : a;
+            const int pn = __shfl(pv, (l == k) ? p : (l == p ? k : l));
+            pv = (l < 6) ? pn : pv;
+        }
+        const double d = __shfl(a, k * 7);
+        const bool colk = inA && ri > k && ci == k;
+        if (fabs(d) <= DBL_MIN) {
+            if (colk) a = 0.0;
+        } else {
+            if (colk) a = a / d;
+            const int hi = max(ri, ci), lo = min(ri, ci);
+            const double lhi = __shfl(a, inA ? hi * 6 + k : 0);
+            const double llo = __shfl(a, inA ? lo * 6 + k : 0);
+            if (inA && ri > k && ci > k) a = a - lhi * llo * d;
+        }
+    }
+    // y = P b, forward substitution, D^+ , backward substitution, x = P^T y
+    double y = __shfl(bv, 36 + ((l < 6) ? pv : 0));
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const double yj = __shfl(y, j);
+        const double lij = __shfl(a, (l < 6) ? l * 6 + j : 0);
+        if (l < 6 && l > j) y = y - lij * yj;
+    }
+    const double dd = __shfl(a, (l < 6) ? l * 7 : 0);
+    int zero = 0;
+    if (l < 6) {
+        if (fabs(dd) > DBL_MIN) y = y / dd;
+        else { y = 0.0; zero = 1; }
+    }
+    const int zeros = __popcll(__ballot(zero));
+#pragma unroll
+    for (int i = 4; i >= 0; --i)
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j) {
+            const double yj = __shfl(y, j);
+            const double aji = __shfl(a, j * 6 + i);
+            if (l == i) y = y - aji * yj;
+        }
+    double x[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) x[q] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const int pi = __shfl(pv, i);
+        const double yi = __shfl(y, i);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) x[q] = (pi == q) ? yi : x[q];
+    }
+    // computeUpdateSE3 (utils.h:54-74) and the pose update (RGBDOdometryef.cpp:563-575),
+    // evaluated identically in every lane
+    double rot[9];
+    rodrigues(x + 3, rot);
+    double upd[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) upd[q] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) upd[r * 4 + c] = rot[r * 3 + c];
+        upd[r * 4 + 3] = x[r];
+    }
+    upd[15] = 1.0;
+    const double oc = S.resultRt[l & 15];
+    const int r4 = (l >> 2) & 3, c4 = l & 3;
+    const double o0 = __shfl(oc, c4), o1 = __shfl(oc, 4 + c4), o2 = __shfl(oc, 8 + c4), o3 = __shfl(oc, 12 + c4);
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 1.0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        u0 = (r4 == r) ? upd[r * 4] : u0;
+        u1 = (r4 == r) ? upd[r * 4 + 1] : u1;
+        u2 = (r4 == r) ? upd[r * 4 + 2] : u2;
+        u3 = (r4 == r) ? upd[r * 4 + 3] : u3;
+    }
+    const double nrt = u0 * o0 + u1 * o1 + u2 * o2 + u3 * o3;
+    float Rf[9], tf[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = (float)__shfl(nrt, r * 4 + c);
+        tf[r] = (float)__shfl(nrt, r * 4 + 3);
+    }
+    if (l < 16) S.resultRt[l] = nrt;
+    float Ri[9], ti[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Ri[r * 3 + c] = Rf[c * 3 + r];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r * 3] * tf[0] + Ri[r * 3 + 1] * tf[1] + Ri[r * 3 + 2] * tf[2]);
+    if (l < 9) {
+        const int r = l / 3, c = l - 3 * (l / 3);
+        S.Rcurr[l] = Rprev[r * 3] * Ri[c] + Rprev[r * 3 + 1] * Ri[3 + c] + Rprev[r * 3 + 2] * Ri[6 + c];
+    }
+    if (l < 3) S.tcurr[l] = Rprev[l * 3] * ti[0] + Rprev[l * 3 + 1] * ti[1] + Rprev[l * 3 + 2] * ti[2] + tprev[l];
+    if (l == 0) {
+        S.degenerate = (zeros == 6);
+        S.iters_done++;
+    }
+}
+
+// RGBDOdometryef.cpp:434-471: photometric bookkeeping after the association
+// pass of an iteration.  cmp: the lastRGBError this iteration compares to.
+// Sets S.broke (rgbOnly early exit) or S.pending (a solve is due).
+__device__ void assoc_rgb(IterState& S, const TrackParams& P, float cmp, int count, int sigma)
 {
     float sigmaVal, rgbError;
     if (P.rgb) {
-        st->rgb_count = count;
-        st->rgb_sigma = sigma;
+        S.rgb_count = count;
+        S.rgb_sigma = sigma;
         sigmaVal = (float)sqrt((double)((((float)sigma / (float)count) == 0.0f) ? 1 : count));
         rgbError = (float)(sqrt((double)sigma) / (double)(count == 0 ? 1 : count));
     } else {
@@ -743,262 +1208,459 @@ __device__ bool after_association(TrackState* st, const PairDesc& D, const Track
         rgbError = 0.0f;
         count = 0;
     }
-    const float cmp = (iter == 0) ? FLT_MAX : st->lastRGBError;
     if (P.rgbOnly && rgbError > cmp) {
-        // the level ends here; the next level starts from the unchanged increment
-        st->break_level = level;
-        if (break_next >= 0) iter_constants(st, D.lv[break_next]);
-        return false;
+        S.broke = 1;
+        S.pending = 0;
+        return;
     }
-    st->lastRGBError = rgbError;
-    st->lastRGBCount = (float)count;
+    S.broke = 0;
+    S.lastRGBError = rgbError;
+    S.lastRGBCount = (float)count;
     if (P.rgbOnly) sigmaVal = -1;
-    st->sigmaVal = sigmaVal;
+    S.sigmaVal = sigmaVal;
+    S.pending = 1;
+}
+
+// RGBDOdometryef.cpp:509-510: residual[] of the ICP step and its errors.
+__device__ void icp_stats(IterState& S, const TrackParams& P, const double* icp_sums)
+{
     if (P.icp) {
-        st->res0 = (float)st->icp_sums[27];
-        st->res1 = (float)st->icp_sums[28];
+        for (int q = 0; q < kSE3; ++q) S.icp_sums[q] = icp_sums[q];
+        S.res0 = (float)icp_sums[27];
+        S.res1 = (float)icp_sums[28];
     }
-    st->lastICPError = sqrtf(st->res0) / st->res1;
-    st->lastICPCount = st->res1;
-    return true;
+    S.lastICPError = sqrtf(S.res0) / S.res1;
+    S.lastICPCount = S.res1;
+}
+
+// Word-by-word copy of a state record by all threads of the block.
+template <int NT, class T>
+__device__ __forceinline__ void copy_words(T* dst, const T* src)
+{
+    static_assert(sizeof(T) % 4 == 0, "state records are word aligned");
+    const unsigned int* s = (const unsigned int*)src;
+    unsigned int* d = (unsigned int*)dst;
+    for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += NT) d[i] = s[i];
+}
+
+// Scratch shared by the prologues.
+template <int NT>
+struct Prolog {
+    IterState S;
+    double fin[2 * kSE3][NT / 64];
+    double tot[2 * kSE3];
+    unsigned int redi[NT / 64][2];
+    unsigned int toti[2];
+};
+
+// Finish iteration k-1 and leave the state entering iteration k in W.S: one
+// fused FP64 reduction of its ICP partials (pass A) and photometric partials
+// (pass B), the ICP statistics, the photometric bookkeeping when there is no
+// pass B, and the solve.  Executed by every thread of the block; every block
+// computes the same state.
+template <int NT>
+__device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level, Prolog<NT>& W)
+{
+    TrackState* st = D.st;
+    copy_words<NT>(&W.S, &st->it[(k - 1) & 1]);
+    __syncthreads();
+    if (k == 0) return;
+    const int pb = (k - 1) & 1;
+    const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
+    if (run) {
+        partials_total2<NT>(P.icp ? D.part_a[pb] : nullptr, D.nblk_a[prev_level], P.rgb ? D.part_b[pb] : nullptr,
+                            D.nblk_b[prev_level], W.tot, W.fin);
+        if (threadIdx.x < 64) {
+            if (threadIdx.x == 0) {
+                if (!P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
+                icp_stats(W.S, P, W.tot);
+            }
+            solve_update_wave(W.S, P, W.tot, W.tot + kSE3, st->Rprev, st->tprev);
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (W.S.broke) W.S.break_level = prev_level;
+        W.S.pending = 0;
+        W.S.broke = 0;
+    }
+    __syncthreads();
 }
 
 // ======================================================== step kernel A
 // ICP products (a14) and the photometric association count (a15) of one
-// level of every pair.  The last block per pair reduces in FP64 and either
-// records the sums (mode 1) or continues the Gauss-Newton iteration.
+// level of every pair, after finishing the previous iteration in the
+// prologue.  Each pair's level is cut into tiles of NT*PPT pixels; its
+// blocks (at most kMaxBlk) stride over the tiles.  mode 1: parity pass with
+// the constants of state slot 0.
+template <int NT, int PPT>
+struct TileA {
+    Px6 src[PPT];
+    float zs[PPT];
+    uint8_t in[PPT];
+};
+
+template <int NT, int PPT>
+__device__ __forceinline__ void load_tile_a(const LevelDesc& L, const TrackParams& P, int base, int N, size_t P3,
+                                            TileA<NT, PPT>& T)
+{
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = base + q * NT;
+        T.zs[q] = QNAN;
+        T.in[q] = 0;
+        T.src[q].v[0] = QNAN;
+        if (idx < N) {
+            if (P.icp) icp_load(L, idx, P3, T.src[q]);
+            if (P.rgb) {
+                T.zs[q] = L.zsrc[idx];
+                T.in[q] = L.inext[idx];
+            }
+        }
+    }
+}
+
+template <int NT, int PPT>
+__device__ __forceinline__ void work_tile_a(const LevelDesc& L, const TrackParams& P, const IcpC& ic, const RgbC& rc,
+                                            int base, int N, size_t P3, const TileA<NT, PPT>& T,
+                                            float (&acc)[kSE3], unsigned int& cnt, unsigned int& sig,
+                                            uint8_t* __restrict__ dbg_mask)
+{
+    unsigned int m[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) m[q] = 0;
+    if (P.icp) {
+        float vg[PPT][3], vcp[PPT][3];
+        int j[PPT];
+        Px6 dst[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) j[q] = icp_project(L, ic, T.src[q], vg[q], vcp[q]);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            if (j[q] >= 0) icp_gather(L, j[q], P3, dst[q]);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            float row[7];
+            if (j[q] >= 0 && icp_row(ic, T.src[q], vg[q], vcp[q], dst[q], row)) {
+                acc_row7(acc, row);
+                m[q] = 1u;
+            }
+        }
+    }
+    if (P.rgb) {
+        int jm[PPT];
+        float tz[PPT], d0[PPT];
+        uint8_t il[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = base + q * NT;
+            const int y = idx / L.w, x = idx - y * L.w;
+            jm[q] = (idx < N) ? rgb_warp(L, rc, x, y, T.zs[q], tz[q]) : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            if (jm[q] >= 0) {
+                d0[q] = L.zlast[jm[q]];
+                il[q] = L.ilast[jm[q]];
+            }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            if (jm[q] >= 0 && rgb_test(rc, tz[q], d0[q], il[q])) {
+                const float diff = (float)T.in[q] - (float)il[q];
+                cnt += 1u;
+                sig += (unsigned int)(int)(diff * diff);
+                m[q] |= 2u;
+            }
+    }
+    if (dbg_mask)
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = base + q * NT;
+            if (idx < N) dbg_mask[idx] = (uint8_t)m[q];
+        }
+}
+
 template <int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pairs, const int* __restrict__ blk_pair,
-                                               int level, int iter, int next_level, int break_next, TrackParams P,
-                                               int mode, uint8_t* __restrict__ dbg_mask)
+                                               int level, int iter, int k, int prev_level, TrackParams P, int mode,
+                                               uint8_t* __restrict__ dbg_mask)
 {
+    __shared__ Prolog<NT> W;
     __shared__ float s_red[NT / 64][kSE3];
-    __shared__ unsigned int s_redi[NT / 64][2];
-    __shared__ double s_fin[kSE3][NT / 32];
-    __shared__ double s_tot[kSE3];
-    __shared__ int s_last;
     const int p = blk_pair ? blk_pair[blockIdx.x] : 0;
     const PairDesc& D = pairs[p];
     TrackState* st = D.st;
-    if (mode == 0 && st->break_level == level) return;
     const LevelDesc L = D.lv[level];
     const int lb = blockIdx.x - D.blk0_a[level];
+    const int nblk = D.nblk_a[level];
     const int N = L.w * L.h;
     const size_t P3 = (size_t)N;
+    const int ntiles = (N + NT * PPT - 1) / (NT * PPT);
+
+    // pose-independent loads of the first tile overlap the prologue
+    TileA<NT, PPT> T;
+    load_tile_a<NT, PPT>(L, P, lb * NT * PPT + threadIdx.x, N, P3, T);
+
+    if (mode == 0) {
+        finish_previous<NT>(D, P, k, prev_level, W);
+        if (threadIdx.x == 0) {
+            W.S.skip = (W.S.break_level == level);
+            if (!W.S.skip) iter_constants(W.S, L);
+            W.S.rgb_err_cmp = (iter == 0) ? FLT_MAX : W.S.lastRGBError;
+        }
+        __syncthreads();
+        if (lb == 0) copy_words<NT>(&st->it[k & 1], &W.S);
+        if (W.S.skip) return;
+    } else {
+        copy_words<NT>(&W.S, &st->it[0]);
+        __syncthreads();
+    }
+
     IcpC ic;
     RgbC rc;
-    for (int k = 0; k < 9; ++k) { ic.Rc[k] = st->Rcurr[k]; ic.Rpi[k] = st->Rprev_inv[k]; rc.K[k] = st->krkinv[k]; }
-    for (int k = 0; k < 3; ++k) { ic.tc[k] = st->tcurr[k]; ic.tp[k] = st->tprev[k]; rc.kt[k] = st->kt[k]; }
+    for (int q = 0; q < 9; ++q) { ic.Rc[q] = W.S.Rcurr[q]; ic.Rpi[q] = st->Rprev_inv[q]; rc.K[q] = W.S.krkinv[q]; }
+    for (int q = 0; q < 3; ++q) { ic.tc[q] = W.S.tcurr[q]; ic.tp[q] = st->tprev[q]; rc.kt[q] = W.S.kt[q]; }
     ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
     ic.dist = P.distThres; ic.angle = P.angleThres;
     rc.maxDepthDelta = P.maxDepthDelta;
 
     float acc[kSE3];
 #pragma unroll
-    for (int k = 0; k < kSE3; ++k) acc[k] = 0.0f;
+    for (int q = 0; q < kSE3; ++q) acc[q] = 0.0f;
     unsigned int cnt = 0, sig = 0;
-    const int base = lb * NT * PPT + threadIdx.x;
+    for (int tile = lb; tile < ntiles; tile += nblk) {
+        const int base = tile * NT * PPT + threadIdx.x;
+        if (tile != lb) load_tile_a<NT, PPT>(L, P, base, N, P3, T);
+        work_tile_a<NT, PPT>(L, P, ic, rc, base, N, P3, T, acc, cnt, sig, dbg_mask);
+    }
+    const int pb = k & 1;
+    block_partial<kSE3, NT>(acc, D.part_a[pb] + (size_t)lb * kPartStride, s_red);
+    block_partial_int<NT>(cnt, sig, D.part_ai[pb] + (size_t)lb * 2, W.redi);
+}
+
+// ======================================================== step kernel B
+// Photometric products (a16) with the robust weight of the global count.
+// The prologue reduces pass A's partials of the same iteration: counts
+// (every block: sigma) and, in the pair's block 0, the ICP sums it records.
+template <int NT, int PPT>
+struct TileB {
+    float zs[PPT];
+    uint8_t in[PPT];
+    short2 gr[PPT];
+};
+
+template <int NT, int PPT>
+__device__ __forceinline__ void load_tile_b(const LevelDesc& L, int base, int N, TileB<NT, PPT>& T)
+{
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int idx = base + k * NT;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = base + q * NT;
+        T.zs[q] = QNAN;
+        T.in[q] = 0;
+        T.gr[q] = make_short2(0, 0);
         if (idx < N) {
-            unsigned int m = 0;
-            if (P.icp) {
-                float row[7];
-                if (icp_pixel(L, ic, idx, P3, row)) {
-                    acc_row7(acc, row);
-                    m |= 1u;
-                }
-            }
-            if (P.rgb) {
-                const int y = idx / L.w, x = idx - y * L.w;
-                float diff, Z;
-                int u0, v0;
-                if (rgb_pixel(L, rc, x, y, idx, diff, Z, u0, v0)) {
-                    cnt += 1u;
-                    sig += (unsigned int)(int)(diff * diff);
-                    m |= 2u;
-                }
-            }
-            if (dbg_mask) dbg_mask[idx] = (uint8_t)m;
-        }
-    }
-    block_partial<kSE3, NT>(acc, D.part_a + (size_t)lb * kPartStride, s_red);
-    {
-        // integer partials: count and the wrapping int32 sum of diff^2
-        unsigned int c2[2] = {cnt, sig};
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            c2[0] += __shfl_xor(c2[0], off);
-            c2[1] += __shfl_xor(c2[1], off);
-        }
-        if ((threadIdx.x & 63) == 0) { s_redi[threadIdx.x >> 6][0] = c2[0]; s_redi[threadIdx.x >> 6][1] = c2[1]; }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            unsigned int s = 0;
-            for (int wv = 0; wv < NT / 64; ++wv) s += s_redi[wv][threadIdx.x];
-            D.part_ai[(size_t)lb * 2 + threadIdx.x] = s;
-        }
-    }
-    const int nblk = D.nblk_a[level];
-    if (!take_ticket(&st->ticket_a, nblk, &s_last)) return;
-    final_sum<kSE3, NT>(D.part_a, nblk, s_tot, s_fin);
-    if (threadIdx.x == 0) {
-        unsigned int c = 0, sg = 0;
-        for (int b = 0; b < nblk; ++b) { c += D.part_ai[2 * b]; sg += D.part_ai[2 * b + 1]; }
-        st->ticket_a = 0;
-        if (P.icp)
-            for (int k = 0; k < kSE3; ++k) st->icp_sums[k] = s_tot[k];
-        if (mode == 1) {
-            st->rgb_count = (int)c;
-            st->rgb_sigma = (int)sg;
-        } else if (after_association(st, D, P, level, iter, break_next, (int)c, (int)sg) && !P.rgb) {
-            solve_update(st, P);
-            if (next_level >= 0) iter_constants(st, D.lv[next_level]);
+            T.zs[q] = L.zsrc[idx];
+            T.in[q] = L.inext[idx];
+            T.gr[q] = L.dxy[idx];
         }
     }
 }
 
-// ======================================================== step kernel B
-// Photometric products (a16) with the robust weight of the global count, then
-// (mode 0) the combined solve and SE(3) update by the last block per pair.
 template <int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pairs, const int* __restrict__ blk_pair,
-                                               int level, int iter, int next_level, TrackParams P, int mode)
+                                               int level, int iter, int k, TrackParams P, int mode)
 {
+    __shared__ Prolog<NT> W;
     __shared__ float s_red[NT / 64][kSE3];
-    __shared__ double s_fin[kSE3][NT / 32];
-    __shared__ double s_tot[kSE3];
-    __shared__ int s_last;
     const int p = blk_pair ? blk_pair[blockIdx.x] : 0;
     const PairDesc& D = pairs[p];
     TrackState* st = D.st;
-    if (mode == 0 && st->break_level == level) return;
     const LevelDesc L = D.lv[level];
     const int lb = blockIdx.x - D.blk0_b[level];
+    const int nblk = D.nblk_b[level];
     const int N = L.w * L.h;
+    const int ntiles = (N + NT * PPT - 1) / (NT * PPT);
+    const int pb = k & 1;
+    const IterState& Sg = st->it[mode == 0 ? pb : 0];
+    if (mode == 0 && Sg.skip) return;
+
+    TileB<NT, PPT> T;
+    load_tile_b<NT, PPT>(L, lb * NT * PPT + threadIdx.x, N, T);
+
+    float sigma;
+    if (mode == 0) {
+        int_total<NT>(D.part_ai[pb], D.nblk_a[level], W.toti, W.redi);
+        if (threadIdx.x == 0) {
+            IterState& S = W.S;
+            assoc_rgb(S, P, Sg.rgb_err_cmp, (int)W.toti[0], (int)W.toti[1]);
+            if (lb == 0) {
+                IterState* G = &st->it[pb];
+                G->rgb_count = S.rgb_count;
+                G->rgb_sigma = S.rgb_sigma;
+                if (!S.broke) {
+                    G->lastRGBError = S.lastRGBError;
+                    G->lastRGBCount = S.lastRGBCount;
+                    G->sigmaVal = S.sigmaVal;
+                }
+                G->pending = S.pending;
+                G->broke = S.broke;
+            }
+        }
+        __syncthreads();
+        if (W.S.broke) return;
+        sigma = W.S.sigmaVal;
+    } else {
+        sigma = Sg.sigmaVal;
+    }
+
     RgbC rc;
-    for (int k = 0; k < 9; ++k) rc.K[k] = st->krkinv[k];
-    for (int k = 0; k < 3; ++k) rc.kt[k] = st->kt[k];
+    for (int q = 0; q < 9; ++q) rc.K[q] = Sg.krkinv[q];
+    for (int q = 0; q < 3; ++q) rc.kt[q] = Sg.kt[q];
     rc.maxDepthDelta = P.maxDepthDelta;
-    const float sigma = st->sigmaVal;
 
     float acc[kSE3];
 #pragma unroll
-    for (int k = 0; k < kSE3; ++k) acc[k] = 0.0f;
-    const int base = lb * NT * PPT + threadIdx.x;
+    for (int q = 0; q < kSE3; ++q) acc[q] = 0.0f;
+    for (int tile = lb; tile < ntiles; tile += nblk) {
+        const int base = tile * NT * PPT + threadIdx.x;
+        if (tile != lb) load_tile_b<NT, PPT>(L, base, N, T);
+        int jm[PPT];
+        float tz[PPT], d0[PPT];
+        uint8_t il[PPT];
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int idx = base + k * NT;
-        if (idx < N) {
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = base + q * NT;
             const int y = idx / L.w, x = idx - y * L.w;
-            float diff, Z;
-            int u0, v0;
-            if (rgb_pixel(L, rc, x, y, idx, diff, Z, u0, v0)) {
+            jm[q] = (idx < N) ? rgb_warp(L, rc, x, y, T.zs[q], tz[q]) : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            if (jm[q] >= 0) {
+                d0[q] = L.zlast[jm[q]];
+                il[q] = L.ilast[jm[q]];
+            }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            if (jm[q] >= 0 && rgb_test(rc, tz[q], d0[q], il[q])) {
+                const float diff = (float)T.in[q] - (float)il[q];
                 float row[7];
-                rgb_row(L, sigma, P.sobelScale, idx, diff, Z, u0, v0, row);
+                rgb_row(L, sigma, P.sobelScale, T.gr[q], diff, d0[q], jm[q], row);
                 acc_row7(acc, row);
             }
-        }
     }
-    block_partial<kSE3, NT>(acc, D.part_b + (size_t)lb * kPartStride, s_red);
-    const int nblk = D.nblk_b[level];
-    if (!take_ticket(&st->ticket_b, nblk, &s_last)) return;
-    final_sum<kSE3, NT>(D.part_b, nblk, s_tot, s_fin);
-    if (threadIdx.x == 0) {
-        st->ticket_b = 0;
-        for (int k = 0; k < kSE3; ++k) st->rgb_sums[k] = s_tot[k];
-        if (mode == 0) {
-            solve_update(st, P);
-            if (next_level >= 0) iter_constants(st, D.lv[next_level]);
-        }
+    block_partial<kSE3, NT>(acc, D.part_b[pb] + (size_t)lb * kPartStride, s_red);
+}
+
+// Totals of one parity pass (mode 1) into the state's dbg fields.
+__global__ __launch_bounds__(256) void k_reduce_dbg(const PairDesc* __restrict__ pairs, int which, int level)
+{
+    __shared__ Prolog<256> W;
+    const PairDesc& D = pairs[0];
+    TrackState* st = D.st;
+    const int nblk = which == 0 ? D.nblk_a[level] : D.nblk_b[level];
+    partials_total2<256>(which == 0 ? D.part_a[0] : D.part_b[0], nblk, nullptr, 1, W.tot, W.fin);
+    if (which == 0) int_total<256>(D.part_ai[0], nblk, W.toti, W.redi);
+    if (threadIdx.x < kSE3) st->dbg_sums[threadIdx.x] = W.tot[threadIdx.x];
+    if (threadIdx.x == 0 && which == 0) {
+        st->dbg_count = (int)W.toti[0];
+        st->dbg_sigma = (int)W.toti[1];
     }
+}
+
+void launch_reduce_dbg(const PairDesc* pairs, int which, int level, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_dbg, dim3(1), dim3(256), 0, s, pairs, which, level);
 }
 
 int step_shape_supported(StepShape sh)
 {
-    const bool t = sh.threads == 128 || sh.threads == 256 || sh.threads == 512;
-    const bool p = sh.ppt == 1 || sh.ppt == 2 || sh.ppt == 4 || sh.ppt == 8;
+    const bool t = sh.threads == 256 || sh.threads == 512;
+    const bool p = sh.ppt == 1 || sh.ppt == 2 || sh.ppt == 4;
     return t && p;
 }
 
-#define STEP_DISPATCH(KER, ...)                                                                         \
-    do {                                                                                               \
-        const dim3 grid(L.total_blocks);                                                               \
-        switch (L.shape.threads * 100 + L.shape.ppt) {                                                 \
-            case 12801: hipLaunchKernelGGL((KER<128, 1>), grid, dim3(128), 0, s, __VA_ARGS__); break;  \
-            case 12802: hipLaunchKernelGGL((KER<128, 2>), grid, dim3(128), 0, s, __VA_ARGS__); break;  \
-            case 12804: hipLaunchKernelGGL((KER<128, 4>), grid, dim3(128), 0, s, __VA_ARGS__); break;  \
-            case 12808: hipLaunchKernelGGL((KER<128, 8>), grid, dim3(128), 0, s, __VA_ARGS__); break;  \
-            case 25601: hipLaunchKernelGGL((KER<256, 1>), grid, dim3(256), 0, s, __VA_ARGS__); break;  \
-            case 25602: hipLaunchKernelGGL((KER<256, 2>), grid, dim3(256), 0, s, __VA_ARGS__); break;  \
-            case 25604: hipLaunchKernelGGL((KER<256, 4>), grid, dim3(256), 0, s, __VA_ARGS__); break;  \
-            case 25608: hipLaunchKernelGGL((KER<256, 8>), grid, dim3(256), 0, s, __VA_ARGS__); break;  \
-            case 51201: hipLaunchKernelGGL((KER<512, 1>), grid, dim3(512), 0, s, __VA_ARGS__); break;  \
-            case 51202: hipLaunchKernelGGL((KER<512, 2>), grid, dim3(512), 0, s, __VA_ARGS__); break;  \
-            case 51204: hipLaunchKernelGGL((KER<512, 4>), grid, dim3(512), 0, s, __VA_ARGS__); break;  \
-            case 51208: hipLaunchKernelGGL((KER<512, 8>), grid, dim3(512), 0, s, __VA_ARGS__); break;  \
-            default: break;                                                                            \
-        }                                                                                              \
+#define STEP_DISPATCH(KER, ...)                                                                           \
+    do {                                                                                                 \
+        const dim3 grid(L.total_blocks);                                                                 \
+        switch (L.shape.threads * 100 + L.shape.ppt) {                                                   \
+            case 25601: hipLaunchKernelGGL((KER<256, 1>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
+            case 25602: hipLaunchKernelGGL((KER<256, 2>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
+            case 25604: hipLaunchKernelGGL((KER<256, 4>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
+            case 51201: hipLaunchKernelGGL((KER<512, 1>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
+            case 51202: hipLaunchKernelGGL((KER<512, 2>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
+            case 51204: hipLaunchKernelGGL((KER<512, 4>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
+            default: break;                                                                              \
+        }                                                                                                \
     } while (0)
 
-void launch_step_a(const StepLaunch& L, int level, int iter, int next_level, int break_next, const TrackParams& P,
-                   int mode, uint8_t* dbg_mask, hipStream_t s)
+void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_level, const TrackParams& P, int mode,
+                   uint8_t* dbg_mask, hipStream_t s)
 {
-    STEP_DISPATCH(k_step_a, L.pairs, L.blk_pair, level, iter, next_level, break_next, P, mode, dbg_mask);
+    STEP_DISPATCH(k_step_a, L.pairs, L.blk_pair, level, iter, k, prev_level, P, mode, dbg_mask);
 }
 
-void launch_step_b(const StepLaunch& L, int level, int iter, int next_level, const TrackParams& P, int mode,
-                   hipStream_t s)
+void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s)
 {
-    STEP_DISPATCH(k_step_b, L.pairs, L.blk_pair, level, iter, next_level, P, mode);
+    STEP_DISPATCH(k_step_b, L.pairs, L.blk_pair, level, iter, k, P, mode);
 }
 
 // ======================================================= begin / end
-// RGBDOdometryef.cpp:278-282, 386-401: pose prior, its inverse, the
-// increment seeded with the SO3 rotation, constants of the first iteration.
-__global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P, int first_level)
+// RGBDOdometryef.cpp:278-282, 386-401: pose prior and its inverse, and the
+// state before the first iteration (the increment seeded with the SO3
+// rotation) in slot 1.
+__global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs) return;
-    const PairDesc& D = pairs[p];
-    TrackState* st = D.st;
+    TrackState* st = pairs[p].st;
     for (int a = 0; a < 3; ++a) {
         for (int b = 0; b < 3; ++b) st->Rprev[a * 3 + b] = st->pose_out[a * 4 + b];
         st->tprev[a] = st->pose_out[a * 4 + 3];
     }
     inv3f(st->Rprev, st->Rprev_inv);
-    for (int a = 0; a < 9; ++a) st->Rcurr[a] = st->Rprev[a];
-    for (int a = 0; a < 3; ++a) st->tcurr[a] = st->tprev[a];
-    for (int a = 0; a < 16; ++a) st->resultRt[a] = (a % 5 == 0) ? 1.0 : 0.0;
+    IterState& S = st->it[1];
+    for (int a = 0; a < 9; ++a) S.Rcurr[a] = st->Rprev[a];
+    for (int a = 0; a < 3; ++a) S.tcurr[a] = st->tprev[a];
+    for (int a = 0; a < 16; ++a) S.resultRt[a] = (a % 5 == 0) ? 1.0 : 0.0;
     if (P.so3)
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) st->resultRt[a * 4 + b] = st->so3R[a * 3 + b];
-    st->break_level = -1;
-    st->iters_done = 0;
-    st->degenerate = 0;
-    st->reverted = 0;
-    st->res0 = 0.0f;
-    st->res1 = 0.0f;
-    st->ticket_a = 0;
-    st->ticket_b = 0;
-    if (first_level >= 0) iter_constants(st, D.lv[first_level]);
+            for (int b = 0; b < 3; ++b) S.resultRt[a * 4 + b] = st->so3R[a * 3 + b];
+    for (int a = 0; a < 36; ++a) S.lastA[a] = st->lastA[a];
+    for (int a = 0; a < 6; ++a) S.lastb[a] = st->lastb[a];
+    S.lastICPError = st->lastICPError;
+    S.lastICPCount = st->lastICPCount;
+    S.lastRGBError = st->lastRGBError;
+    S.lastRGBCount = st->lastRGBCount;
+    S.res0 = 0.0f;
+    S.res1 = 0.0f;
+    S.skip = 0;
+    S.break_level = -1;
+    S.pending = 0;
+    S.broke = 0;
+    S.degenerate = 0;
+    S.iters_done = 0;
+    S.rgb_err_cmp = FLT_MAX;
 }
 
-void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, int first_level, hipStream_t s)
+void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_track_begin, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, P, first_level);
+    hipLaunchKernelGGL(k_track_begin, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, P);
 }
 
-// RGBDOdometryef.cpp:579-583, 593-594
-__global__ void k_track_end(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
+// Finish the last iteration, then RGBDOdometryef.cpp:579-583, 593-594: the
+// 0.3 m revert and the result.  One workgroup per pair.
+__global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ pairs, TrackParams P, int K,
+                                                   int last_level)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    TrackState* st = pairs[p].st;
+    __shared__ Prolog<256> W;
+    const PairDesc& D = pairs[blockIdx.x];
+    TrackState* st = D.st;
+    finish_previous<256>(D, P, K, last_level, W);
+    if (threadIdx.x != 0) return;
+    const IterState& S = W.S;
     float R[9], t[3];
-    for (int a = 0; a < 9; ++a) R[a] = st->Rcurr[a];
-    for (int a = 0; a < 3; ++a) t[a] = st->tcurr[a];
+    for (int a = 0; a < 9; ++a) R[a] = S.Rcurr[a];
+    for (int a = 0; a < 3; ++a) t[a] = S.tcurr[a];
+    st->reverted = 0;
     if (P.rgb) {
         const float d0 = t[0] - st->tprev[0], d1 = t[1] - st->tprev[1], d2 = t[2] - st->tprev[2];
         if (sqrtf(d0 * d0 + d1 * d1 + d2 * d2) > 0.3f) {
@@ -1011,11 +1673,24 @@ __global__ void k_track_end(const PairDesc* __restrict__ pairs, int npairs, Trac
         for (int b = 0; b < 3; ++b) st->pose_out[a * 4 + b] = R[a * 3 + b];
         st->pose_out[a * 4 + 3] = t[a];
     }
+    for (int a = 0; a < 36; ++a) st->lastA[a] = S.lastA[a];
+    for (int a = 0; a < 6; ++a) st->lastb[a] = S.lastb[a];
+    st->lastICPError = S.lastICPError;
+    st->lastICPCount = S.lastICPCount;
+    st->lastRGBError = S.lastRGBError;
+    st->lastRGBCount = S.lastRGBCount;
+    st->iters_done = S.iters_done;
+    st->degenerate = S.degenerate;
+    if (st->pose_log && st->pose_log_cap > 0) {
+        float* dst = st->pose_log + (size_t)(st->frame_count % st->pose_log_cap) * 12;
+        for (int a = 0; a < 12; ++a) dst[a] = st->pose_out[a];
+    }
+    st->frame_count++;
 }
 
-void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s)
+void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_track_end, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, P);
+    hipLaunchKernelGGL(k_track_end, dim3(npairs), dim3(256), 0, s, pairs, P, K, last_level);
 }
 
 // ================================================================= SO3
@@ -1084,7 +1759,7 @@ template <int NT>
 __device__ __forceinline__ void so3_block_total(float (&acc)[kSO3], float (*red)[kSO3], double* tot)
 {
     wave_sum<kSO3>(acc);
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 63)
         for (int k = 0; k < kSO3; ++k) red[threadIdx.x >> 6][k] = acc[k];
     __syncthreads();
     if (threadIdx.x < kSO3) {
@@ -1095,10 +1770,14 @@ __device__ __forceinline__ void so3_block_total(float (&acc)[kSO3], float (*red)
     __syncthreads();
 }
 
-constexpr int kSO3Threads = 1024;
+constexpr int kSO3Threads = 512;
 
+// With LDS, both level-2 images are staged in LDS (dynamic shared memory of
+// 2 * max(w*h) bytes) before the iterations.
+template <bool LDS>
 __global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict__ pairs)
 {
+    extern __shared__ uint8_t s_img[];
     __shared__ float s_red[kSO3Threads / 64][kSO3];
     __shared__ double s_tot[kSO3];
     __shared__ float sH[9], sKi[9], sKR[9];
@@ -1106,9 +1785,19 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict_
     const PairDesc& D = pairs[blockIdx.x];
     TrackState* st = D.st;
     const LevelDesc& L = D.lv[2];
-    const int w = L.w, h = L.h;
+    const int w = L.w, h = L.h, n = w * h;
+    const uint8_t* last = D.so3_last;
+    const uint8_t* next = D.so3_next;
+    if (LDS) {
+        for (int i = threadIdx.x; i < n; i += kSO3Threads) {
+            s_img[i] = last[i];
+            s_img[n + i] = next[i];
+        }
+        last = s_img;
+        next = s_img + n;
+    }
     double K[9], Ki[9];
-    kmat(L.fx, L.fy, L.cx, L.cy, K, Ki);
+    for (int k = 0; k < 9; ++k) { K[k] = L.K[k]; Ki[k] = L.Ki[k]; }
     // loop state, meaningful in thread 0
     float R_lr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     double resultR[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -1123,13 +1812,15 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict_
             for (int k = 0; k < 9; ++k) { sH[k] = (float)Hd[k]; sKi[k] = (float)Ki[k]; sKR[k] = (float)KR[k]; }
         }
         __syncthreads();
+        float Hr[9], Kir[9], KRr[9];
+        for (int k = 0; k < 9; ++k) { Hr[k] = sH[k]; Kir[k] = sKi[k]; KRr[k] = sKR[k]; }
         float acc[kSO3];
 #pragma unroll
         for (int k = 0; k < kSO3; ++k) acc[k] = 0.0f;
-        for (int idx = threadIdx.x; idx < w * h; idx += kSO3Threads) {
+        for (int idx = threadIdx.x; idx < n; idx += kSO3Threads) {
             const int y = idx / w, x = idx - y * w;
             float row[4];
-            if (so3_pixel(D.so3_last, D.so3_next, w, h, x, y, sH, sKi, sKR, row)) acc_row4(acc, row);
+            if (so3_pixel(last, next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
         }
         so3_block_total<kSO3Threads>(acc, s_red, s_tot);
         if (threadIdx.x == 0) {
@@ -1153,7 +1844,7 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict_
                 for (int k = 0; k < 9; ++k) lastResultR[k] = resultR[k];
                 double A3[9], b3[3], delta[3], rot[9];
                 unpack_so3(s_tot, A3, b3);
-                ldlt_solve(3, A3, b3, delta);
+                ldlt_solve<3>(A3, b3, delta);
                 rodrigues(delta, rot);
                 float rotf[9], nR[9];
                 for (int k = 0; k < 9; ++k) rotf[k] = (float)rot[k];
@@ -1171,10 +1862,15 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict_
     }
 }
 
-void launch_so3(const PairDesc* pairs, int npairs, int mode, hipStream_t s)
+constexpr int kSO3MaxLds = 60 * 1024;
+
+void launch_so3(const PairDesc* pairs, int npairs, int max_px_l2, hipStream_t s)
 {
-    (void)mode;
-    hipLaunchKernelGGL(k_so3, dim3(npairs), dim3(kSO3Threads), 0, s, pairs);
+    const int lds = 2 * max_px_l2;
+    if (lds <= kSO3MaxLds)
+        hipLaunchKernelGGL((k_so3<true>), dim3(npairs), dim3(kSO3Threads), lds, s, pairs);
+    else
+        hipLaunchKernelGGL((k_so3<false>), dim3(npairs), dim3(kSO3Threads), 0, s, pairs);
 }
 
 // One SO3 step with explicit constants (H, K^-1, K R as 27 floats): sums in

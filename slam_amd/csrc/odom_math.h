@@ -99,48 +99,88 @@ ODOM_HD void kmat(float fx, float fy, float cx, float cy, double* K, double* Ki)
 // Symmetric solve, LDL^T with diagonal pivoting on the largest remaining
 // |diagonal| and a pseudo-inverse of D (|d| <= DBL_MIN gives a zero
 // component): the semantics of Eigen's LDLT::solve used by the reference
-// (RGBDOdometryef.cpp:366, 544).  n <= 6.  Returns the number of zero pivots.
-ODOM_HD int ldlt_solve(int n, const double* Ain, const double* b, double* x)
+// (RGBDOdometryef.cpp:366, 544).  Fully unrolled for a fixed N, with the
+// pivot swaps done by compile-time-indexed selects, so the arrays stay in
+// registers on the device.  Returns the number of zero pivots.
+template <int N>
+ODOM_HD int ldlt_solve(const double* Ain, const double* b, double* x)
 {
-    double A[36];
-    int perm[6];
-    double y[6];
+    double A[N][N];
+    int perm[N];
+    double y[N];
     int zeros = 0;
-    for (int i = 0; i < n * n; ++i) A[i] = Ain[i];
-    for (int i = 0; i < n; ++i) perm[i] = i;
-    for (int k = 0; k < n; ++k) {
-        int p = k;
-        double best = fabs(A[k * n + k]);
-        for (int i = k + 1; i < n; ++i)
-            if (fabs(A[i * n + i]) > best) { best = fabs(A[i * n + i]); p = i; }
-        if (p != k) {
-            for (int j = 0; j < n; ++j) { double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
-            for (int i = 0; i < n; ++i) { double t = A[i * n + k]; A[i * n + k] = A[i * n + p]; A[i * n + p] = t; }
-            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
-        }
-        const double d = A[k * n + k];
-        if (fabs(d) <= DBL_MIN) {
-            for (int i = k + 1; i < n; ++i) A[i * n + k] = 0.0;
-            continue;
-        }
-        for (int i = k + 1; i < n; ++i) A[i * n + k] = A[i * n + k] / d;
-        for (int i = k + 1; i < n; ++i)
-            for (int j = k + 1; j <= i; ++j) {
-                A[i * n + j] = A[i * n + j] - A[i * n + k] * A[j * n + k] * d;
-                A[j * n + i] = A[i * n + j];
-            }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        perm[i] = i;
+#pragma unroll
+        for (int j = 0; j < N; ++j) A[i][j] = Ain[i * N + j];
     }
-    for (int i = 0; i < n; ++i) y[i] = b[perm[i]];
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < i; ++j) y[i] = y[i] - A[i * n + j] * y[j];
-    for (int i = 0; i < n; ++i) {
-        const double d = A[i * n + i];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int p = k;
+        double best = fabs(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const bool g = fabs(A[i][i]) > best;
+            best = g ? fabs(A[i][i]) : best;
+            p = g ? i : p;
+        }
+#pragma unroll
+        for (int q = k + 1; q < N; ++q) {
+            if (p == q) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) { const double t = A[k][j]; A[k][j] = A[q][j]; A[q][j] = t; }
+#pragma unroll
+                for (int i = 0; i < N; ++i) { const double t = A[i][k]; A[i][k] = A[i][q]; A[i][q] = t; }
+                const int t = perm[k]; perm[k] = perm[q]; perm[q] = t;
+            }
+        }
+        const double d = A[k][k];
+        if (fabs(d) <= DBL_MIN) {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) A[i][k] = 0.0;
+        } else {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) A[i][k] = A[i][k] / d;
+#pragma unroll
+            for (int i = k + 1; i < N; ++i)
+#pragma unroll
+                for (int j = k + 1; j <= i; ++j) {
+                    A[i][j] = A[i][j] - A[i][k] * A[j][k] * d;
+                    A[j][i] = A[i][j];
+                }
+        }
+    }
+    // y = P b
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < N; ++q) v = (perm[i] == q) ? b[q] : v;
+        y[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) y[i] = y[i] - A[i][j] * y[j];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double d = A[i][i];
         if (fabs(d) > DBL_MIN) y[i] = y[i] / d;
         else { y[i] = 0.0; ++zeros; }
     }
-    for (int i = n - 1; i >= 0; --i)
-        for (int j = i + 1; j < n; ++j) y[i] = y[i] - A[j * n + i] * y[j];
-    for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i)
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) y[i] = y[i] - A[j][i] * y[j];
+    // x = P^T y
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) v = (perm[i] == q) ? y[i] : v;
+        x[q] = v;
+    }
     return zeros;
 }
 

@@ -57,6 +57,7 @@ EXPORTS = [
     "odom_get_pose", "odom_synchronize", "odom_get_covariance", "odom_get_last_system", "odom_get_stats",
     "odom_icp_step", "odom_compute_derivatives", "odom_rgb_residual", "odom_rgb_step", "odom_so3_step",
     "odom_read_buffer", "odom_set_step_shape", "odom_get_step_shape", "odom_autotune", "odom_device_info",
+    "odom_init_icp_model_current", "odom_profile_track", "odom_set_pose_log",
 ]
 
 _V = ctypes.c_void_p
@@ -97,6 +98,10 @@ def lib() -> ctypes.CDLL:
     L.odom_get_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]
     L.odom_autotune.argtypes = [_V, ctypes.c_int, ctypes.c_float]
     L.odom_device_info.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.odom_init_icp_model_current.argtypes = [_V, _P(FrameView), _P(FrameView), ctypes.c_float]
+    L.odom_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     _V, _V, _V, _V, _P(ctypes.c_float)]
+    L.odom_set_pose_log.argtypes = [_V, _V, ctypes.c_int]
     L._typed = True
     return L
 
@@ -212,6 +217,14 @@ class RGBDOdometry:
                lib().odom_init_icp_model(self._h, ctypes.byref(v), ctypes.byref(n), depthCutoff, _p(pose)))
         self._keep_model = (kv, kn)
 
+    def initICPModelCurrent(self, predictedVertices, predictedNormals, depthCutoff: float) -> None:
+        """initICPModel with the device-resident current pose as modelPose."""
+        v, kv = frame_view(predictedVertices, FMT_RGBA32F)
+        n, kn = frame_view(predictedNormals, FMT_RGBA32F)
+        _check("odom_init_icp_model_current",
+               lib().odom_init_icp_model_current(self._h, ctypes.byref(v), ctypes.byref(n), depthCutoff))
+        self._keep_model = (kv, kn)
+
     def initRGB(self, rgb) -> None:
         v, k = frame_view(rgb, _rgb_fmt(rgb))
         _check("odom_init_rgb", lib().odom_init_rgb(self._h, ctypes.byref(v)))
@@ -250,6 +263,9 @@ class RGBDOdometry:
         R = np.zeros(9, np.float32)
         _check("odom_get_pose", lib().odom_get_pose(self._h, _p(t), _p(R)))
         return t, R.reshape(3, 3)
+
+    def set_pose_log(self, dev_ptr: int, capacity: int) -> None:
+        _check("odom_set_pose_log", lib().odom_set_pose_log(self._h, dev_ptr, capacity))
 
     def synchronize(self) -> None:
         _check("odom_synchronize", lib().odom_synchronize(self._h))
@@ -343,6 +359,17 @@ class RGBDOdometry:
         _check("odom_get_step_shape", lib().odom_get_step_shape(self._h, kernel, level, ctypes.byref(t),
                                                                 ctypes.byref(p)))
         return t.value, p.value
+
+    def profile_track(self, rgbOnly=False, icpWeight=10.0, pyramid=True, fastOdom=False, so3=True) -> dict:
+        """One tracking call with HIP events around every step launch."""
+        ua, ub = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        na, nb = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        tot = ctypes.c_float()
+        _check("odom_profile_track", lib().odom_profile_track(self._h, int(rgbOnly), float(icpWeight), int(pyramid),
+                                                              int(fastOdom), int(so3), _p(ua), _p(ub), _p(na),
+                                                              _p(nb), ctypes.byref(tot)))
+        return {"us_a": ua.tolist(), "us_b": ub.tolist(), "n_a": na.tolist(), "n_b": nb.tolist(),
+                "us_total": tot.value}
 
     def autotune(self, rgbOnly=False, icpWeight=10.0) -> None:
         _check("odom_autotune", lib().odom_autotune(self._h, int(rgbOnly), float(icpWeight)))

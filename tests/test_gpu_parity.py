@@ -275,13 +275,13 @@ def test_step_shapes_agree(pair):
     T = pair["T1"]
     Rc, tc, Rpi, tp = _constants(pair["T0"], T, 0, cam)
     base, mb = gpu.icp_step(0, Rc, tc, Rpi, tp)
-    for t in (128, 256, 512):
-        for q in (1, 2, 4, 8):
+    for t in (256, 512):
+        for q in (1, 2, 4):
             gpu.set_step_shape(0, 0, t, q)
             out, m = gpu.icp_step(0, Rc, tc, Rpi, tp)
             assert np.array_equal(m, mb)
             assert sums_close(out, base) < SUM_RTOL
-    gpu.set_step_shape(0, 0, 256, 4)
+    gpu.set_step_shape(0, 0, 256, 2)
 
 
 def test_device_memory_inputs(orc):
