@@ -630,6 +630,7 @@ struct orc_odom {
     orc_dataterm *corres[NPYR];
     float lastICPError, lastICPCount, lastRGBError, lastRGBCount, lastSO3Error, lastSO3Count;
     int so3_iters;
+    double so3R[9];                 /* resultR after the SO3 loop (identity when off) */
     double lastA[36], lastb[6];
 };
 
@@ -826,6 +827,8 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
         }
     }
 
+    memcpy(o->so3R, resultR, sizeof resultR);
+
     int iterations[NPYR];
     iterations[0] = fastOdom ? 3 : 10;
     iterations[1] = pyramid ? 5 : 0;
@@ -971,6 +974,11 @@ void orc_get_stats(const orc_odom *o, float s[7])
 {
     s[0] = o->lastICPError; s[1] = o->lastICPCount; s[2] = o->lastRGBError; s[3] = o->lastRGBCount;
     s[4] = o->lastSO3Error; s[5] = o->lastSO3Count; s[6] = (float)o->so3_iters;
+}
+
+void orc_get_so3_rotation(const orc_odom *o, double R[9])
+{
+    memcpy(R, o->so3R, sizeof o->so3R);
 }
 
 void orc_get_last_system(const orc_odom *o, double A[36], double b[6])

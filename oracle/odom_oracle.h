@@ -159,6 +159,8 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
  * lastSO3Error, lastSO3Count; stats[6] = number of SO3 iterations run. */
 void orc_get_stats(const orc_odom *o, float stats[7]);
 void orc_get_last_system(const orc_odom *o, double A[36], double b[6]);
+/* rotation found by the SO3 pre-alignment of the last track call (:294-380) */
+void orc_get_so3_rotation(const orc_odom *o, double R[9]);
 /* RGBDOdometryef.cpp:597-600: lastA^-1 (Gauss-Jordan with partial pivoting) */
 int orc_get_covariance(const orc_odom *o, double cov[36]);
 /* access to internal pyramids for tests; level 0..2 */

@@ -59,6 +59,7 @@ def lib():
         L.orc_get_last_system.argtypes = [_V, _V, _V]
         L.orc_get_covariance.argtypes = [_V, _V]
         L.orc_get_covariance.restype = ctypes.c_int
+        L.orc_get_so3_rotation.argtypes = [_V, _V]
         for n, rt in (("orc_buf_f32", _F), ("orc_buf_u8", _U8), ("orc_buf_s16", _S16), ("orc_buf_u16", _U16)):
             getattr(L, n).restype = rt
             getattr(L, n).argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
@@ -336,6 +337,11 @@ class Odometry:
         keys = ["lastICPError", "lastICPCount", "lastRGBError", "lastRGBCount", "lastSO3Error",
                 "lastSO3Count", "so3Iterations"]
         return dict(zip(keys, s.tolist()))
+
+    def so3_rotation(self):
+        R = np.zeros(9, np.float64)
+        lib().orc_get_so3_rotation(self._h, _p(R))
+        return R.reshape(3, 3)
 
     def last_system(self):
         A = np.zeros(36, np.float64)

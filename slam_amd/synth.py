@@ -170,9 +170,14 @@ def spline_trajectory(n: int, seed: int, max_step_m: float = 0.01, max_step_deg:
     pos *= max_step_m / dp
     Rs = [so3_exp(a) for a in ang]
     if n > 1:
-        dr = max(np.degrees(np.linalg.norm(so3_log(Rs[i].T @ Rs[i + 1]))) for i in range(n - 1))
-        ang *= max_step_deg / dr
-        Rs = [so3_exp(a) for a in ang]
+        # scale to the bound; the step angle is not linear in the scale, so
+        # shrink again while it overshoots
+        for it in range(5):
+            dr = max(np.degrees(np.linalg.norm(so3_log(Rs[i].T @ Rs[i + 1]))) for i in range(n - 1))
+            if it > 0 and dr <= max_step_deg:
+                break
+            ang *= max_step_deg / dr * (1.0 - 1e-6)
+            Rs = [so3_exp(a) for a in ang]
     # bound the excursion so the camera stays well inside the room
     ext = np.abs(pos).max()
     if ext > 0.8:

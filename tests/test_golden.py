@@ -1,0 +1,86 @@
+"""The committed fixture tests/golden/pair96.npz (a seeded 96x72 pair and the
+oracle's results on it, written by tests/golden/make_golden.py): the oracle
+must keep reproducing it, and the HIP path must match it."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from slam_amd import synth
+
+from helpers import CUTOFF, MAXD, pose_of
+
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pair96.npz")
+CAM = synth.Camera(96, 72, 72.0, 72.0, 47.5, 35.5)
+MODES = {"joint": dict(icpWeight=10.0, so3=True), "icp": dict(icpWeight=100.0, so3=False),
+         "rgbonly": dict(icpWeight=10.0, so3=True, rgbOnly=True)}
+
+
+@pytest.fixture(scope="module")
+def g():
+    with np.load(PATH, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _feed(odo, g):
+    odo.initFirstRGB(g["in_rgb0"])
+    odo.initICPModel(g["in_vert4"], g["in_norm4"], CUTOFF, g["in_T0"].astype(np.float32))
+    odo.initRGBModel(g["in_rgba"])
+    odo.initICP(g["out_bilateral"], CUTOFF)
+    odo.initRGB(g["in_rgb1"])
+
+
+def _track(odo, g, mode):
+    T0 = g["in_T0"]
+    return odo.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32),
+                                            **MODES[mode])
+
+
+def test_oracle_reproduces_fixture(orc, g):
+    assert np.array_equal(orc.bilateral(g["in_depth1"], MAXD), g["out_bilateral"])
+    for mode in MODES:
+        for fp64 in (False, True):
+            o = orc.Odometry(CAM.width, CAM.height, CAM.cx, CAM.cy, CAM.fx, CAM.fy, fp64=fp64)
+            _feed(o, g)
+            if mode == "joint" and not fp64:
+                for lv in range(3):
+                    assert np.array_equal(~np.isnan(o.buf("vmap_curr", lv)[0]), g[f"out_vmask{lv}"])
+                    assert np.array_equal(~np.isnan(o.buf("nmap_curr", lv)[0]), g[f"out_nmask{lv}"])
+                    assert np.array_equal(o.buf("nextImage", lv), g[f"out_nextImage{lv}"])
+            t, R = _track(o, g, mode)
+            tag = f"{mode}_{'f64' if fp64 else 'f32'}"
+            # the oracle's OpenMP partial sums may combine in another order on
+            # another core count: 1e-6 m / 1e-6 on the rotation entries
+            np.testing.assert_allclose(t, g[f"out_t_{tag}"], atol=1e-6)
+            np.testing.assert_allclose(R, g[f"out_R_{tag}"], atol=1e-6)
+            A, _ = o.last_system()
+            np.testing.assert_allclose(A, g[f"out_A_{tag}"], rtol=1e-4, atol=1e-9 * np.abs(A).max())
+
+
+def test_fixture_pair_is_tracked(g):
+    # the fixture is a meaningful case: the joint FP64 result is closer to the
+    # rendered truth than the start pose
+    T0, T1 = g["in_T0"], g["in_T1"]
+    e_res = synth.pose_error(pose_of(g["out_t_joint_f64"], g["out_R_joint_f64"]), T1)
+    e_start = synth.pose_error(T0, T1)
+    assert e_res[0] < 0.5 * e_start[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", list(MODES))
+def test_hip_matches_fixture(g, mode):
+    from slam_amd.odometry import RGBDOdometry
+    odo = RGBDOdometry(CAM.width, CAM.height, CAM.cx, CAM.cy, CAM.fx, CAM.fy)
+    assert np.array_equal(odo.bilateral(g["in_depth1"], MAXD), g["out_bilateral"])
+    _feed(odo, g)
+    for lv in range(3):
+        assert np.array_equal(~np.isnan(odo.read_buffer("vmap_curr", lv)[0]), g[f"out_vmask{lv}"]), lv
+        assert np.array_equal(~np.isnan(odo.read_buffer("nmap_curr", lv)[0]), g[f"out_nmask{lv}"]), lv
+        assert np.array_equal(odo.read_buffer("next_image", lv), g[f"out_nextImage{lv}"]), lv
+    t, R = _track(odo, g, mode)
+    Tg = pose_of(t, R)
+    for prec in ("f32", "f64"):
+        dt, dr = synth.pose_error(Tg, pose_of(g[f"out_t_{mode}_{prec}"], g[f"out_R_{mode}_{prec}"]))
+        assert dt <= 1e-3 and dr <= 0.1, (mode, prec, dt, dr)

@@ -1,0 +1,373 @@
+"""Pins of the oracle's Gauss-Newton steps, solve and tracker (a14-a23) on the CPU.
+
+The expected values come from closed forms on planes and identity warps, from
+a finite-difference derivative of the warp, from numpy/scipy solvers and
+rotation routines, and from the ground-truth motion of the rendered pairs --
+never from retyping the oracle's own per-pixel formula.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from helpers import CUTOFF, MAXD, feed, make_pair, pose_of
+from slam_amd import synth
+
+W, H = 40, 30
+K0 = (36.0, 36.0, 19.5, 14.5)
+ANGLE = math.sin(20.0 * 3.14159254 / 180.0)
+I3 = np.eye(3, dtype=np.float32)
+Z3 = np.zeros(3, np.float32)
+
+
+def kin(orc, level=0):
+    return orc.level_intr(orc.intr(*K0), level)
+
+
+def plane_maps(orc, depth_mm=2000):
+    d = np.full((H, W), depth_mm, np.uint16)
+    vm = orc.vmap(d, kin(orc), 20.0)
+    return vm, orc.nmap(vm)
+
+
+# ------------------------------------------------------------------ a19 packing
+def test_unpack_se3_packing_order(orc):
+    A, b = orc.unpack_se3(np.arange(29, dtype=np.float64))
+    # rows aa..ag, bb..bg, cc..cg, dd..dg, ee..eg, ff fg (types.cuh JtJJtrSE3)
+    s = 0
+    for i in range(6):
+        for j in range(i, 7):
+            if j == 6:
+                assert b[i] == s
+            else:
+                assert A[i, j] == s and A[j, i] == s
+            s += 1
+    assert s == 27
+
+
+# ------------------------------------------------------------------ a14 ICP
+def test_icp_identity_static_scene(orc):
+    """Model == current and identity poses: every valid pixel associates with
+    itself, so the residual and b vanish and the inlier count is the number
+    of pixels with a valid vertex and normal."""
+    cam = synth.Camera(W, H, *K0)
+    f = synth.Scene(3, 20).render(synth.base_pose(), cam, noise_seed=5)
+    vm = orc.vmap(f.depth, kin(orc), 20.0)
+    nm = orc.nmap(vm)
+    for fp64 in (False, True):
+        sums, mask = orc.icp_step(vm, nm, vm, nm, kin(orc), I3, Z3, I3, Z3, fp64=fp64)
+        valid = ~np.isnan(vm[0]) & ~np.isnan(nm[0])
+        assert mask.sum() == valid.sum() == sums[28]
+        A, b = orc.unpack_se3(sums)
+        assert sums[27] == 0.0 and np.all(b == 0.0)
+        assert np.all(mask.astype(bool) == valid)
+        ev = np.linalg.eigvalsh(A)
+        assert ev.min() > -1e-9 * ev.max()
+
+
+def test_icp_translated_plane_closed_form(orc):
+    """Model plane shifted by delta along z: residual n.(s - d) = -n_z delta for
+    every inlier, J = [n, s x n]; A, b and the residual sum are closed forms in
+    the current vertices."""
+    delta = 0.05
+    vm, nm = plane_maps(orc)
+    vp = vm.copy()
+    vp[2] += delta
+    sums, mask = orc.icp_step(vm, nm, vp, nm, kin(orc), I3, Z3, I3, Z3, fp64=True)
+    inl = mask.astype(bool)
+    valid = ~np.isnan(nm[0])
+    assert np.array_equal(inl, valid)
+    N = inl.sum()
+    x, y = vm[0][inl].astype(np.float64), vm[1][inl].astype(np.float64)
+    z = vm[2][inl].astype(np.float64)
+    n = nm[:, inl].astype(np.float64)
+    assert np.allclose(n[:2], 0.0) and np.allclose(np.abs(n[2]), 1.0)
+    nz = n[2]
+    s = np.stack([x, y, z])
+    J = np.concatenate([n, np.cross(s.T, n.T).T])
+    r = -nz * delta
+    A, b = orc.unpack_se3(sums)
+    np.testing.assert_allclose(A, J @ J.T, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(b, J @ r, rtol=1e-6, atol=1e-9)
+    assert abs(sums[27] - N * delta * delta) < 1e-5 * N * delta * delta   # z + delta is float32
+    assert sums[28] == N
+    # J = [0, 0, nz, y nz, -x nz, 0] on this plane
+    np.testing.assert_allclose(A[2, 3], (y * nz * nz).sum(), rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(b[4], (x * nz * nz * delta).sum(), rtol=1e-6, atol=1e-9)
+
+
+def test_icp_distance_and_angle_gates(orc):
+    vm, nm = plane_maps(orc)
+    far = vm.copy()
+    far[2] += 0.11                                       # beyond 0.1 m
+    _, mask = orc.icp_step(vm, nm, far, nm, kin(orc), I3, Z3, I3, Z3)
+    assert mask.sum() == 0
+    # model normals tilted by 25 degrees about x fail sin(20 deg)
+    c, s_ = math.cos(math.radians(25)), math.sin(math.radians(25))
+    Rx = np.array([[1, 0, 0], [0, c, -s_], [0, s_, c]])
+    tilted = np.einsum("ij,jhw->ihw", Rx, nm.astype(np.float64)).astype(np.float32)
+    _, mask = orc.icp_step(vm, nm, vm, tilted, kin(orc), I3, Z3, I3, Z3)
+    assert mask.sum() == 0
+    c, s_ = math.cos(math.radians(15)), math.sin(math.radians(15))
+    Rx = np.array([[1, 0, 0], [0, c, -s_], [0, s_, c]])
+    tilted = np.einsum("ij,jhw->ihw", Rx, nm.astype(np.float64)).astype(np.float32)
+    _, mask = orc.icp_step(vm, nm, vm, tilted, kin(orc), I3, Z3, I3, Z3)
+    assert mask.sum() == (~np.isnan(nm[0])).sum()
+
+
+def test_icp_fp32_matches_fp64(orc):
+    cam = synth.Camera(W, H, *K0)
+    sc = synth.Scene(8, 20)
+    f0 = sc.render(synth.base_pose(), cam, noise_seed=1)
+    f1 = sc.render(synth.perturb(synth.base_pose(), np.random.default_rng(8), 0.02, 1.0), cam, noise_seed=2)
+    vc = orc.vmap(f1.depth, kin(orc), 20.0)
+    nc = orc.nmap(vc)
+    vp = orc.vmap(f0.depth, kin(orc), 20.0)
+    npv = orc.nmap(vp)
+    s32, m32 = orc.icp_step(vc, nc, vp, npv, kin(orc), I3, Z3, I3, Z3)
+    s64, m64 = orc.icp_step(vc, nc, vp, npv, kin(orc), I3, Z3, I3, Z3, fp64=True)
+    assert (m32 != m64).sum() <= 2
+    np.testing.assert_allclose(s32[:27], s64[:27], rtol=2e-3, atol=1e-6 * np.abs(s64[:27]).max())
+
+
+# ------------------------------------------------------------------ a12/a15 RGB residual
+def _rgb_inputs(orc, seed=21):
+    rng = np.random.default_rng(seed)
+    img = rng.integers(20, 240, size=(H, W)).astype(np.uint8)
+    dx, dy = orc.derivative(img)
+    depth = np.full((H, W), 2.0, np.float32)
+    return img, dx, dy, depth
+
+
+def test_rgb_residual_identity_gives_zero_difference(orc):
+    img, dx, dy, depth = _rgb_inputs(orc)
+    minScale = 64.0
+    corres, count, sig = orc.rgb_residual(minScale, dx, dy, depth, depth, img, img, I3, Z3)
+    v = corres["valid"].astype(bool)
+    assert count == v.sum() > 0 and sig == 0
+    assert np.all(corres["diff"][v] == 0.0)
+    # with the identity warp every valid pixel maps to itself
+    assert np.all(corres["zero_x"][v] == corres["one_x"][v])
+    assert np.all(corres["zero_y"][v] == corres["one_y"][v])
+    # the validity conditions: j < cols-5, i < rows-1, gradient >= minScale
+    ii, jj = np.nonzero(v)
+    assert jj.max() < W - 5 and ii.max() < H - 1
+    g2 = dx.astype(np.float64) ** 2 + dy.astype(np.float64) ** 2
+    assert np.all(g2[v] >= minScale)
+
+
+def test_rgb_residual_constant_offset(orc):
+    img, dx, dy, depth = _rgb_inputs(orc)
+    last = (img - 3).astype(np.uint8)
+    corres, count, sig = orc.rgb_residual(64.0, dx, dy, depth, depth, last, img, I3, Z3)
+    v = corres["valid"].astype(bool)
+    assert count > 0 and np.all(corres["diff"][v] == 3.0) and sig == 9 * count
+
+
+def test_rgb_residual_one_pixel_translation(orc):
+    """kt = (2, 0, 0) at depth 2 moves every pixel one column right in the last
+    image; a last image shifted by one column gives zero differences."""
+    img, dx, dy, depth = _rgb_inputs(orc)
+    last = np.zeros_like(img)
+    last[:, 1:] = img[:, :-1]
+    last[:, 0] = 1
+    kt = np.array([2.0, 0.0, 0.0], np.float32)
+    corres, count, sig = orc.rgb_residual(64.0, dx, dy, depth, depth, last, img, I3, kt)
+    v = corres["valid"].astype(bool)
+    assert count > 0 and sig == 0
+    assert np.all(corres["zero_x"][v] == corres["one_x"][v] + 1)
+    # a depth step beyond 0.07 m rejects the correspondence
+    far = depth + 0.08
+    _, count2, _ = orc.rgb_residual(64.0, dx, dy, far, depth, last, img, I3, kt)
+    assert count2 == 0
+
+
+# ------------------------------------------------------------------ a16 RGB step
+def _single_term(zero, one, diff):
+    from oracle.oracle import DATATERM
+    c = np.zeros((H, W), DATATERM)
+    c["valid"][one[1], one[0]] = 1
+    c["zero_x"][one[1], one[0]] = zero[0]
+    c["zero_y"][one[1], one[0]] = zero[1]
+    c["one_x"][one[1], one[0]] = one[0]
+    c["one_y"][one[1], one[0]] = one[1]
+    c["diff"][one[1], one[0]] = diff
+    return c
+
+
+def test_rgb_step_jacobian_is_warp_derivative(orc):
+    """One correspondence with w = 1 (sigma = -1): b = J r, r = -diff.  J must
+    be the derivative of g . pi(P + t + omega x P) at zero motion, with
+    g = sobelScale * (dIdx, dIdy) at the next pixel and P the last point."""
+    zero, one, diff, Z = (23, 11), (22, 12), 5.0, 1.7
+    depth = np.full((H, W), np.nan, np.float32)
+    depth[zero[1], zero[0]] = Z
+    dx = np.zeros((H, W), np.int16)
+    dy = np.zeros((H, W), np.int16)
+    dx[one[1], one[0]], dy[one[1], one[0]] = 37, -52
+    k = kin(orc)
+    sums = orc.rgb_step(_single_term(zero, one, diff), -1.0, depth, k, dx, dy)
+    A, b = orc.unpack_se3(sums)
+    J = b / (-diff)
+    np.testing.assert_allclose(A, np.outer(J, J), rtol=1e-5, atol=1e-8)
+    assert sums[28] == 1 and abs(sums[27] - diff * diff) < 1e-9
+    P = np.array([(zero[0] - k.cx) * Z / k.fx, (zero[1] - k.cy) * Z / k.fy, Z])
+    g = 0.125 * np.array([37.0, -52.0])
+
+    def proj(X):
+        return np.array([k.fx * X[0] / X[2] + k.cx, k.fy * X[1] / X[2] + k.cy])
+
+    eps = 1e-6
+    Jfd = np.zeros(6)
+    for q in range(6):
+        e = np.zeros(3)
+        e[q % 3] = eps
+        Xp = P + e if q < 3 else P + np.cross(e, P)
+        Xm = P - e if q < 3 else P - np.cross(e, P)
+        Jfd[q] = g @ (proj(Xp) - proj(Xm)) / (2 * eps)
+    np.testing.assert_allclose(J, Jfd, rtol=1e-4, atol=1e-6)
+
+
+def test_rgb_step_weights(orc):
+    zero, one, diff, Z = (10, 10), (10, 10), 4.0, 2.0
+    depth = np.full((H, W), 2.0, np.float32)
+    dx = np.zeros((H, W), np.int16)
+    dy = np.zeros((H, W), np.int16)
+    dx[10, 10] = 40
+    k = kin(orc)
+    c = _single_term(zero, one, diff)
+    s1 = orc.rgb_step(c, -1.0, depth, k, dx, dy)
+    s2 = orc.rgb_step(c, 6.0, depth, k, dx, dy)
+    w = 1.0 / (6.0 + abs(diff))
+    np.testing.assert_allclose(s2[:27], s1[:27] * w * w, rtol=1e-5)
+    assert abs(s2[27] - (w * diff) ** 2) < 1e-7
+    # a zero gradient gives a zero system
+    s0 = orc.rgb_step(c, 6.0, depth, k, np.zeros_like(dx), np.zeros_like(dy))
+    A, b = orc.unpack_se3(s0)
+    assert np.all(A == 0) and np.all(b == 0)
+
+
+# ------------------------------------------------------------------ a17 SO3 step
+def test_so3_identity(orc):
+    rng = np.random.default_rng(31)
+    img = rng.integers(0, 256, size=(H, W)).astype(np.uint8)
+    k = kin(orc)
+    Kf = np.array([[k.fx, 0, k.cx], [0, k.fy, k.cy], [0, 0, 1]], np.float32)
+    Ki = np.linalg.inv(Kf.astype(np.float64)).astype(np.float32)
+    sums, mask = orc.so3_step(img, img, I3, Ki, Kf)
+    # packing (JtJJtrSO3): A00 A01 A02 b0, A11 A12 b1, A22 b2, r^2, count
+    assert sums[9] == 0.0 and np.all(sums[[3, 6, 8]] == 0.0)
+    assert sums[10] == mask.sum() == (W - 2) * (H - 2)
+
+
+# ------------------------------------------------------------------ a20 solve + rodrigues
+def test_ldlt_matches_numpy_on_spd(orc):
+    rng = np.random.default_rng(41)
+    for n in (3, 6):
+        for _ in range(20):
+            M = rng.normal(size=(n, n))
+            A = M @ M.T + 0.1 * np.eye(n)
+            b = rng.normal(size=n)
+            x, z = orc.ldlt_solve(A, b)
+            assert z == 0
+            np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+
+
+def test_ldlt_pseudo_inverse_of_zero_pivots(orc):
+    d = np.array([4.0, 0.0, 9.0, 0.0, 1.0, 2.0])
+    b = np.ones(6)
+    x, z = orc.ldlt_solve(np.diag(d), b)
+    assert z == 2
+    np.testing.assert_allclose(x, np.where(d > 0, 1.0 / np.where(d > 0, d, 1), 0.0))
+    x, z = orc.ldlt_solve(np.zeros((6, 6)), b)
+    assert z == 6 and np.all(x == 0)
+    # indefinite symmetric matrices are solved exactly too (pivoted LDL^T)
+    A = np.array([[0.0, 2, 0], [2, 0, 1], [0, 1, 3]])
+    x, z = orc.ldlt_solve(A, np.array([1.0, 2, 3]))
+    np.testing.assert_allclose(A @ x, [1, 2, 3], atol=1e-12)
+
+
+def test_rodrigues(orc):
+    rng = np.random.default_rng(43)
+    for _ in range(20):
+        r = rng.normal(size=3) * rng.uniform(1e-6, 2.0)
+        R = orc.rodrigues(r)
+        np.testing.assert_allclose(R, Rotation.from_rotvec(r).as_matrix(), atol=1e-12)
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+        np.testing.assert_allclose(R @ r, r, atol=1e-12)
+    assert np.array_equal(orc.rodrigues(np.array([1e-17, 0.0, 0.0])), np.eye(3))
+
+
+# ------------------------------------------------------------------ a20-a23 tracker
+@pytest.mark.parametrize("seed", [1, 2])
+def test_tracker_recovers_rendered_motion_icp_only(orc, seed):
+    cam = synth.CAM_320
+    T0, T1, f0, f1 = make_pair(cam, seed)
+    o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    feed(o, f0, f1, T0, orc.bilateral(f1.depth, MAXD))
+    t, R = o.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32),
+                                          icpWeight=100.0, so3=False)
+    et, er = synth.pose_error(pose_of(t, R), T1)
+    mt, mr = synth.pose_error(T0, T1)
+    assert et < 1.5e-3 and er < 0.05
+    assert et < 0.1 * mt
+    st = o.stats()
+    assert st["lastRGBCount"] == 0 and st["so3Iterations"] == 0
+    cov = o.getCovariance()
+    A, _ = o.last_system()
+    np.testing.assert_allclose(A @ cov, np.eye(6), atol=1e-8)
+
+
+def test_tracker_joint_fp32_close_to_fp64_and_truth(orc):
+    cam = synth.CAM_320
+    T0, T1, f0, f1 = make_pair(cam, 3)
+    filt = orc.bilateral(f1.depth, MAXD)
+    res = []
+    for fp64 in (False, True):
+        o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, fp64=fp64)
+        feed(o, f0, f1, T0, filt)
+        t, R = o.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32))
+        res.append(pose_of(t, R))
+    et, er = synth.pose_error(res[0], res[1])
+    assert et < 1e-4 and er < 0.01
+    gt, gr = synth.pose_error(res[1], T1)
+    assert gt < 6e-3 and gr < 0.2
+
+
+def test_so3_prealignment_recovers_pure_rotation(orc):
+    cam = synth.CAM_320
+    scene = synth.Scene(4, 20)
+    T0 = synth.base_pose()
+    a = np.array([0.3, -0.8, 0.2])
+    a /= np.linalg.norm(a)
+    T1 = T0 @ synth.make_pose(synth.so3_exp(a * math.radians(1.5)), np.zeros(3))
+    f0 = scene.render(T0, cam, noise_seed=1)
+    f1 = scene.render(T1, cam, noise_seed=2)
+    o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    feed(o, f0, f1, T0, orc.bilateral(f1.depth, MAXD))
+    t, R = o.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32))
+    Rrel = (np.linalg.inv(T0) @ T1)[:3, :3]
+    # the SO3 result seeds resultRt, the inverse of the camera increment
+    err = np.degrees(np.linalg.norm(synth.so3_log(o.so3_rotation() @ Rrel)))
+    assert err < 0.5
+    assert o.stats()["so3Iterations"] >= 2
+    et, er = synth.pose_error(pose_of(t, R), T1)
+    assert et < 2e-3 and er < 0.05
+
+
+def test_cross_call_state(orc):
+    """vmaps_tmp coupling: nextDepth is the model's vertex depth (maxDepthRGB
+    6.0), and the so3 pass swaps lastNextImage and nextImage at the end."""
+    cam = synth.CAM_320
+    T0, T1, f0, f1 = make_pair(cam, 6)
+    o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    feed(o, f0, f1, T0, orc.bilateral(f1.depth, MAXD))
+    np.testing.assert_array_equal(o.buf("nextDepth", 0), orc.vertices_to_depth(f0.vert4, 6.0))
+    nxt = o.buf("nextImage", 0).copy()
+    assert np.array_equal(nxt, orc.intensity(f1.rgb))
+    o.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32), so3=True)
+    assert np.array_equal(o.buf("lastNextImage", 0), nxt)
+    assert np.array_equal(o.buf("nextImage", 0), orc.intensity(f0.rgb))
