@@ -992,201 +992,79 @@ __device__ void iter_constants(IterState& S, const LevelDesc& L)
 }
 
 // RGBDOdometryef.cpp:533-575: combine, solve, computeUpdateSE3, pose update.
+// Evaluated redundantly by every lane of one wave from registers (no lane
+// exchanges: the factorisation is a chain of dependent FP64 operations that
+// cross-lane shuffles would only lengthen); lane 0 stores the results.
 // rgb_sums: FP64 totals of the photometric pass (unused without it).
-__device__ void solve_update(IterState& S, const TrackParams& P, const double* rgb_sums, const float* Rprev,
-                             const float* tprev)
+__device__ void solve_update_uniform(IterState& S, const TrackParams& P, const double* icp_sums,
+                                     const double* rgb_sums, const float* Rprev, const float* tprev)
 {
-    double Ai[36], bi[6], Ar[36], br[6], A[36], b[6], x[6];
-    for (int k = 0; k < 36; ++k) { Ai[k] = 0; Ar[k] = 0; }
-    for (int k = 0; k < 6; ++k) { bi[k] = 0; br[k] = 0; }
-    if (P.icp) unpack_se3(S.icp_sums, Ai, bi);
-    if (P.rgb) unpack_se3(rgb_sums, Ar, br);
-    if (P.icp && P.rgb) {
-        const double wgt = P.icpWeight;
-        for (int k = 0; k < 36; ++k) A[k] = Ar[k] + wgt * wgt * Ai[k];
-        for (int k = 0; k < 6; ++k) b[k] = br[k] + wgt * bi[k];
-    } else if (P.icp) {
-        for (int k = 0; k < 36; ++k) A[k] = Ai[k];
-        for (int k = 0; k < 6; ++k) b[k] = bi[k];
-    } else {
-        for (int k = 0; k < 36; ++k) A[k] = Ar[k];
-        for (int k = 0; k < 6; ++k) b[k] = br[k];
-    }
-    for (int k = 0; k < 36; ++k) S.lastA[k] = A[k];
-    for (int k = 0; k < 6; ++k) S.lastb[k] = b[k];
-    const int zeros = ldlt_solve<6>(A, b, x);
-    S.degenerate = (zeros == 6);
-    double rot[9], upd[16], nrt[16], old[16];
-    rodrigues(x + 3, rot);
-    for (int a = 0; a < 16; ++a) { upd[a] = 0.0; old[a] = S.resultRt[a]; }
-    for (int a = 0; a < 3; ++a) {
-        for (int c = 0; c < 3; ++c) upd[a * 4 + c] = rot[a * 3 + c];
-        upd[a * 4 + 3] = x[a];
-    }
-    upd[15] = 1.0;
-    mm4d(upd, old, nrt);
-    for (int a = 0; a < 16; ++a) S.resultRt[a] = nrt[a];
-    float Rf[9], tf[3], Ri[9], ti[3], Rn[9];
-    for (int a = 0; a < 3; ++a) {
-        for (int c = 0; c < 3; ++c) Rf[a * 3 + c] = (float)nrt[a * 4 + c];
-        tf[a] = (float)nrt[a * 4 + 3];
-    }
-    for (int a = 0; a < 3; ++a)
-        for (int c = 0; c < 3; ++c) Ri[a * 3 + c] = Rf[c * 3 + a];
-    for (int a = 0; a < 3; ++a) ti[a] = -(Ri[a * 3] * tf[0] + Ri[a * 3 + 1] * tf[1] + Ri[a * 3 + 2] * tf[2]);
-    mm3f(Rprev, Ri, Rn);
-    for (int a = 0; a < 9; ++a) S.Rcurr[a] = Rn[a];
-    for (int a = 0; a < 3; ++a)
-        S.tcurr[a] = Rprev[a * 3] * ti[0] + Rprev[a * 3 + 1] * ti[1] + Rprev[a * 3 + 2] * ti[2] + tprev[a];
-    S.iters_done++;
-}
-
-// The same solve and update, run by one full wave (64 lanes): lane l < 36
-// holds A[l/6][l%6], lanes 36..41 hold b.  Pivot search, the symmetric row
-// and column swaps, the column scaling and the rank-1 update of each step are
-// lane-parallel; every element sees the operations of the scalar routine in
-// the same order, so the result is the same.
-__device__ __forceinline__ int packed_index(int i, int j)
-{
-    return i * 7 - (i * (i - 1)) / 2 + (j - i);
-}
-
-__device__ void solve_update_wave(IterState& S, const TrackParams& P, const double* icp_sums,
-                                  const double* rgb_sums, const float* Rprev, const float* tprev)
-{
-    const int l = threadIdx.x & 63;
-    const int ri = l / 6, ci = l - 6 * (l / 6);
-    const bool inA = l < 36;
-    // combined system (RGBDOdometryef.cpp:539-557)
-    double v = 0.0;
-    if (l < 42) {
-        const int s = inA ? packed_index(min(ri, ci), max(ri, ci)) : packed_index(l - 36, 6);
-        const double ai = P.icp ? icp_sums[s] : 0.0;
-        const double ar = P.rgb ? rgb_sums[s] : 0.0;
-        if (P.icp && P.rgb) {
-            const double wgt = P.icpWeight;
-            v = inA ? ar + wgt * wgt * ai : ar + wgt * ai;
-        } else {
-            v = P.icp ? ai : ar;
-        }
-        if (inA) S.lastA[l] = v;
-        else S.lastb[l - 36] = v;
-    }
-    double a = inA ? v : 0.0;
-    const double bv = (l >= 36 && l < 42) ? v : 0.0;
-    int pv = l;
+    const bool lead = (threadIdx.x & 63) == 0;
+    double A[36], b[6], x[6];
+    const double wgt = P.icpWeight;
+    int s = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        int p = k;
-        double best = fabs(__shfl(a, k * 7));
+    for (int i = 0; i < 6; ++i)
 #pragma unroll
-        for (int i = k + 1; i < 6; ++i) {
-            const double di = fabs(__shfl(a, i * 7));
-            if (di > best) {
-                best = di;
-                p = i;
+        for (int j = i; j < 7; ++j, ++s) {
+            const double ai = P.icp ? icp_sums[s] : 0.0;
+            const double ar = P.rgb ? rgb_sums[s] : 0.0;
+            double v;
+            if (P.icp && P.rgb) v = (j == 6) ? ar + wgt * ai : ar + wgt * wgt * ai;
+            else v = P.icp ? ai : ar;
+            if (j == 6) {
+                b[i] = v;
+            } else {
+                A[i * 6 + j] = v;
+                A[j * 6 + i] = v;
             }
         }
-        if (p != k) {
-            const int si = (ri == k) ? p : (ri == p ? k : ri);
-            const int sj = (ci == k) ? p : (ci == p ? k : ci);
-            const double an = __shfl(a, inA ? si * 6 + sj : l);
-            a = inA ? an : a;
-            const int pn = __shfl(pv, (l == k) ? p : (l == p ? k : l));
-            pv = (l < 6) ? pn : pv;
-        }
-        const double d = __shfl(a, k * 7);
-        const bool colk = inA && ri > k && ci == k;
-        if (fabs(d) <= DBL_MIN) {
-            if (colk) a = 0.0;
-        } else {
-            if (colk) a = a / d;
-            const int hi = max(ri, ci), lo = min(ri, ci);
-            const double lhi = __shfl(a, inA ? hi * 6 + k : 0);
-            const double llo = __shfl(a, inA ? lo * 6 + k : 0);
-            if (inA && ri > k && ci > k) a = a - lhi * llo * d;
-        }
+    if (lead) {
+#pragma unroll
+        for (int q = 0; q < 36; ++q) S.lastA[q] = A[q];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) S.lastb[q] = b[q];
     }
-    // y = P b, forward substitution, D^+ , backward substitution, x = P^T y
-    double y = __shfl(bv, 36 + ((l < 6) ? pv : 0));
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-        const double yj = __shfl(y, j);
-        const double lij = __shfl(a, (l < 6) ? l * 6 + j : 0);
-        if (l < 6 && l > j) y = y - lij * yj;
-    }
-    const double dd = __shfl(a, (l < 6) ? l * 7 : 0);
-    int zero = 0;
-    if (l < 6) {
-        if (fabs(dd) > DBL_MIN) y = y / dd;
-        else { y = 0.0; zero = 1; }
-    }
-    const int zeros = __popcll(__ballot(zero));
-#pragma unroll
-    for (int i = 4; i >= 0; --i)
-#pragma unroll
-        for (int j = i + 1; j < 6; ++j) {
-            const double yj = __shfl(y, j);
-            const double aji = __shfl(a, j * 6 + i);
-            if (l == i) y = y - aji * yj;
-        }
-    double x[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) x[q] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        const int pi = __shfl(pv, i);
-        const double yi = __shfl(y, i);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) x[q] = (pi == q) ? yi : x[q];
-    }
-    // computeUpdateSE3 (utils.h:54-74) and the pose update (RGBDOdometryef.cpp:563-575),
-    // evaluated identically in every lane
+    const int zeros = ldlt_solve<6>(A, b, x);
     double rot[9];
     rodrigues(x + 3, rot);
-    double upd[16];
+    double old[16], nrt[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) upd[q] = 0.0;
+    for (int q = 0; q < 16; ++q) old[q] = S.resultRt[q];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) upd[r * 4 + c] = rot[r * 3 + c];
-        upd[r * 4 + 3] = x[r];
-    }
-    upd[15] = 1.0;
-    const double oc = S.resultRt[l & 15];
-    const int r4 = (l >> 2) & 3, c4 = l & 3;
-    const double o0 = __shfl(oc, c4), o1 = __shfl(oc, 4 + c4), o2 = __shfl(oc, 8 + c4), o3 = __shfl(oc, 12 + c4);
-    double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 1.0;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        u0 = (r4 == r) ? upd[r * 4] : u0;
-        u1 = (r4 == r) ? upd[r * 4 + 1] : u1;
-        u2 = (r4 == r) ? upd[r * 4 + 2] : u2;
-        u3 = (r4 == r) ? upd[r * 4 + 3] : u3;
-    }
-    const double nrt = u0 * o0 + u1 * o1 + u2 * o2 + u3 * o3;
-    float Rf[9], tf[3];
+        for (int c = 0; c < 4; ++c) {
+            if (r == 3) {
+                nrt[r * 4 + c] = old[12 + c];
+            } else {
+                nrt[r * 4 + c] = rot[r * 3] * old[c] + rot[r * 3 + 1] * old[4 + c] + rot[r * 3 + 2] * old[8 + c] +
+                                 x[r] * old[12 + c];
+            }
+        }
+    float Rf[9], tf[3], Ri[9], ti[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = (float)__shfl(nrt, r * 4 + c);
-        tf[r] = (float)__shfl(nrt, r * 4 + 3);
+        for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = (float)nrt[r * 4 + c];
+        tf[r] = (float)nrt[r * 4 + 3];
     }
-    if (l < 16) S.resultRt[l] = nrt;
-    float Ri[9], ti[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Ri[r * 3 + c] = Rf[c * 3 + r];
 #pragma unroll
     for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r * 3] * tf[0] + Ri[r * 3 + 1] * tf[1] + Ri[r * 3 + 2] * tf[2]);
-    if (l < 9) {
-        const int r = l / 3, c = l - 3 * (l / 3);
-        S.Rcurr[l] = Rprev[r * 3] * Ri[c] + Rprev[r * 3 + 1] * Ri[3 + c] + Rprev[r * 3 + 2] * Ri[6 + c];
-    }
-    if (l < 3) S.tcurr[l] = Rprev[l * 3] * ti[0] + Rprev[l * 3 + 1] * ti[1] + Rprev[l * 3 + 2] * ti[2] + tprev[l];
-    if (l == 0) {
+    if (lead) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) S.resultRt[q] = nrt[q];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                S.Rcurr[r * 3 + c] = Rprev[r * 3] * Ri[c] + Rprev[r * 3 + 1] * Ri[3 + c] + Rprev[r * 3 + 2] * Ri[6 + c];
+            S.tcurr[r] = Rprev[r * 3] * ti[0] + Rprev[r * 3 + 1] * ti[1] + Rprev[r * 3 + 2] * ti[2] + tprev[r];
+        }
         S.degenerate = (zeros == 6);
         S.iters_done++;
     }
@@ -1244,10 +1122,104 @@ __device__ __forceinline__ void copy_words(T* dst, const T* src)
 }
 
 // Scratch shared by the prologues.
+// The pose the call started from, held in registers from the first
+// instructions of a step kernel so no later phase waits on its loads.
+struct PrevPose {
+    float R[9], t[3], Ri[9];
+};
+
+__device__ __forceinline__ void load_prev(const TrackState* st, PrevPose& pp)
+{
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        pp.R[q] = st->Rprev[q];
+        pp.Ri[q] = st->Rprev_inv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pp.t[q] = st->tprev[q];
+}
+
+// The block partials of a finished pass, loaded as float4 chunks: thread t
+// owns chunk t % 8 (floats 4c..4c+3 of a record) of the records t / 8,
+// t / 8 + NT / 8, ...  The loads are issued together with the state copy so
+// both complete in one memory round trip.
+template <int NT>
+struct PartLoad {
+    static constexpr int RG = NT / 8;           // record groups
+    static constexpr int IT = kMaxBlk / RG;     // records per thread and pass
+    float4 v[2][IT];
+};
+
+template <int NT>
+__device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int na, const float* __restrict__ pb,
+                                               int nb, PartLoad<NT>& L)
+{
+    constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT;
+    const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const float* part = a ? pb : pa;
+        const int n = a ? nb : na;
+        if (part) {
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const int r = rg + i * RG;
+                L.v[a][i] = *(const float4*)(part + (size_t)(r < n ? r : 0) * kPartStride + 4 * c);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < IT; ++i) L.v[a][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+// FP64 totals of the loaded partials in a fixed order: per thread over its
+// records, then over the record groups in LDS.  tot[0..28] pass A, tot[29..57]
+// pass B.
+template <int NT>
+__device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int nb, double* tot,
+                                             double (*red)[NT / 8])
+{
+    constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT;
+    const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int n = a ? nb : na;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const bool ok = rg + i * RG < n;
+            s0 += ok ? (double)L.v[a][i].x : 0.0;
+            s1 += ok ? (double)L.v[a][i].y : 0.0;
+            s2 += ok ? (double)L.v[a][i].z : 0.0;
+            s3 += ok ? (double)L.v[a][i].w : 0.0;
+        }
+        red[a * 32 + 4 * c][rg] = s0;
+        red[a * 32 + 4 * c + 1][rg] = s1;
+        red[a * 32 + 4 * c + 2][rg] = s2;
+        red[a * 32 + 4 * c + 3][rg] = s3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int a = threadIdx.x >> 5, k = threadIdx.x & 31;
+        if (k < kSE3) {
+            double e = 0.0, o = 0.0;
+#pragma unroll 8
+            for (int g = 0; g < RG; g += 2) {
+                e += red[threadIdx.x][g];
+                o += red[threadIdx.x][g + 1];
+            }
+            tot[a * kSE3 + k] = e + o;
+        }
+    }
+    __syncthreads();
+}
+
 template <int NT>
 struct Prolog {
     IterState S;
     double fin[2 * kSE3][NT / 64];
+    double red[64][NT / 8];
     double tot[2 * kSE3];
     unsigned int redi[NT / 64][2];
     unsigned int toti[2];
@@ -1259,23 +1231,30 @@ struct Prolog {
 // pass B, and the solve.  Executed by every thread of the block; every block
 // computes the same state.
 template <int NT>
-__device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level, Prolog<NT>& W)
+__device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level, const PrevPose& pp,
+                                Prolog<NT>& W)
 {
     TrackState* st = D.st;
-    copy_words<NT>(&W.S, &st->it[(k - 1) & 1]);
-    __syncthreads();
-    if (k == 0) return;
+    if (k == 0) {
+        copy_words<NT>(&W.S, &st->it[1]);
+        __syncthreads();
+        return;
+    }
     const int pb = (k - 1) & 1;
+    const int na = D.nblk_a[prev_level], nb = D.nblk_b[prev_level];
+    PartLoad<NT> PL;
+    copy_words<NT>(&W.S, &st->it[pb]);
+    partials_issue<NT>(P.icp ? D.part_a[pb] : nullptr, na, P.rgb ? D.part_b[pb] : nullptr, nb, PL);
+    __syncthreads();
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
     if (run) {
-        partials_total2<NT>(P.icp ? D.part_a[pb] : nullptr, D.nblk_a[prev_level], P.rgb ? D.part_b[pb] : nullptr,
-                            D.nblk_b[prev_level], W.tot, W.fin);
+        partials_sum<NT>(PL, na, nb, W.tot, W.red);
         if (threadIdx.x < 64) {
             if (threadIdx.x == 0) {
                 if (!P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
                 icp_stats(W.S, P, W.tot);
             }
-            solve_update_wave(W.S, P, W.tot, W.tot + kSE3, st->Rprev, st->tprev);
+            solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t);
         }
     }
     if (threadIdx.x == 0) {
@@ -1399,9 +1378,11 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     // pose-independent loads of the first tile overlap the prologue
     TileA<NT, PPT> T;
     load_tile_a<NT, PPT>(L, P, lb * NT * PPT + threadIdx.x, N, P3, T);
+    PrevPose pp;
+    load_prev(st, pp);
 
     if (mode == 0) {
-        finish_previous<NT>(D, P, k, prev_level, W);
+        finish_previous<NT>(D, P, k, prev_level, pp, W);
         if (threadIdx.x == 0) {
             W.S.skip = (W.S.break_level == level);
             if (!W.S.skip) iter_constants(W.S, L);
@@ -1417,8 +1398,8 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
 
     IcpC ic;
     RgbC rc;
-    for (int q = 0; q < 9; ++q) { ic.Rc[q] = W.S.Rcurr[q]; ic.Rpi[q] = st->Rprev_inv[q]; rc.K[q] = W.S.krkinv[q]; }
-    for (int q = 0; q < 3; ++q) { ic.tc[q] = W.S.tcurr[q]; ic.tp[q] = st->tprev[q]; rc.kt[q] = W.S.kt[q]; }
+    for (int q = 0; q < 9; ++q) { ic.Rc[q] = W.S.Rcurr[q]; ic.Rpi[q] = pp.Ri[q]; rc.K[q] = W.S.krkinv[q]; }
+    for (int q = 0; q < 3; ++q) { ic.tc[q] = W.S.tcurr[q]; ic.tp[q] = pp.t[q]; rc.kt[q] = W.S.kt[q]; }
     ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
     ic.dist = P.distThres; ic.angle = P.angleThres;
     rc.maxDepthDelta = P.maxDepthDelta;
@@ -1654,7 +1635,9 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     __shared__ Prolog<256> W;
     const PairDesc& D = pairs[blockIdx.x];
     TrackState* st = D.st;
-    finish_previous<256>(D, P, K, last_level, W);
+    PrevPose pp;
+    load_prev(st, pp);
+    finish_previous<256>(D, P, K, last_level, pp, W);
     if (threadIdx.x != 0) return;
     const IterState& S = W.S;
     float R[9], t[3];

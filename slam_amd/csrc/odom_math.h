@@ -191,7 +191,13 @@ ODOM_HD void rodrigues(const double* src, double* R)
     double rx = src[0], ry = src[1], rz = src[2];
     const double theta = sqrt(rx * rx + ry * ry + rz * rz);
     if (theta >= DBL_EPSILON) {
-        const double c = cos(theta), s = sin(theta), c1 = 1. - c;
+#ifdef __HIP_DEVICE_COMPILE__
+        double s, c;
+        sincos(theta, &s, &c);
+#else
+        const double c = cos(theta), s = sin(theta);
+#endif
+        const double c1 = 1. - c;
         const double itheta = theta ? 1. / theta : 0.;
         rx *= itheta; ry *= itheta; rz *= itheta;
         const double rrt[9] = {rx * rx, rx * ry, rx * rz, rx * ry, ry * ry, ry * rz, rx * rz, ry * rz, rz * rz};
