@@ -78,6 +78,10 @@ struct odom_ctx {
     StepShape shape_a[kLevels], shape_b[kLevels];
     TrackState h_st{};
     bool have_state = false;
+    // call-order bookkeeping: which inputs have been provided at least once.
+    // vtmp: the model vertex buffer that initRGB/initRGBModel read
+    // (vmaps_tmp, RGBDOdometryef.cpp:208-235) has been filled.
+    bool icp_cur = false, icp_model = false, rgb_next = false, rgb_last = false, vtmp = false;
     hipError_t last_hip = hipSuccess;
 };
 
@@ -283,6 +287,12 @@ TrackParams make_params(odom_ctx* c, int rgbOnly, float icpWeight, int pyramid, 
     P.iters[1] = pyramid ? 5 : 0;
     P.iters[2] = pyramid ? 4 : 0;
     return P;
+}
+
+// The inputs a track call with these options reads have all been provided.
+bool inputs_ready(const odom_ctx* c, const TrackParams& P)
+{
+    return (!P.icp || (c->icp_cur && c->icp_model)) && (!P.rgb || (c->rgb_next && c->rgb_last));
 }
 
 StepLaunch step_launch(odom_ctx* c, int kernel, int level)
@@ -531,7 +541,9 @@ int odom_init_icp_depth(odom_handle c, const odom_frame_view* depth, float cutof
     if (rc) return rc;
     HIPCHK(c, hipMemcpy2DAsync(c->depth[0], (size_t)c->w[0] * 2, p, pitch, (size_t)c->w[0] * 2, c->h[0],
                                hipMemcpyDeviceToDevice, c->stream));
-    return enqueue_depth_pyramid_maps(c, cutoff);
+    rc = enqueue_depth_pyramid_maps(c, cutoff);
+    if (rc == ODOM_OK) c->icp_cur = true;
+    return rc;
 }
 
 int odom_init_icp_raw_depth(odom_handle c, const odom_frame_view* raw, float maxD, float cutoff)
@@ -542,14 +554,18 @@ int odom_init_icp_raw_depth(odom_handle c, const odom_frame_view* raw, float max
     int rc = resolve(c, raw, ODOM_FMT_DEPTH_U16, ODOM_FMT_DEPTH_U16, c->stage_u16, &p, &pitch, &fmt);
     if (rc) return rc;
     launch_bilateral((const uint16_t*)p, pitch / 2, c->depth[0], c->w[0], c->h[0], maxD, c->stream);
-    return enqueue_depth_pyramid_maps(c, cutoff);
+    rc = enqueue_depth_pyramid_maps(c, cutoff);
+    if (rc == ODOM_OK) c->icp_cur = true;
+    return rc;
 }
 
 int odom_init_icp_maps(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff)
 {
     (void)cutoff;
     if (!c) return ODOM_ERR_ARG;
-    return enqueue_model(c, v, n, nullptr, false);
+    const int rc = enqueue_model(c, v, n, nullptr, false);
+    if (rc == ODOM_OK) c->icp_cur = c->vtmp = true;
+    return rc;
 }
 
 int odom_init_icp_model(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff,
@@ -557,27 +573,37 @@ int odom_init_icp_model(odom_handle c, const odom_frame_view* v, const odom_fram
 {
     (void)cutoff;
     if (!c || !pose) return ODOM_ERR_ARG;
-    return enqueue_model(c, v, n, pose, true);
+    const int rc = enqueue_model(c, v, n, pose, true);
+    if (rc == ODOM_OK) c->icp_model = c->vtmp = true;
+    return rc;
 }
 
 int odom_init_icp_model_current(odom_handle c, const odom_frame_view* v, const odom_frame_view* n, float cutoff)
 {
     (void)cutoff;
     if (!c) return ODOM_ERR_ARG;
-    return enqueue_model(c, v, n, nullptr, true, true);
+    const int rc = enqueue_model(c, v, n, nullptr, true, true);
+    if (rc == ODOM_OK) c->icp_model = c->vtmp = true;
+    return rc;
 }
 
 int odom_init_rgb(odom_handle c, const odom_frame_view* rgb)
 {
     if (!c) return ODOM_ERR_ARG;
+    if (!c->vtmp) return ODOM_ERR_STATE;  // "initICP* must be called before initRGB*"
     uint8_t* dst[kLevels] = {next_img(c, 0), next_img(c, 1), next_img(c, 2)};
-    return enqueue_populate(c, rgb, c->nextDepth, dst);
+    const int rc = enqueue_populate(c, rgb, c->nextDepth, dst);
+    if (rc == ODOM_OK) c->rgb_next = true;
+    return rc;
 }
 
 int odom_init_rgb_model(odom_handle c, const odom_frame_view* rgb)
 {
     if (!c) return ODOM_ERR_ARG;
-    return enqueue_populate(c, rgb, c->lastDepth, c->lastImage);
+    if (!c->vtmp) return ODOM_ERR_STATE;
+    const int rc = enqueue_populate(c, rgb, c->lastDepth, c->lastImage);
+    if (rc == ODOM_OK) c->rgb_last = true;
+    return rc;
 }
 
 int odom_init_first_rgb(odom_handle c, const odom_frame_view* rgb)
@@ -634,6 +660,7 @@ int odom_track_async(odom_handle c, int rgbOnly, float icpWeight, int pyramid, i
     if (!c) return ODOM_ERR_ARG;
     const TrackParams P = make_params(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
     if (!P.icp && !P.rgb) return ODOM_ERR_ARG;  // RGBDOdometryef.cpp:558-561
+    if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     return enqueue_track(c, P);
 }
 
@@ -855,6 +882,7 @@ int odom_profile_track(odom_handle c, int rgbOnly, float icpWeight, int pyramid,
     if (!c || !us_a || !us_b || !n_a || !n_b || !us_total) return ODOM_ERR_ARG;
     const TrackParams P = make_params(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
     if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
@@ -901,6 +929,8 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
 {
     if (!c) return ODOM_ERR_ARG;
     TrackParams P = make_params(c, rgbOnly, icpWeight, 1, 0, 0);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     const int threads[] = {256, 512};
     const int ppts[] = {1, 2, 4};
     hipEvent_t e0, e1;

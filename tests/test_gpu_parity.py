@@ -269,6 +269,32 @@ def test_bad_arguments_raise():
         gpu.set_step_shape(0, 0, 96, 4)
 
 
+def test_call_order_is_checked():
+    """initRGB* read the model vertex buffer that initICPModel / initICP(v, n)
+    fill (elastic_fusion_file.cpp:461 "initICP* must be called before
+    initRGB*"), and a track call needs every input its mode reads."""
+    cam = synth.CAM_320
+    T0, T1, f0, f1 = make_pair(cam, 4)
+    gpu = mk(cam)
+    with pytest.raises(OdomError) as e:
+        gpu.initRGB(f1.rgb)
+    assert e.value.code == -3
+    with pytest.raises(OdomError) as e:
+        gpu.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 100.0, True, False, False)
+    assert e.value.code == -3
+    gpu.initFirstRGB(f0.rgb)
+    gpu.initICPModel(f0.vert4, f0.norm4, CUTOFF, T0.astype(np.float32))
+    gpu.initICP(f1.depth, CUTOFF)
+    # ICP-only is now complete; ICP+RGB still lacks both RGB inputs
+    gpu.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 100.0, True, False, False)
+    with pytest.raises(OdomError) as e:
+        gpu.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 10.0, True, False, True)
+    assert e.value.code == -3
+    gpu.initRGBModel(f0.rgba)
+    gpu.initRGB(f1.rgb)
+    gpu.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 10.0, True, False, True)
+
+
 def test_step_shapes_agree(pair):
     """Every launch shape of the autotuner gives the same mask and sums."""
     gpu, cam = pair["gpu"], pair["cam"]

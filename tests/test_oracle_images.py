@@ -359,3 +359,22 @@ def test_point_cloud_closed_form(orc):
     np.testing.assert_allclose(pc[0][ok], ((u - k1.cx) * 2.5 / k1.fx + 0 * v)[ok], rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(pc[1][ok], ((v - k1.cy) * 2.5 / k1.fy + 0 * u)[ok], rtol=1e-6, atol=1e-6)
     np.testing.assert_array_equal(pc[2][ok], 2.5)
+
+
+def test_gauss_pyramids_border_weight_shift(orc):
+    """pyrDownKernelGaussF / IntensityGauss (utils.cu:332-363, 470-500) index
+    the weights as (ty - cy - 1) * 5 + (tx - cx - 1) over rows cy < ty =
+    min(2y + 3, rows - 1).  On the first output row that gives source rows
+    0, 1, 2 the weights 6, 4, 1 (a correctly clipped Gaussian); on the last
+    one (2y = rows - 2) source rows rows-4, rows-3, rows-2 get 6, 4, 1 -- the
+    weights shift away from the centre.  A ramp in y exposes both."""
+    ramp = np.arange(H, dtype=np.float32)[:, None].repeat(W, 1) + 10.0
+    out = orc.pyrdown_f32(ramp)
+    top = (6 * 10.0 + 4 * 11.0 + 1 * 12.0) / 11.0
+    bottom = (6 * (H - 4) + 4 * (H - 3) + 1 * (H - 2)) / 11.0 + 10.0
+    np.testing.assert_allclose(out[0, 1:-1], top, rtol=1e-6)
+    np.testing.assert_allclose(out[-1, 1:-1], bottom, rtol=1e-6)
+    # interior rows stay centred
+    np.testing.assert_allclose(out[3, 1:-1], 16.0, rtol=1e-6)
+    out8 = orc.pyrdown_u8(ramp.astype(np.uint8))
+    assert np.all(out8[0, 1:-1] == int(top)) and np.all(out8[-1, 1:-1] == int(bottom))
