@@ -173,6 +173,35 @@ int odom_profile_track(odom_handle h, int rgb_only, float icp_weight, int pyrami
  * fastest per kernel and level. */
 int odom_autotune(odom_handle h, int rgb_only, float icp_weight);
 
+/* ---- batches: many frame pairs (handles, of any size) tracked by the same
+ * launches.  No counterpart in the reference, whose RGBDOdometryef tracks one
+ * pair per object and call (RGBDOdometryef.cpp:267-595); this is the batched
+ * form of that call.  Every launch covers every pair: the ICP / association
+ * and photometric passes of one Gauss-Newton iteration are one launch each
+ * over all pairs, with blocks shared out in proportion to each pair's pixel
+ * count.  The handles keep their own inputs (init* calls on each handle),
+ * state and results (odom_get_pose / odom_get_stats per handle).  All
+ * handles must share dist_thresh and angle_thresh. ---- */
+typedef struct odom_batch_s *odom_batch;
+int odom_batch_create(const odom_handle *handles, int n, odom_batch *out);
+int odom_batch_destroy(odom_batch b);
+/* getIncrementalTransformation on every pair of the batch, asynchronously on
+ * the batch's stream; each handle's stream waits for it, so per-handle calls
+ * made afterwards see the results. */
+int odom_track_batch(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3);
+int odom_batch_synchronize(odom_batch b);
+/* Blocks per step launch over the whole batch (default: 2 per compute unit),
+ * shared out by pixel count, at most 256 per pair. */
+int odom_batch_set_blocks(odom_batch b, int target_blocks);
+int odom_batch_get_blocks(odom_batch b, int kernel, int level, int *total_blocks);
+int odom_batch_set_step_shape(odom_batch b, int kernel, int level, int threads, int pixels_per_thread);
+/* Capture each distinct call (options and image parities) as a hipGraph on
+ * first use and replay it afterwards. */
+int odom_batch_set_graph(odom_batch b, int enable);
+/* As odom_profile_track, for one batched call. */
+int odom_batch_profile_track(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3,
+                             float us_a[3], float us_b[3], int n_a[3], int n_b[3], float *us_total);
+
 /* Device name / arch of the current device, for logs. */
 int odom_device_info(char *name, int name_len, char *arch, int arch_len);
 

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -965,6 +966,404 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ODOM_OK;
+}
+
+}  // extern "C"
+
+// ================================================================ batches
+// Many handles (frame pairs of any size) tracked by the same launches: one
+// launch per pass and Gauss-Newton iteration covers every pair, each pair
+// owning a share of the blocks proportional to its pixel count (at most
+// kMaxBlk).  The per-pair state, partials and descriptors are the handles'
+// own; the batch holds the descriptor arrays with its block split, the
+// block-to-pair tables and a stream.
+struct odom_batch_s {
+    std::vector<odom_ctx*> h;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> ready;
+    int target_blocks = 0;
+    StepShape shape[2][kLevels];
+    int* d_blk[2][kLevels] = {};
+    int total[2][kLevels] = {};
+    std::vector<int> nblk[2][kLevels], blk0[2][kLevels];
+    std::map<std::vector<int>, PairDesc*> slots;   // descriptor arrays by the pairs' image parities
+    PairDesc* h_stage = nullptr;                   // pinned upload buffer
+    bool use_graph = false;
+    struct G { std::vector<int> key; hipGraphExec_t exec; };
+    std::vector<G> graphs;
+    hipError_t last_hip = hipSuccess;
+};
+
+namespace {
+
+void batch_drop_graphs(odom_batch_s* b)
+{
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
+}
+
+void batch_drop_slots(odom_batch_s* b)
+{
+    for (auto& kv : b->slots) (void)hipFree(kv.second);
+    b->slots.clear();
+}
+
+// Split each pass and level's tiles over the pairs and upload the tables.
+int batch_plan(odom_batch_s* b)
+{
+    batch_drop_graphs(b);
+    batch_drop_slots(b);
+    const int n = (int)b->h.size();
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < kLevels; ++l) {
+            const int tile = b->shape[k][l].threads * b->shape[k][l].ppt;
+            std::vector<int> tiles(n);
+            long long T = 0;
+            for (int i = 0; i < n; ++i) {
+                tiles[i] = cdiv(b->h[i]->n[l], tile);
+                T += tiles[i];
+            }
+            auto& nb = b->nblk[k][l];
+            auto& b0 = b->blk0[k][l];
+            nb.assign(n, 1);
+            b0.assign(n, 0);
+            int tot = 0;
+            for (int i = 0; i < n; ++i) {
+                const double share = (double)tiles[i] * b->target_blocks / (double)std::max(T, 1LL);
+                nb[i] = std::max(1, std::min({(int)(share + 0.5), kMaxBlk, tiles[i]}));
+                b0[i] = tot;
+                tot += nb[i];
+            }
+            b->total[k][l] = tot;
+            std::vector<int> table(tot);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < nb[i]; ++j) table[b0[i] + j] = i;
+            if (b->d_blk[k][l]) (void)hipFree(b->d_blk[k][l]);
+            b->d_blk[k][l] = nullptr;
+            hipError_t e = hipMalloc((void**)&b->d_blk[k][l], sizeof(int) * tot);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->d_blk[k][l], table.data(), sizeof(int) * tot, hipMemcpyHostToDevice, b->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+            if (e != hipSuccess) {
+                b->last_hip = e;
+                return ODOM_ERR_HIP;
+            }
+        }
+    return ODOM_OK;
+}
+
+// The descriptor array for the pairs' current image parities (built once per
+// parity pattern: a batch tracked with so3 alternates between two).
+int batch_slot(odom_batch_s* b, PairDesc** out)
+{
+    const int n = (int)b->h.size();
+    std::vector<int> key(n);
+    for (int i = 0; i < n; ++i) key[i] = b->h[i]->parity;
+    auto it = b->slots.find(key);
+    if (it != b->slots.end()) {
+        *out = it->second;
+        return ODOM_OK;
+    }
+    for (int i = 0; i < n; ++i) {
+        PairDesc D = b->h[i]->h_pair[key[i]];
+        for (int l = 0; l < kLevels; ++l) {
+            D.nblk_a[l] = b->nblk[0][l][i];
+            D.blk0_a[l] = b->blk0[0][l][i];
+            D.nblk_b[l] = b->nblk[1][l][i];
+            D.blk0_b[l] = b->blk0[1][l][i];
+        }
+        b->h_stage[i] = D;
+    }
+    PairDesc* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, sizeof(PairDesc) * n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d, b->h_stage, sizeof(PairDesc) * n, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        if (d) (void)hipFree(d);
+        b->last_hip = e;
+        return ODOM_ERR_HIP;
+    }
+    b->slots[key] = d;
+    *out = d;
+    return ODOM_OK;
+}
+
+// The launches of one batched getIncrementalTransformation (the same
+// sequence as enqueue_track, over every pair at once).
+void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEvent_t* ev)
+{
+    const int n = (int)b->h.size();
+    int max_l2 = 0;
+    for (auto* c : b->h) max_l2 = std::max(max_l2, c->n[2]);
+    if (P.so3) launch_so3(dp, n, max_l2, b->stream);
+    std::vector<int> sched;
+    for (int l = kLevels - 1; l >= 0; --l)
+        for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
+    launch_track_begin(dp, n, P, b->stream);
+    int e = 0;
+    for (size_t k = 0; k < sched.size(); ++k) {
+        const int level = sched[k] / 64, iter = sched[k] % 64;
+        const int prev = k > 0 ? sched[k - 1] / 64 : -1;
+        StepLaunch La;
+        La.pairs = dp;
+        La.npairs = n;
+        La.blk_pair = b->d_blk[0][level];
+        La.total_blocks = b->total[0][level];
+        La.shape = b->shape[0][level];
+        if (ev) (void)hipEventRecord(ev[e++], b->stream);
+        launch_step_a(La, level, iter, (int)k, prev, P, 0, nullptr, b->stream);
+        if (ev) (void)hipEventRecord(ev[e++], b->stream);
+        if (P.rgb) {
+            StepLaunch Lb;
+            Lb.pairs = dp;
+            Lb.npairs = n;
+            Lb.blk_pair = b->d_blk[1][level];
+            Lb.total_blocks = b->total[1][level];
+            Lb.shape = b->shape[1][level];
+            if (ev) (void)hipEventRecord(ev[e++], b->stream);
+            launch_step_b(Lb, level, iter, (int)k, P, 0, b->stream);
+            if (ev) (void)hipEventRecord(ev[e++], b->stream);
+        }
+    }
+    const int K = (int)sched.size();
+    launch_track_end(dp, n, P, K, K > 0 ? sched[K - 1] / 64 : 0, b->stream);
+}
+
+#define BCHK(b, x)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) {                                                                          \
+            (b)->last_hip = e_;                                                                          \
+            fprintf(stderr, "slam_odom: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return ODOM_ERR_HIP;                                                                         \
+        }                                                                                                \
+    } while (0)
+
+// Everything before the batch's own launches: checks, the per-pair
+// derivative pass on each handle's stream, and the cross-stream waits.
+int batch_prepare(odom_batch_s* b, const TrackParams& P, PairDesc** dp)
+{
+    for (auto* c : b->h)
+        if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
+    for (size_t i = 0; i < b->h.size(); ++i) {
+        odom_ctx* c = b->h[i];
+        if (P.rgb) {
+            PrepLevels a{};
+            prep_levels(c, a);
+            launch_rgb_prep(a, c->stream);
+            if (int rc = check_launch(c)) return rc;
+        }
+        BCHK(b, hipEventRecord(b->ready[i], c->stream));
+        BCHK(b, hipStreamWaitEvent(b->stream, b->ready[i], 0));
+    }
+    return batch_slot(b, dp);
+}
+
+// Handles continue after the batch: their streams wait for it, and the image
+// swap of RGBDOdometryef.cpp:585-591 flips each pair's parity.
+int batch_finish(odom_batch_s* b, const TrackParams& P)
+{
+    BCHK(b, hipEventRecord(b->done, b->stream));
+    for (auto* c : b->h) {
+        BCHK(b, hipStreamWaitEvent(c->stream, b->done, 0));
+        if (P.so3) c->parity ^= 1;
+        c->have_state = false;
+    }
+    return ODOM_OK;
+}
+
+std::vector<int> graph_key(const TrackParams& P, const std::vector<int>& parities)
+{
+    std::vector<int> k = {P.icp, P.rgb, P.rgbOnly, P.so3, P.iters[0], P.iters[1], P.iters[2]};
+    int w;
+    memcpy(&w, &P.icpWeight, 4);
+    k.push_back(w);
+    k.insert(k.end(), parities.begin(), parities.end());
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int odom_batch_create(const odom_handle* handles, int n, odom_batch* out)
+{
+    if (!handles || !out || n <= 0 || n > 4096) return ODOM_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!handles[i]) return ODOM_ERR_ARG;
+        if (handles[i]->prm.dist_thresh != handles[0]->prm.dist_thresh ||
+            handles[i]->prm.angle_thresh != handles[0]->prm.angle_thresh)
+            return ODOM_ERR_ARG;  // one TrackParams serves every pair of a launch
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == handles[i]) return ODOM_ERR_ARG;
+    }
+    odom_batch_s* b = new odom_batch_s();
+    b->h.assign(handles, handles + n);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    b->target_blocks = 2 * cus;
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < kLevels; ++l) b->shape[k][l] = StepShape{256, 2};
+    b->ready.resize(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess &&
+              hipHostMalloc((void**)&b->h_stage, sizeof(PairDesc) * n, 0) == hipSuccess;
+    for (int i = 0; ok && i < n; ++i)
+        ok = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok || batch_plan(b) != ODOM_OK) {
+        odom_batch_destroy(b);
+        return ODOM_ERR_HIP;
+    }
+    *out = b;
+    return ODOM_OK;
+}
+
+int odom_batch_destroy(odom_batch b)
+{
+    if (!b) return ODOM_ERR_ARG;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    batch_drop_slots(b);
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < kLevels; ++l)
+            if (b->d_blk[k][l]) (void)hipFree(b->d_blk[k][l]);
+    for (auto e : b->ready)
+        if (e) (void)hipEventDestroy(e);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return ODOM_OK;
+}
+
+int odom_batch_set_blocks(odom_batch b, int target_blocks)
+{
+    if (!b || target_blocks < 1 || target_blocks > 1 << 20) return ODOM_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->target_blocks = target_blocks;
+    return batch_plan(b);
+}
+
+int odom_batch_set_step_shape(odom_batch b, int kernel, int level, int threads, int ppt)
+{
+    if (!b || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
+    if (!step_shape_supported(StepShape{threads, ppt})) return ODOM_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->shape[kernel][level] = StepShape{threads, ppt};
+    return batch_plan(b);
+}
+
+int odom_batch_get_blocks(odom_batch b, int kernel, int level, int* total_blocks)
+{
+    if (!b || !total_blocks || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
+    *total_blocks = b->total[kernel][level];
+    return ODOM_OK;
+}
+
+int odom_batch_set_graph(odom_batch b, int enable)
+{
+    if (!b) return ODOM_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->use_graph = enable != 0;
+    if (!b->use_graph) batch_drop_graphs(b);
+    return ODOM_OK;
+}
+
+int odom_track_batch(odom_batch b, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3)
+{
+    if (!b) return ODOM_ERR_ARG;
+    const TrackParams P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;  // RGBDOdometryef.cpp:558-561
+    PairDesc* dp = nullptr;
+    int rc = batch_prepare(b, P, &dp);
+    if (rc) return rc;
+    if (!b->use_graph) {
+        batch_launches(b, P, dp, nullptr);
+        hipError_t e = hipPeekAtLastError();
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            b->last_hip = e;
+            return ODOM_ERR_HIP;
+        }
+    } else {
+        std::vector<int> par;
+        for (auto* c : b->h) par.push_back(c->parity);
+        const std::vector<int> key = graph_key(P, par);
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : b->graphs)
+            if (g.key == key) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+            batch_launches(b, P, dp, nullptr);
+            BCHK(b, hipStreamEndCapture(b->stream, &graph));
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            BCHK(b, e);
+            b->graphs.push_back({key, exec});
+        }
+        BCHK(b, hipGraphLaunch(exec, b->stream));
+    }
+    return batch_finish(b, P);
+}
+
+int odom_batch_synchronize(odom_batch b)
+{
+    if (!b) return ODOM_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return ODOM_OK;
+}
+
+int odom_batch_profile_track(odom_batch b, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3,
+                             float us_a[3], float us_b[3], int n_a[3], int n_b[3], float* us_total)
+{
+    if (!b || !us_a || !us_b || !n_a || !n_b || !us_total) return ODOM_ERR_ARG;
+    const TrackParams P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    PairDesc* dp = nullptr;
+    int rc = batch_prepare(b, P, &dp);
+    if (rc) return rc;
+    std::vector<int> sched;
+    for (int l = kLevels - 1; l >= 0; --l)
+        for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
+    std::vector<hipEvent_t> ev(sched.size() * (P.rgb ? 4 : 2) + 2, nullptr);
+    for (auto& e : ev) BCHK(b, hipEventCreate(&e));
+    BCHK(b, hipEventRecord(ev[ev.size() - 2], b->stream));
+    batch_launches(b, P, dp, ev.data());
+    BCHK(b, hipEventRecord(ev[ev.size() - 1], b->stream));
+    BCHK(b, hipEventSynchronize(ev[ev.size() - 1]));
+    for (int l = 0; l < kLevels; ++l) { us_a[l] = 0; us_b[l] = 0; n_a[l] = 0; n_b[l] = 0; }
+    int e = 0;
+    for (size_t k = 0; k < sched.size(); ++k) {
+        const int l = sched[k];
+        float ms = 0;
+        BCHK(b, hipEventElapsedTime(&ms, ev[e], ev[e + 1]));
+        us_a[l] += ms * 1000.0f;
+        n_a[l]++;
+        e += 2;
+        if (P.rgb) {
+            BCHK(b, hipEventElapsedTime(&ms, ev[e], ev[e + 1]));
+            us_b[l] += ms * 1000.0f;
+            n_b[l]++;
+            e += 2;
+        }
+    }
+    for (int l = 0; l < kLevels; ++l) {
+        if (n_a[l]) us_a[l] /= n_a[l];
+        if (n_b[l]) us_b[l] /= n_b[l];
+    }
+    float ms = 0;
+    BCHK(b, hipEventElapsedTime(&ms, ev[ev.size() - 2], ev[ev.size() - 1]));
+    *us_total = ms * 1000.0f;
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    return batch_finish(b, P);
 }
 
 }  // extern "C"

@@ -58,6 +58,9 @@ EXPORTS = [
     "odom_icp_step", "odom_compute_derivatives", "odom_rgb_residual", "odom_rgb_step", "odom_so3_step",
     "odom_read_buffer", "odom_set_step_shape", "odom_get_step_shape", "odom_autotune", "odom_device_info",
     "odom_init_icp_model_current", "odom_profile_track", "odom_set_pose_log",
+    "odom_batch_create", "odom_batch_destroy", "odom_track_batch", "odom_batch_synchronize",
+    "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
+    "odom_batch_profile_track",
 ]
 
 _V = ctypes.c_void_p
@@ -102,6 +105,16 @@ def lib() -> ctypes.CDLL:
     L.odom_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _V, _V, _V, _V, _P(ctypes.c_float)]
     L.odom_set_pose_log.argtypes = [_V, _V, ctypes.c_int]
+    L.odom_batch_create.argtypes = [_V, ctypes.c_int, _P(_V)]
+    L.odom_batch_destroy.argtypes = [_V]
+    L.odom_track_batch.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.odom_batch_synchronize.argtypes = [_V]
+    L.odom_batch_set_blocks.argtypes = [_V, ctypes.c_int]
+    L.odom_batch_get_blocks.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int)]
+    L.odom_batch_set_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.odom_batch_set_graph.argtypes = [_V, ctypes.c_int]
+    L.odom_batch_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _V, _V, _V, _V, _P(ctypes.c_float)]
     L._typed = True
     return L
 
@@ -373,6 +386,61 @@ class RGBDOdometry:
 
     def autotune(self, rgbOnly=False, icpWeight=10.0) -> None:
         _check("odom_autotune", lib().odom_autotune(self._h, int(rgbOnly), float(icpWeight)))
+
+
+class Batch:
+    """Many RGBDOdometry handles (frame pairs of any size) tracked by the same
+    launches (``odom_track_batch``).  Inputs go to each handle with its own
+    init* calls; results are read per handle."""
+
+    def __init__(self, odos):
+        self.odos = list(odos)
+        arr = (_V * len(self.odos))(*[o.handle for o in self.odos])
+        b = _V()
+        _check("odom_batch_create", lib().odom_batch_create(arr, len(self.odos), ctypes.byref(b)))
+        self._b = b
+
+    def close(self) -> None:
+        if getattr(self, "_b", None):
+            lib().odom_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def track(self, rgbOnly=False, icpWeight=10.0, pyramid=True, fastOdom=False, so3=True) -> None:
+        _check("odom_track_batch", lib().odom_track_batch(self._b, int(rgbOnly), float(icpWeight), int(pyramid),
+                                                          int(fastOdom), int(so3)))
+
+    def synchronize(self) -> None:
+        _check("odom_batch_synchronize", lib().odom_batch_synchronize(self._b))
+
+    def set_blocks(self, target_blocks: int) -> None:
+        _check("odom_batch_set_blocks", lib().odom_batch_set_blocks(self._b, int(target_blocks)))
+
+    def get_blocks(self, kernel: int, level: int) -> int:
+        n = ctypes.c_int()
+        _check("odom_batch_get_blocks", lib().odom_batch_get_blocks(self._b, kernel, level, ctypes.byref(n)))
+        return n.value
+
+    def set_step_shape(self, kernel: int, level: int, threads: int, ppt: int) -> None:
+        _check("odom_batch_set_step_shape", lib().odom_batch_set_step_shape(self._b, kernel, level, threads, ppt))
+
+    def set_graph(self, enable: bool) -> None:
+        _check("odom_batch_set_graph", lib().odom_batch_set_graph(self._b, int(enable)))
+
+    def profile_track(self, rgbOnly=False, icpWeight=10.0, pyramid=True, fastOdom=False, so3=True) -> dict:
+        ua, ub = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        na, nb = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        tot = ctypes.c_float()
+        _check("odom_batch_profile_track",
+               lib().odom_batch_profile_track(self._b, int(rgbOnly), float(icpWeight), int(pyramid), int(fastOdom),
+                                              int(so3), _p(ua), _p(ub), _p(na), _p(nb), ctypes.byref(tot)))
+        return {"us_a": ua.tolist(), "us_b": ub.tolist(), "n_a": na.tolist(), "n_b": nb.tolist(),
+                "us_total": tot.value}
 
 
 def device_info() -> tuple[str, str]:
