@@ -44,6 +44,131 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+BATCH_LAYOUT = (("CAM_320", 16), ("CAM_640", 32), ("CAM_848", 16))
+
+
+def batched_pairs(args) -> dict:
+    """64 independent pairs of uneven size (16 at 320x240, 32 at 640x480, 16
+    at 848x480), seeded rooms, per-pair motion up to 3 cm / 2 deg, tracked by
+    one batched call (ICP+RGB+SO3, 10/5/4) replayed as a hipGraph.  Reports
+    pairs/s of the batched call (inputs prepared per handle beforehand), the
+    roofline of the batched ICP pass, and the pose error of a few pairs against
+    the FP64 oracle."""
+    from slam_amd import synth
+    from slam_amd.odometry import Batch, RGBDOdometry
+
+    t0 = time.time()
+    cams, frames = [], []
+    i = 0
+    for cam_name, count in BATCH_LAYOUT:
+        cam = getattr(synth, cam_name)
+        for _ in range(count):
+            seed = 500 + i
+            scene = synth.Scene(seed)
+            rng = np.random.default_rng(seed)
+            T0 = synth.base_pose()
+            T1 = synth.perturb(T0, rng, 0.03, 2.0)
+            f0 = scene.render(T0, cam, noise_seed=2 * seed + 1)
+            f1 = scene.render(T1, cam, noise_seed=2 * seed + 2)
+            cams.append(cam)
+            frames.append((T0, T1, f0, f1))
+            i += 1
+    log(f"batch: rendered {len(frames)} pairs in {time.time() - t0:.1f}s")
+    odos = [RGBDOdometry(c.width, c.height, c.cx, c.cy, c.fx, c.fy) for c in cams]
+
+    def feed_all():
+        for o, (T0, T1, f0, f1) in zip(odos, frames):
+            o.set_pose(T0[:3, 3], T0[:3, :3])
+            o.initFirstRGB(f0.rgb)
+            o.initICPModel(f0.vert4, f0.norm4, CUTOFF, T0.astype(np.float32))
+            o.initRGBModel(f0.rgba)
+            o.initICPRaw(f1.depth, MAXD, CUTOFF)
+            o.initRGB(f1.rgb)
+        for o in odos:
+            o.synchronize()
+
+    b = Batch(odos)
+    b.set_graph(True)
+    feed_all()
+    b.track(False, ICP_WEIGHT, True, False, True)
+    b.synchronize()
+    got = [o.get_pose() for o in odos]
+
+    # accuracy: a few pairs of each size against the FP64 oracle
+    acc = None
+    if not args.no_cpu and args.batch_oracle_pairs > 0:
+        from oracle import oracle as orc
+        pick = []
+        k = 0
+        for _, count in BATCH_LAYOUT:
+            pick += list(range(k, k + max(1, args.batch_oracle_pairs // len(BATCH_LAYOUT))))
+            k += count
+        errs = []
+        for j in pick:
+            T0, T1, f0, f1 = frames[j]
+            c = cams[j]
+            o = orc.Odometry(c.width, c.height, c.cx, c.cy, c.fx, c.fy, fp64=True)
+            o.initFirstRGB(f0.rgb)
+            o.initICPModel(f0.vert4, f0.norm4, CUTOFF, T0.astype(np.float32))
+            o.initRGBModel(f0.rgba)
+            o.initICP(orc.bilateral(f1.depth, MAXD), CUTOFF)
+            o.initRGB(f1.rgb)
+            t, R = o.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, ICP_WEIGHT, True, False, True)
+            ref = synth.make_pose(R.astype(np.float64), t.astype(np.float64))
+            g = synth.make_pose(got[j][1].astype(np.float64), got[j][0].astype(np.float64))
+            errs.append(synth.pose_error(g, ref))
+        acc = {"pairs": len(pick), "vs_fp64_oracle_mean_mm": round(1e3 * float(np.mean([e[0] for e in errs])), 5),
+               "vs_fp64_oracle_max_mm": round(1e3 * float(np.max([e[0] for e in errs])), 5),
+               "vs_fp64_oracle_max_deg": round(float(np.max([e[1] for e in errs])), 6)}
+        log(f"batch accuracy {acc}")
+
+    # launch shapes for the batch's own launches
+    b.autotune(False, ICP_WEIGHT)
+    bshapes = {f"{'ab'[k]}{l}": list(b.get_step_shape(k, l)) for k in range(2) for l in range(3)}
+    log(f"batch shapes {bshapes}")
+
+    # timing: repeated batched calls on the prepared pairs (each call swaps the
+    # pairs' images, as the reference does, so consecutive calls alternate)
+    for _ in range(3):
+        b.track(False, ICP_WEIGHT, True, False, True)
+    b.synchronize()
+    reps = max(1, args.batch_reps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.track(False, ICP_WEIGHT, True, False, True)
+    b.synchronize()
+    call_s = (time.perf_counter() - t0) / reps
+    prof = [b.profile_track(False, ICP_WEIGHT, True, False, True) for _ in range(5)]
+    iters = [10, 5, 4]
+    npx = [sum((c.width >> l) * (c.height >> l) for c in cams) for l in range(3)]
+    us_a = [float(np.mean([p["us_a"][l] for p in prof])) for l in range(3)]
+    us_b = [float(np.mean([p["us_b"][l] for p in prof])) for l in range(3)]
+    launches = sum(iters)
+    bytes_a = sum(iters[l] * BYTES_ICP * npx[l] for l in range(3)) / launches
+    avg_a = sum(iters[l] * us_a[l] for l in range(3)) / launches
+    ach = bytes_a / (avg_a * 1e-6) / 1e9
+    l0_gbs = BYTES_ICP * npx[0] / (us_a[0] * 1e-6) / 1e9
+    joint = sum(iters[l] * (BYTES_ICP + BYTES_RGB) * npx[l] for l in range(3)) / \
+        (sum(iters[l] * (us_a[l] + us_b[l]) for l in range(3)) * 1e-6) / 1e9
+    out = {"workload": "64 pairs: 16x320x240 + 32x640x480 + 16x848x480, <=3cm/2deg, ICP+RGB+SO3 10/5/4, "
+                       "one batched call per step (hipGraph)",
+           "pairs_per_s": round(len(odos) / call_s, 1), "ms_per_call": round(call_s * 1e3, 3),
+           "roofline": {"bound": "hbm", "kernel": "k_step_a (batched)", "achieved": round(ach, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "level0_GBs": round(l0_gbs, 1), "bytes_per_launch": int(bytes_a),
+                        "avg_launch_us": round(avg_a, 2)},
+           "per_level_us": {"k_step_a": [round(x, 2) for x in us_a], "k_step_b": [round(x, 2) for x in us_b]},
+           "joint_iteration_gbs": round(joint, 1),
+           "blocks_per_launch": [b.get_blocks(0, l) for l in range(3)],
+           "shapes": bshapes,
+           "accuracy": acc}
+    log(f"batch {out}")
+    b.close()
+    for o in odos:
+        o.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +178,10 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=20, help="frames of the CPU baseline / FP64 oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-pair measurement")
+    ap.add_argument("--batch-reps", type=int, default=20)
+    ap.add_argument("--batch-oracle-pairs", type=int, default=6,
+                    help="pairs of the batch checked against the FP64 oracle on the host")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -213,6 +342,8 @@ def main() -> int:
                          f"(oracle/odom_oracle.c) with OpenMP over rows"}
         log(f"accuracy {accuracy}")
 
+    batched = None if args.no_batch else batched_pairs(args)
+
     line = {
         "metric": "frames/s 640x480 ICP+RGB at <=1mm vs FP64 oracle; learned-init vs plain",
         "value": round(fps, 2),
@@ -227,7 +358,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "640x480 ICP+RGB+SO3 frame-to-model tracking, 3 levels 10/5/4, icpWeight 10, "
-                               "smooth spline <=1cm/0.5deg per frame, plain constant-velocity init",
+                               "smooth spline <=1cm/0.5deg per frame, plain init (the previous pose, as the reference's frame loop)",
                    "frames": args.steps, "width": cam.width, "height": cam.height, "seed": args.seed,
                    "init": "plain"},
         "roofline": roofline,
@@ -236,6 +367,7 @@ def main() -> int:
         "drift_vs_truth": {"mean_mm": round(1e3 * drift_t, 3), "mean_deg": round(drift_r, 4)},
         "last_stats": stats,
         "shapes": shapes,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     return 0

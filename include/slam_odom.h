@@ -190,11 +190,19 @@ int odom_batch_destroy(odom_batch b);
  * made afterwards see the results. */
 int odom_track_batch(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3);
 int odom_batch_synchronize(odom_batch b);
-/* Blocks per step launch over the whole batch (default: 2 per compute unit),
+/* Blocks per step launch over the whole batch (default: 4 per compute unit),
  * shared out by pixel count, at most 256 per pair. */
 int odom_batch_set_blocks(odom_batch b, int target_blocks);
 int odom_batch_get_blocks(odom_batch b, int kernel, int level, int *total_blocks);
 int odom_batch_set_step_shape(odom_batch b, int kernel, int level, int threads, int pixels_per_thread);
+int odom_batch_get_step_shape(odom_batch b, int kernel, int level, int *threads, int *pixels_per_thread);
+/* As odom_autotune for the batch's launches; needs a completed batched call
+ * on the current inputs. */
+int odom_batch_autotune(odom_batch b, int rgb_only, float icp_weight);
+/* 1 (default): each iteration's solve runs once per pair in a kernel of its
+ * own before the ICP pass; 0: every block of the ICP pass repeats it in its
+ * prologue (the single-pair form). */
+int odom_batch_set_split_solve(odom_batch b, int enable);
 /* Capture each distinct call (options and image parities) as a hipGraph on
  * first use and replay it afterwards. */
 int odom_batch_set_graph(odom_batch b, int enable);

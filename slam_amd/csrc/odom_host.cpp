@@ -745,8 +745,10 @@ int odom_icp_step(odom_handle c, int level, const float Rc[9], const float tc[3]
     if (!c || level < 0 || level >= kLevels || !Rc || !tc || !Rpi || !tp || !out) return ODOM_ERR_ARG;
     int rc;
     IterState* S0 = &c->st->it[0];
+    const int zero = 0;
     if ((rc = write_field(c, S0->Rcurr, Rc, 9)) || (rc = write_field(c, S0->tcurr, tc, 3)) ||
-        (rc = write_field(c, c->st->Rprev_inv, Rpi, 9)) || (rc = write_field(c, c->st->tprev, tp, 3)))
+        (rc = write_field(c, c->st->Rprev_inv, Rpi, 9)) || (rc = write_field(c, c->st->tprev, tp, 3)) ||
+        (rc = write_field(c, &S0->skip, &zero, 1)))
         return rc;
     TrackParams P = make_params(c, 0, 100.0f, 1, 0, 0);
     P.icp = 1;
@@ -778,7 +780,10 @@ int odom_rgb_residual(odom_handle c, int level, const float K[9], const float kt
     if (!c || level < 0 || level >= kLevels || !K || !kt || !count || !sigma) return ODOM_ERR_ARG;
     int rc;
     IterState* S0 = &c->st->it[0];
-    if ((rc = write_field(c, S0->krkinv, K, 9)) || (rc = write_field(c, S0->kt, kt, 3))) return rc;
+    const int zero = 0;
+    if ((rc = write_field(c, S0->krkinv, K, 9)) || (rc = write_field(c, S0->kt, kt, 3)) ||
+        (rc = write_field(c, &S0->skip, &zero, 1)))
+        return rc;
     TrackParams P = make_params(c, 0, 10.0f, 1, 0, 0);
     P.icp = 0;
     P.rgb = 1;
@@ -939,6 +944,8 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
     HIPCHK(c, hipEventCreate(&e1));
     int rc = odom_compute_derivatives(c);
     if (rc) return rc;
+    const int zero = 0;
+    if ((rc = write_field(c, &c->st->it[0].skip, &zero, 1))) return rc;
     for (int kernel = 0; kernel < 2; ++kernel) {
         if (kernel == 1 && !P.rgb) break;
         for (int level = 0; level < kLevels; ++level) {
@@ -990,6 +997,7 @@ struct odom_batch_s {
     std::map<std::vector<int>, PairDesc*> slots;   // descriptor arrays by the pairs' image parities
     PairDesc* h_stage = nullptr;                   // pinned upload buffer
     bool use_graph = false;
+    bool split_solve = true;        // k_solve + unfused pass A (else the fused prologue in every block)
     struct G { std::vector<int> key; hipGraphExec_t exec; };
     std::vector<G> graphs;
     hipError_t last_hip = hipSuccess;
@@ -1113,7 +1121,12 @@ void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEven
         La.total_blocks = b->total[0][level];
         La.shape = b->shape[0][level];
         if (ev) (void)hipEventRecord(ev[e++], b->stream);
-        launch_step_a(La, level, iter, (int)k, prev, P, 0, nullptr, b->stream);
+        if (b->split_solve) {
+            launch_solve(dp, n, level, iter, (int)k, prev, P, b->stream);
+            launch_step_a(La, level, iter, (int)k, prev, P, 1, nullptr, b->stream);
+        } else {
+            launch_step_a(La, level, iter, (int)k, prev, P, 0, nullptr, b->stream);
+        }
         if (ev) (void)hipEventRecord(ev[e++], b->stream);
         if (P.rgb) {
             StepLaunch Lb;
@@ -1208,7 +1221,7 @@ int odom_batch_create(const odom_handle* handles, int n, odom_batch* out)
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
-    b->target_blocks = 2 * cus;
+    b->target_blocks = 4 * cus;
     for (int k = 0; k < 2; ++k)
         for (int l = 0; l < kLevels; ++l) b->shape[k][l] = StepShape{256, 2};
     b->ready.resize(n, nullptr);
@@ -1267,6 +1280,15 @@ int odom_batch_get_blocks(odom_batch b, int kernel, int level, int* total_blocks
     return ODOM_OK;
 }
 
+int odom_batch_set_split_solve(odom_batch b, int enable)
+{
+    if (!b) return ODOM_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->split_solve = enable != 0;
+    batch_drop_graphs(b);
+    return ODOM_OK;
+}
+
 int odom_batch_set_graph(odom_batch b, int enable)
 {
     if (!b) return ODOM_ERR_ARG;
@@ -1312,6 +1334,85 @@ int odom_track_batch(odom_batch b, int rgbOnly, float icpWeight, int pyramid, in
         BCHK(b, hipGraphLaunch(exec, b->stream));
     }
     return batch_finish(b, P);
+}
+
+// Times every supported shape of the two step kernels per level on the
+// batch's current inputs (pixel passes only, with the state of the last
+// batched call) and keeps the fastest.
+int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
+{
+    if (!b) return ODOM_ERR_ARG;
+    TrackParams P = make_params(b->h[0], rgbOnly, icpWeight, 1, 0, 0);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    for (auto* c : b->h)
+        if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (auto* c : b->h) {
+        BCHK(b, hipStreamSynchronize(c->stream));
+        const int zero = 0;
+        if (int rc = write_field(c, &c->st->it[0].skip, &zero, 1)) return rc;
+        BCHK(b, hipStreamSynchronize(c->stream));
+    }
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    const int threads[] = {256, 512};
+    const int ppts[] = {1, 2, 4};
+    int rc = ODOM_OK;
+    for (int kernel = 0; kernel < 2 && rc == ODOM_OK; ++kernel) {
+        if (kernel == 1 && !P.rgb) break;
+        for (int level = 0; level < kLevels && rc == ODOM_OK; ++level) {
+            float best = 1e30f;
+            StepShape best_sh = b->shape[kernel][level];
+            for (int t : threads)
+                for (int q : ppts) {
+                    b->shape[kernel][level] = StepShape{t, q};
+                    if ((rc = batch_plan(b))) break;
+                    PairDesc* dp = nullptr;
+                    if ((rc = batch_slot(b, &dp))) break;
+                    StepLaunch Ls;
+                    Ls.pairs = dp;
+                    Ls.npairs = (int)b->h.size();
+                    Ls.blk_pair = b->d_blk[kernel][level];
+                    Ls.total_blocks = b->total[kernel][level];
+                    Ls.shape = b->shape[kernel][level];
+                    const int reps = 10;
+                    for (int r = 0; r < reps + 2; ++r) {
+                        if (r == 2) BCHK(b, hipEventRecord(e0, b->stream));
+                        if (kernel == 0) launch_step_a(Ls, level, 0, 0, -1, P, 1, nullptr, b->stream);
+                        else launch_step_b(Ls, level, 0, 0, P, 1, b->stream);
+                    }
+                    BCHK(b, hipEventRecord(e1, b->stream));
+                    BCHK(b, hipEventSynchronize(e1));
+                    hipError_t e = hipPeekAtLastError();
+                    if (e != hipSuccess) {
+                        (void)hipGetLastError();
+                        b->last_hip = e;
+                        rc = ODOM_ERR_HIP;
+                        break;
+                    }
+                    float ms = 0.0f;
+                    BCHK(b, hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        best_sh = StepShape{t, q};
+                    }
+                }
+            b->shape[kernel][level] = best_sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return batch_plan(b);
+}
+
+int odom_batch_get_step_shape(odom_batch b, int kernel, int level, int* threads, int* ppt)
+{
+    if (!b || !threads || !ppt || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
+    *threads = b->shape[kernel][level].threads;
+    *ppt = b->shape[kernel][level].ppt;
+    return ODOM_OK;
 }
 
 int odom_batch_synchronize(odom_batch b)

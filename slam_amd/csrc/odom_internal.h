@@ -97,6 +97,10 @@ void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P,
 void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_level, const TrackParams& P, int mode,
                    uint8_t* dbg_mask, hipStream_t s);
 void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s);
+// Finish iteration k-1 and store the state of iteration k, one workgroup per
+// pair (followed by launch_step_a with mode 1).
+void launch_solve(const PairDesc* pairs, int npairs, int level, int iter, int k, int prev_level, const TrackParams& P,
+                  hipStream_t s);
 // finishes the last iteration (K-1, at last_level), revert, result
 void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s);
 // FP64 totals of a parity pass: which 0 = pass A (ICP sums, count, sigma), 1 = pass B

@@ -917,6 +917,13 @@ struct RgbC {
     float maxDepthDelta;
 };
 
+// A block-uniform value moved to a scalar register (the constants of an
+// iteration are the same in every lane; kept in SGPRs they cost no VGPRs).
+__device__ __forceinline__ float uni(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
 __device__ __forceinline__ int rgb_warp(const LevelDesc& L, const RgbC& c, int x, int y, float d1, float& tz)
 {
     if (isnan(d1)) return -1;
@@ -1225,6 +1232,16 @@ struct Prolog {
     unsigned int toti[2];
 };
 
+// State entering iteration `iter` of `level`: skipped when the rgbOnly test
+// stopped this level, else the warp constants of the current estimate
+// (RGBDOdometryef.cpp:418-432) and the lastRGBError the test compares to.
+__device__ __forceinline__ void enter_iteration(IterState& S, const LevelDesc& L, int level, int iter)
+{
+    S.skip = (S.break_level == level);
+    if (!S.skip) iter_constants(S, L);
+    S.rgb_err_cmp = (iter == 0) ? FLT_MAX : S.lastRGBError;
+}
+
 // Finish iteration k-1 and leave the state entering iteration k in W.S: one
 // fused FP64 reduction of its ICP partials (pass A) and photometric partials
 // (pass B), the ICP statistics, the photometric bookkeeping when there is no
@@ -1358,9 +1375,9 @@ __device__ __forceinline__ void work_tile_a(const LevelDesc& L, const TrackParam
         }
 }
 
-template <int NT, int PPT>
+template <int NT, int PPT, bool FUSED>
 __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pairs, const int* __restrict__ blk_pair,
-                                               int level, int iter, int k, int prev_level, TrackParams P, int mode,
+                                               int level, int iter, int k, int prev_level, TrackParams P,
                                                uint8_t* __restrict__ dbg_mask)
 {
     __shared__ Prolog<NT> W;
@@ -1381,25 +1398,29 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     PrevPose pp;
     load_prev(st, pp);
 
-    if (mode == 0) {
+    if (FUSED) {
         finish_previous<NT>(D, P, k, prev_level, pp, W);
-        if (threadIdx.x == 0) {
-            W.S.skip = (W.S.break_level == level);
-            if (!W.S.skip) iter_constants(W.S, L);
-            W.S.rgb_err_cmp = (iter == 0) ? FLT_MAX : W.S.lastRGBError;
-        }
+        if (threadIdx.x == 0) enter_iteration(W.S, L, level, iter);
         __syncthreads();
         if (lb == 0) copy_words<NT>(&st->it[k & 1], &W.S);
-        if (W.S.skip) return;
     } else {
-        copy_words<NT>(&W.S, &st->it[0]);
+        copy_words<NT>(&W.S, &st->it[k & 1]);
         __syncthreads();
     }
+    if (W.S.skip) return;
 
     IcpC ic;
     RgbC rc;
-    for (int q = 0; q < 9; ++q) { ic.Rc[q] = W.S.Rcurr[q]; ic.Rpi[q] = pp.Ri[q]; rc.K[q] = W.S.krkinv[q]; }
-    for (int q = 0; q < 3; ++q) { ic.tc[q] = W.S.tcurr[q]; ic.tp[q] = pp.t[q]; rc.kt[q] = W.S.kt[q]; }
+    for (int q = 0; q < 9; ++q) {
+        ic.Rc[q] = uni(W.S.Rcurr[q]);
+        ic.Rpi[q] = uni(pp.Ri[q]);
+        rc.K[q] = uni(W.S.krkinv[q]);
+    }
+    for (int q = 0; q < 3; ++q) {
+        ic.tc[q] = uni(W.S.tcurr[q]);
+        ic.tp[q] = uni(pp.t[q]);
+        rc.kt[q] = uni(W.S.kt[q]);
+    }
     ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
     ic.dist = P.distThres; ic.angle = P.angleThres;
     rc.maxDepthDelta = P.maxDepthDelta;
@@ -1416,6 +1437,24 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     const int pb = k & 1;
     block_partial<kSE3, NT>(acc, D.part_a[pb] + (size_t)lb * kPartStride, s_red);
     block_partial_int<NT>(cnt, sig, D.part_ai[pb] + (size_t)lb * 2, W.redi);
+}
+
+// The prologue of the fused pass A as a kernel of its own, one workgroup per
+// pair: finish iteration k-1 and store the state entering iteration k, which
+// the unfused pass A (FUSED = false) then reads.  Used by batches, where it
+// keeps the solve's registers out of the pixel kernel.
+__global__ __launch_bounds__(256) void k_solve(const PairDesc* __restrict__ pairs, int level, int iter, int k,
+                                               int prev_level, TrackParams P)
+{
+    __shared__ Prolog<256> W;
+    const PairDesc& D = pairs[blockIdx.x];
+    TrackState* st = D.st;
+    PrevPose pp;
+    load_prev(st, pp);
+    finish_previous<256>(D, P, k, prev_level, pp, W);
+    if (threadIdx.x == 0) enter_iteration(W.S, D.lv[level], level, iter);
+    __syncthreads();
+    copy_words<256>(&st->it[k & 1], &W.S);
 }
 
 // ======================================================== step kernel B
@@ -1494,8 +1533,8 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
     }
 
     RgbC rc;
-    for (int q = 0; q < 9; ++q) rc.K[q] = Sg.krkinv[q];
-    for (int q = 0; q < 3; ++q) rc.kt[q] = Sg.kt[q];
+    for (int q = 0; q < 9; ++q) rc.K[q] = uni(Sg.krkinv[q]);
+    for (int q = 0; q < 3; ++q) rc.kt[q] = uni(Sg.kt[q]);
     rc.maxDepthDelta = P.maxDepthDelta;
 
     float acc[kSE3];
@@ -1573,10 +1612,34 @@ int step_shape_supported(StepShape sh)
         }                                                                                                \
     } while (0)
 
+#define STEP_DISPATCH_A(FUS, ...)                                                                        \
+    do {                                                                                                 \
+        const dim3 grid(L.total_blocks);                                                                 \
+        switch (L.shape.threads * 100 + L.shape.ppt) {                                                   \
+            case 25601: hipLaunchKernelGGL((k_step_a<256, 1, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            case 25602: hipLaunchKernelGGL((k_step_a<256, 2, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            case 25604: hipLaunchKernelGGL((k_step_a<256, 4, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            case 51201: hipLaunchKernelGGL((k_step_a<512, 1, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
+            case 51202: hipLaunchKernelGGL((k_step_a<512, 2, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
+            case 51204: hipLaunchKernelGGL((k_step_a<512, 4, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
+            default: break;                                                                              \
+        }                                                                                                \
+    } while (0)
+
+// mode 0: fused prologue (finish iteration k-1 in every block); mode 1: the
+// state entering iteration k is already in slot k & 1 (k_solve, or the
+// parity-check entry points with k = 0).
 void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_level, const TrackParams& P, int mode,
                    uint8_t* dbg_mask, hipStream_t s)
 {
-    STEP_DISPATCH(k_step_a, L.pairs, L.blk_pair, level, iter, k, prev_level, P, mode, dbg_mask);
+    if (mode == 0) STEP_DISPATCH_A(true, L.pairs, L.blk_pair, level, iter, k, prev_level, P, dbg_mask);
+    else STEP_DISPATCH_A(false, L.pairs, L.blk_pair, level, iter, k, prev_level, P, dbg_mask);
+}
+
+void launch_solve(const PairDesc* pairs, int npairs, int level, int iter, int k, int prev_level, const TrackParams& P,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_solve, dim3(npairs), dim3(256), 0, s, pairs, level, iter, k, prev_level, P);
 }
 
 void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s)

@@ -60,7 +60,7 @@ EXPORTS = [
     "odom_init_icp_model_current", "odom_profile_track", "odom_set_pose_log",
     "odom_batch_create", "odom_batch_destroy", "odom_track_batch", "odom_batch_synchronize",
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
-    "odom_batch_profile_track",
+    "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
 ]
 
 _V = ctypes.c_void_p
@@ -113,6 +113,9 @@ def lib() -> ctypes.CDLL:
     L.odom_batch_get_blocks.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int)]
     L.odom_batch_set_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.odom_batch_set_graph.argtypes = [_V, ctypes.c_int]
+    L.odom_batch_set_split_solve.argtypes = [_V, ctypes.c_int]
+    L.odom_batch_autotune.argtypes = [_V, ctypes.c_int, ctypes.c_float]
+    L.odom_batch_get_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]
     L.odom_batch_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, _V, _V, _V, _V, _P(ctypes.c_float)]
     L._typed = True
@@ -428,6 +431,18 @@ class Batch:
 
     def set_step_shape(self, kernel: int, level: int, threads: int, ppt: int) -> None:
         _check("odom_batch_set_step_shape", lib().odom_batch_set_step_shape(self._b, kernel, level, threads, ppt))
+
+    def autotune(self, rgbOnly=False, icpWeight=10.0) -> None:
+        _check("odom_batch_autotune", lib().odom_batch_autotune(self._b, int(rgbOnly), float(icpWeight)))
+
+    def get_step_shape(self, kernel: int, level: int) -> tuple[int, int]:
+        t, p = ctypes.c_int(), ctypes.c_int()
+        _check("odom_batch_get_step_shape", lib().odom_batch_get_step_shape(self._b, kernel, level, ctypes.byref(t),
+                                                                            ctypes.byref(p)))
+        return t.value, p.value
+
+    def set_split_solve(self, enable: bool) -> None:
+        _check("odom_batch_set_split_solve", lib().odom_batch_set_split_solve(self._b, int(enable)))
 
     def set_graph(self, enable: bool) -> None:
         _check("odom_batch_set_graph", lib().odom_batch_set_graph(self._b, int(enable)))
