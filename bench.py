@@ -169,6 +169,128 @@ def batched_pairs(args) -> dict:
     return out
 
 
+def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain_est, ref64) -> dict | None:
+    """The learned form on the same sequence: per frame the same init calls,
+    then the network input from the handle's level-2 buffers, the network
+    (bf16 MFMA) and a seeded call with a short schedule and the keep-or-fall-
+    back rule, all enqueued on the handle's stream.  Reports its frames/s next
+    to the plain form's, its pose error against the FP64 oracle (plain init,
+    full schedule) on the CPU sample, and -- on a fast-motion sequence with
+    bursts up to 8 cm / 6 deg per frame -- both forms' error against the
+    rendered ground truth."""
+    import torch
+    from slam_amd import synth, warmstart as ws
+    if not os.path.exists(ws.WEIGHTS):
+        log("learned: no trained weights; skipped")
+        return None
+    iters = [int(v) for v in args.learned_iters.split(",")]
+    net = ws.WarmStartNet(max_batch=1)
+    net.load()
+    net.set_stream(odo.stream)
+    x = torch.zeros((1, 60, 80, 4), dtype=torch.bfloat16, device="cuda")
+    pred = torch.zeros((1, 6), dtype=torch.float32, device="cuda")
+    it = np.ascontiguousarray(iters, dtype=np.int32)
+    by = ctypes.byref
+
+    def run_learned(k0, k1, vw):
+        for k in range(k0, k1):
+            L.odom_init_icp_model_current(h, by(vw["v4"][k - 1]), by(vw["n4"][k - 1]), CUTOFF)
+            L.odom_init_rgb_model(h, by(vw["rgba"][k - 1]))
+            L.odom_init_icp_raw_depth(h, by(vw["depth"][k]), MAXD, CUTOFF)
+            L.odom_init_rgb(h, by(vw["rgb"][k]))
+            L.odom_init_warmstart_input(h, x.data_ptr())
+            ws._lib().cnn_forward(net.handle, x.data_ptr(), 1, pred.data_ptr())
+            L.odom_track_seeded(h, pred.data_ptr(), ctypes.c_float(ws.T_SCALE), ctypes.c_float(ws.R_SCALE),
+                                it.ctypes.data, 0, ctypes.c_float(ICP_WEIGHT), 1)
+
+    nsteps = args.steps
+    reset()
+    run_learned(1, 1 + args.warmup, views)
+    odo.synchronize()
+    log_t = torch.zeros((nsteps + 1, 12), dtype=torch.float32, device="cuda")
+    odo.set_pose_log(log_t.data_ptr(), nsteps + 1)
+    reset()
+    torch.cuda.synchronize()
+    odo.synchronize()
+    s0 = odo.stats()
+    t0 = time.perf_counter()
+    run_learned(1, 1 + nsteps, views)
+    odo.synchronize()
+    el = time.perf_counter() - t0
+    s1 = odo.stats()
+    fps = nsteps / el
+    est = log_t.cpu().numpy()[:nsteps]
+
+    def pose12(p):
+        T = np.eye(4)
+        T[:3, :4] = p.reshape(3, 4)
+        return T
+
+    kept = (s1["seedKept"] - s0["seedKept"]) / max(1, s1["seedCalls"] - s0["seedCalls"])
+    out = {"workload": "same 640x480 sequence, network init + seeded ICP+RGB, schedule "
+                       f"{iters[0]}/{iters[1]}/{iters[2]}, no SO3, fallback to the previous pose when worse",
+           "frames_per_s": round(fps, 2), "speedup_vs_plain": round(fps / plain_fps, 3),
+           "prediction_kept_frac": round(kept, 3)}
+    if ref64 is not None:
+        n = len(ref64)
+        e = [synth.pose_error(pose12(est[k]), ref64[k]) for k in range(n)]
+        out["vs_fp64_oracle_mean_mm"] = round(1e3 * float(np.mean([q[0] for q in e])), 4)
+        out["vs_fp64_oracle_max_mm"] = round(1e3 * float(np.max([q[0] for q in e])), 4)
+        out["vs_fp64_oracle_mean_deg"] = round(float(np.mean([q[1] for q in e])), 5)
+    dt_l = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, nsteps + 1)]
+    out["drift_vs_truth_mean_mm"] = round(1e3 * float(np.mean([d[0] for d in dt_l])), 3)
+
+    # fast motion: bursts up to 8 cm / 6 deg per frame, both forms vs truth
+    nb = args.burst_frames
+    if nb > 1:
+        bposes = synth.spline_trajectory(nb, seed=args.seed + 1, max_step_m=0.08, max_step_deg=6.0, bursts=True)
+        scene = synth.Scene(args.seed + 1)
+        bdev = {k: [] for k in ("depth", "rgb", "v4", "n4", "rgba")}
+        for i, T in enumerate(bposes):
+            f = scene.render(T, cam, noise_seed=5000 + i)
+            bdev["depth"].append(torch.from_numpy(f.depth).cuda())
+            bdev["rgb"].append(torch.from_numpy(f.rgb).cuda())
+            bdev["v4"].append(torch.from_numpy(f.vert4).cuda())
+            bdev["n4"].append(torch.from_numpy(f.norm4).cuda())
+            bdev["rgba"].append(torch.from_numpy(f.rgba).cuda())
+        from slam_amd.odometry import frame_view, FMT_DEPTH_U16, FMT_RGB8, FMT_RGBA8, FMT_RGBA32F
+        bv = {k: [frame_view(t, fmt)[0] for t in bdev[k]] for k, fmt in
+              (("depth", FMT_DEPTH_U16), ("rgb", FMT_RGB8), ("v4", FMT_RGBA32F), ("n4", FMT_RGBA32F),
+               ("rgba", FMT_RGBA8))}
+        res = {}
+        for mode in ("plain", "learned"):
+            T0 = bposes[0]
+            odo.set_pose(T0[:3, 3], T0[:3, :3])
+            odo.initFirstRGB(bdev["rgb"][0])
+            lg = torch.zeros((nb, 12), dtype=torch.float32, device="cuda")
+            odo.set_pose_log(lg.data_ptr(), nb)
+            if mode == "plain":
+                for k in range(1, nb):
+                    L.odom_init_icp_model_current(h, by(bv["v4"][k - 1]), by(bv["n4"][k - 1]), CUTOFF)
+                    L.odom_init_rgb_model(h, by(bv["rgba"][k - 1]))
+                    L.odom_init_icp_raw_depth(h, by(bv["depth"][k]), MAXD, CUTOFF)
+                    L.odom_init_rgb(h, by(bv["rgb"][k]))
+                    L.odom_track_async(h, 0, ICP_WEIGHT, 1, 0, 1)
+            else:
+                run_learned(1, nb, bv)
+            odo.synchronize()
+            e = lg.cpu().numpy()[: nb - 1]
+            errs = [synth.pose_error(pose12(e[k - 1]), bposes[k]) for k in range(1, nb)]
+            # relative pose error per frame: estimated step vs true step
+            rpe = [synth.pose_error(np.linalg.inv(pose12(e[k - 2])) @ pose12(e[k - 1]),
+                                    np.linalg.inv(bposes[k - 1]) @ bposes[k])[0] for k in range(2, nb)]
+            res[mode] = {"ate_mean_mm": round(1e3 * float(np.mean([q[0] for q in errs])), 2),
+                         "rpe_mean_mm": round(1e3 * float(np.mean(rpe)), 3),
+                         "frames_lost_gt_5cm": int(sum(1 for r in rpe if r > 0.05))}
+        steps = [synth.pose_error(bposes[k], bposes[k - 1]) for k in range(1, nb)]
+        out["bursts"] = {"frames": nb, "max_step_mm": round(1e3 * max(q[0] for q in steps), 1),
+                         "max_step_deg": round(max(q[1] for q in steps), 2), **{k: v for k, v in res.items()}}
+    log(f"learned {out}")
+    odo.set_pose_log(0, 0)
+    net.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +301,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-pair measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
+    ap.add_argument("--burst-frames", type=int, default=150)
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--batch-oracle-pairs", type=int, default=6,
                     help="pairs of the batch checked against the FP64 oracle on the host")
@@ -305,6 +430,7 @@ def main() -> int:
 
     cpu = None
     accuracy = None
+    ref64 = None
     if not args.no_cpu and args.cpu_frames > 0:
         from oracle import oracle as orc
         n = min(args.cpu_frames, args.steps)
@@ -342,6 +468,9 @@ def main() -> int:
                          f"(oracle/odom_oracle.c) with OpenMP over rows"}
         log(f"accuracy {accuracy}")
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_vs_plain(args, cam, poses, views, L, h, odo, reset, fps, est, ref64)
     batched = None if args.no_batch else batched_pairs(args)
 
     line = {
@@ -367,6 +496,7 @@ def main() -> int:
         "drift_vs_truth": {"mean_mm": round(1e3 * drift_t, 3), "mean_deg": round(drift_r, 4)},
         "last_stats": stats,
         "shapes": shapes,
+        "learned": learned,
         "batched": batched,
     }
     print(json.dumps(line), flush=True)

@@ -64,6 +64,8 @@ typedef struct odom_stats {
     int gn_iterations;         /* Gauss-Newton iterations run */
     int degenerate;            /* last solve had no information (identity update) */
     int reverted;              /* the 0.3 m revert was taken */
+    int seed_used;             /* odom_track_seeded: the last call started from the prediction */
+    int seed_calls, seed_kept; /* seeded calls so far, and how many kept the prediction */
 } odom_stats;
 
 typedef struct odom_ctx *odom_handle;
@@ -209,6 +211,25 @@ int odom_batch_set_graph(odom_batch b, int enable);
 /* As odom_profile_track, for one batched call. */
 int odom_batch_profile_track(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3,
                              float us_a[3], float us_b[3], int n_a[3], int n_b[3], float *us_total);
+
+/* ---- learned warm start (no counterpart in the reference, whose tracker
+ * always starts from the previous pose, RGBDOdometryef.cpp:267-401) ---- */
+/* The handle's HIP stream (hipStream_t), so a caller can order its own work
+ * (e.g. the warm-start network, slam_cnn.h) with the handle's. */
+int odom_get_stream(odom_handle h, void **stream);
+/* Enqueue the warm-start network input (bf16 [1][60][80][4], device pointer
+ * x) from the handle's level-2 model depth and intensity and current depth
+ * and intensity (640x480 frames only). */
+int odom_init_warmstart_input(odom_handle h, void *x);
+/* getIncrementalTransformation from a predicted start: pred (device, 6
+ * floats) is the camera motion since the previous pose as (t / t_scale,
+ * rotation vector / r_scale); the solve starts at that motion with the
+ * schedule iters[3] (iterations at levels 0, 1, 2) and no SO3 pass.  With
+ * select, the prediction is kept only if the level-2 ICP association from it
+ * is not worse (inliers, mean squared residual) than from the previous pose;
+ * otherwise the call starts from the previous pose.  Asynchronous. */
+int odom_track_seeded(odom_handle h, const float *pred, float t_scale, float r_scale, const int iters[3],
+                      int rgb_only, float icp_weight, int select);
 
 /* Device name / arch of the current device, for logs. */
 int odom_device_info(char *name, int name_len, char *arch, int arch_len);

@@ -762,8 +762,33 @@ static void d2f(const double *a, float *b, int n)
 }
 
 /* RGBDOdometryef.cpp:267-595 */
+/* currentT = [Rprev|tprev] * resultRt^-1 with resultRt taken to float
+ * (RGBDOdometryef.cpp:563-575) */
+static void pose_from_result(const double resultRt[16], const float Rprev[9], const float tprev[3], float Rcurr[9],
+                             float tcurr[3])
+{
+    float Rf[9], tf[3], Ri[9], ti[3];
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) Rf[a * 3 + b] = (float)resultRt[a * 4 + b];
+        tf[a] = (float)resultRt[a * 4 + 3];
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Ri[a * 3 + b] = Rf[b * 3 + a];
+    for (int a = 0; a < 3; ++a) ti[a] = -(Ri[a * 3] * tf[0] + Ri[a * 3 + 1] * tf[1] + Ri[a * 3 + 2] * tf[2]);
+    mm3f(Rprev, Ri, Rcurr);
+    for (int a = 0; a < 3; ++a)
+        tcurr[a] = Rprev[a * 3] * ti[0] + Rprev[a * 3 + 1] * ti[1] + Rprev[a * 3 + 2] * ti[2] + tprev[a];
+}
+
 void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, int pyramid,
                int fastOdom, int so3)
+{
+    const int iters[NPYR] = {fastOdom ? 3 : 10, pyramid ? 5 : 0, pyramid ? 4 : 0};
+    orc_track_ex(o, t, R, rgbOnly, icpWeight, iters, so3, NULL);
+}
+
+void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
+                  int so3, const double *seed)
 {
     const int icp = !rgbOnly && icpWeight > 0;
     const int rgb = rgbOnly || icpWeight < 100;
@@ -830,9 +855,7 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
     memcpy(o->so3R, resultR, sizeof resultR);
 
     int iterations[NPYR];
-    iterations[0] = fastOdom ? 3 : 10;
-    iterations[1] = pyramid ? 5 : 0;
-    iterations[2] = pyramid ? 4 : 0;
+    for (int l = 0; l < NPYR; ++l) iterations[l] = iters[l];
 
     float Rprev_inv[9];
     inv3f(Rprev, Rprev_inv);
@@ -841,6 +864,12 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
     if (so3)
         for (int x = 0; x < 3; ++x)
             for (int y = 0; y < 3; ++y) resultRt[x * 4 + y] = resultR[x * 3 + y];
+    if (seed) {
+        /* a seeded increment (no counterpart in the reference): the first
+         * iteration already starts from it, for the ICP term too */
+        memcpy(resultRt, seed, sizeof resultRt);
+        pose_from_result(resultRt, Rprev, tprev, Rcurr, tcurr);
+    }
 
     float residual[2] = {0.0f, 0.0f};
     for (int i = NPYR - 1; i >= 0; --i) {
@@ -936,18 +965,8 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
             upd[15] = 1.0;
             mm4d(upd, resultRt, nrt);
             memcpy(resultRt, nrt, sizeof nrt);
-            float Rf[9], tf[3], Ri[9], ti[3];
-            for (int a = 0; a < 3; ++a) {
-                for (int b = 0; b < 3; ++b) Rf[a * 3 + b] = (float)resultRt[a * 4 + b];
-                tf[a] = (float)resultRt[a * 4 + 3];
-            }
             /* rgbOdom.inverse(): [R^T | -R^T t]; currentT = [Rprev|tprev] * inv (:567-575) */
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) Ri[a * 3 + b] = Rf[b * 3 + a];
-            for (int a = 0; a < 3; ++a) ti[a] = -(Ri[a * 3] * tf[0] + Ri[a * 3 + 1] * tf[1] + Ri[a * 3 + 2] * tf[2]);
-            mm3f(Rprev, Ri, Rcurr);
-            for (int a = 0; a < 3; ++a)
-                tcurr[a] = Rprev[a * 3] * ti[0] + Rprev[a * 3 + 1] * ti[1] + Rprev[a * 3 + 2] * ti[2] + tprev[a];
+            pose_from_result(resultRt, Rprev, tprev, Rcurr, tcurr);
         }
     }
 

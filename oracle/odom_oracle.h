@@ -155,6 +155,12 @@ void orc_init_first_rgb(orc_odom *o, const uint8_t *rgb, int channels);
 /* RGBDOdometryef.cpp:267-595.  t and R (row-major) in/out. */
 void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight,
                int pyramid, int fastOdom, int so3);
+/* The same with an explicit schedule (iterations per level, 0..2) and an
+ * optional seeded increment `seed` (row-major 4x4, the resultRt convention:
+ * current = prev * seed^-1); NULL for none.  No counterpart in the reference:
+ * the warm-start form. */
+void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
+                  int so3, const double *seed);
 /* stats[0..5] = lastICPError, lastICPCount, lastRGBError, lastRGBCount,
  * lastSO3Error, lastSO3Count; stats[6] = number of SO3 iterations run. */
 void orc_get_stats(const orc_odom *o, float stats[7]);

@@ -60,6 +60,7 @@ def lib():
         L.orc_get_covariance.argtypes = [_V, _V]
         L.orc_get_covariance.restype = ctypes.c_int
         L.orc_get_so3_rotation.argtypes = [_V, _V]
+        L.orc_track_ex.argtypes = [_V, _V, _V, ctypes.c_int, ctypes.c_float, _V, ctypes.c_int, _V]
         for n, rt in (("orc_buf_f32", _F), ("orc_buf_u8", _U8), ("orc_buf_s16", _S16), ("orc_buf_u16", _U16)):
             getattr(L, n).restype = rt
             getattr(L, n).argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
@@ -329,6 +330,17 @@ class Odometry:
         t = _c(trans, np.float32).copy()
         R = _c(rot, np.float32).copy()
         lib().orc_track(self._h, _p(t), _p(R), int(rgbOnly), icpWeight, int(pyramid), int(fastOdom), int(so3))
+        return t, R
+
+    def track_ex(self, trans, rot, iters=(10, 5, 4), rgbOnly=False, icpWeight=10.0, so3=False, seed=None):
+        """Explicit schedule and optional seeded increment (4x4, resultRt
+        convention: current = prev * seed^-1)."""
+        t = _c(trans, np.float32).copy()
+        R = _c(rot, np.float32).copy()
+        it = np.ascontiguousarray(iters, dtype=np.int32)
+        sd = None if seed is None else _c(seed, np.float64)
+        lib().orc_track_ex(self._h, _p(t), _p(R), int(rgbOnly), icpWeight, _p(it), int(so3),
+                           None if sd is None else _p(sd))
         return t, R
 
     def stats(self):

@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["odom_kernels.hip", "odom_host.cpp", "cnn_kernels.hip"]
-HIP_HEADERS = ["odom_device.h", "odom_internal.h", "cnn_internal.h"]
+HIP_HEADERS = ["odom_device.h", "odom_internal.h"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -1,0 +1,756 @@
+// cnn_kernels.hip -- the learned warm start: a small BF16 convnet on MFMA.
+//
+// Input: 2 x (80x60) depth + intensity, NHWC [N][60][80][4] bf16 in [0, 1]
+// (model depth, model intensity, current depth, current intensity).
+// Six 3x3 stride-2 convolutions (pad 1) with ReLU, 4 -> 32 -> 64 -> 96 ->
+// 128 -> 128 -> 128 channels (60x80 -> 30x40 -> 15x20 -> 8x10 -> 4x5 -> 2x3
+// -> 1x2), global average pooling, FC 128 -> 256 (ReLU) -> 6: a scaled se(3)
+// increment.  No counterpart in the reference (its only network is the
+// unrelated YOLACT segmenter); the tracker uses the prediction as the initial
+// increment of a Gauss-Newton solve.
+//
+// Every convolution and linear layer runs as one GEMM on MFMA
+// (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32 accumulation): forward
+// Y = im2col(X) W, weight gradient dW = im2col(X)^T dY (split over the
+// reduction and summed in a fixed order), input gradient dX = col2im(dY W^T).
+// Master weights and Adam moments are fp32; the GEMMs read bf16 copies.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "slam_cnn.h"
+
+namespace cnn {
+
+typedef uint16_t bf16;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline bf16 f2bf(float x)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);                              // round to nearest even
+    return (bf16)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(bf16 h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ============================================================= GEMM
+// C[m][n] = sum_k A(m, k) B(k, n); A(m, k) = TA ? A[k][m] : A[m][k],
+// B(k, n) = TB ? B[n][k] : B[k][n].  64x64 block tile, 32-deep k tiles, four
+// waves each owning a 32x32 quarter as 2x2 MFMA 16x16 tiles.  Both operands
+// are staged in LDS k-contiguous ([m][k] and [n][k]) so each lane's MFMA
+// fragment (4 consecutive k) is one 8-byte read.
+//   splits == 1: C = acc (+ bias[n]) (ReLU) stored as fp32 (Cf) or bf16 (Cb);
+//   splits  > 1: the k range is cut into gridDim.z chunks of kchunk and chunk z
+//                stores its partial sum to Cf + z * M * ldc (no bias).
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+                                              float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc, int M, int N,
+                                              int K, int kchunk, const float* __restrict__ bias, int relu)
+{
+    __shared__ __align__(16) bf16 As[BM][LDK];
+    __shared__ __align__(16) bf16 Bs[BN][LDK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    v4f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        // ---- A tile: rows m0..m0+63, k0..k0+31
+        if (!TA) {
+            const int r = t >> 2, ks = (t & 3) * 8;
+            const int m = m0 + r;
+            bf16 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = k0 + ks + q;
+                v[q] = (m < M && k < kend) ? A[(size_t)m * lda + k] : (bf16)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[r][ks + q] = v[q];
+        } else {
+            const int kr = t >> 3, ms = (t & 7) * 8;
+            const int k = k0 + kr;
+            bf16 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int m = m0 + ms + q;
+                v[q] = (m < M && k < kend) ? A[(size_t)k * lda + m] : (bf16)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[ms + q][kr] = v[q];
+        }
+        // ---- B tile: k0..k0+31, cols n0..n0+63
+        if (!TB) {
+            const int kr = t >> 3, ns = (t & 7) * 8;
+            const int k = k0 + kr;
+            bf16 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int n = n0 + ns + q;
+                v[q] = (n < N && k < kend) ? B[(size_t)k * ldb + n] : (bf16)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[ns + q][kr] = v[q];
+        } else {
+            const int r = t >> 2, ks = (t & 3) * 8;
+            const int n = n0 + r;
+            bf16 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = k0 + ks + q;
+                v[q] = (n < N && k < kend) ? B[(size_t)n * ldb + k] : (bf16)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[r][ks + q] = v[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            v4s a[2], b[2];
+            const int kq = kk + 4 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = *(const v4s*)&As[wm * 32 + i * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = *(const v4s*)&Bs[wn * 32 + j * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // ---- epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (m >= M || n >= N) continue;
+                float v = acc[i][j][r];
+                if (split) {
+                    Cf[((size_t)blockIdx.z * M + m) * ldc + n] = v;
+                    continue;
+                }
+                if (bias) v += bias[n];
+                if (relu) v = fmaxf(v, 0.0f);
+                if (Cb) Cb[(size_t)m * ldc + n] = f2bf(v);
+                else Cf[(size_t)m * ldc + n] = v;
+            }
+}
+
+// Sum of split partials in split order: out[i] = sum_z part[z * n + i].
+__global__ __launch_bounds__(256) void k_split_sum(const float* __restrict__ part, int splits, size_t n,
+                                                   float* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z) s += part[(size_t)z * n + i];
+    out[i] = s;
+}
+
+// ============================================================= conv helpers
+// im2col of a 3x3 / stride 2 / pad 1 convolution on NHWC bf16:
+// Xcol[m][k], m = (n, ho, wo), k = (kh * 3 + kw) * C + c, zero outside the
+// image and for k >= 9 C (padding of the reduction to Kp).
+__global__ __launch_bounds__(256) void k_im2col(const bf16* __restrict__ X, int Nb, int H, int W, int C, int Ho,
+                                                int Wo, int Kp, bf16* __restrict__ Xcol)
+{
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)Nb * Ho * Wo * Kp;
+    if (idx >= total) return;
+    const int k = (int)(idx % Kp);
+    const size_t m = idx / Kp;
+    const int wo = (int)(m % Wo);
+    const int ho = (int)((m / Wo) % Ho);
+    const int n = (int)(m / ((size_t)Wo * Ho));
+    bf16 v = 0;
+    if (k < 9 * C) {
+        const int c = k % C, kw = (k / C) % 3, kh = k / (3 * C);
+        const int hi = 2 * ho - 1 + kh, wi = 2 * wo - 1 + kw;
+        if (hi >= 0 && hi < H && wi >= 0 && wi < W) v = X[(((size_t)n * H + hi) * W + wi) * C + c];
+    }
+    Xcol[idx] = v;
+}
+
+// col2im (the input gradient), as a gather: dX[n][h][w][c] = sum over the
+// taps (kh, kw) with h = 2 ho - 1 + kh, w = 2 wo - 1 + kw in range of
+// dXcol[(n, ho, wo)][(kh * 3 + kw) * C + c], in tap order.
+__global__ __launch_bounds__(256) void k_col2im(const float* __restrict__ dXcol, int Nb, int H, int W, int C, int Ho,
+                                                int Wo, int Kp, float* __restrict__ dX)
+{
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)Nb * H * W * C;
+    if (idx >= total) return;
+    const int c = (int)(idx % C);
+    const int w = (int)((idx / C) % W);
+    const int h = (int)((idx / ((size_t)C * W)) % H);
+    const int n = (int)(idx / ((size_t)C * W * H));
+    float s = 0.0f;
+    for (int kh = 0; kh < 3; ++kh) {
+        const int hh = h + 1 - kh;
+        if (hh < 0 || (hh & 1) || (hh >> 1) >= Ho) continue;
+        for (int kw = 0; kw < 3; ++kw) {
+            const int ww = w + 1 - kw;
+            if (ww < 0 || (ww & 1) || (ww >> 1) >= Wo) continue;
+            const size_t m = ((size_t)n * Ho + (hh >> 1)) * Wo + (ww >> 1);
+            s += dXcol[m * Kp + (kh * 3 + kw) * C + c];
+        }
+    }
+    dX[idx] = s;
+}
+
+// dZ = dY where the layer's output was positive (ReLU), else 0; bf16.
+__global__ __launch_bounds__(256) void k_relu_grad(const float* __restrict__ dY, const bf16* __restrict__ Y, size_t n,
+                                                   bf16* __restrict__ dZ, float* __restrict__ dZf)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float g = (bf2f(Y[i]) > 0.0f) ? dY[i] : 0.0f;
+    dZ[i] = f2bf(g);
+    if (dZf) dZf[i] = g;
+}
+
+// Column sums of a [M][N] fp32 matrix (bias gradients) in two fixed-order
+// stages: chunk partials, then their sum.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ G, int M, int N, int rows_per,
+                                                     float* __restrict__ part)
+{
+    const int n = blockIdx.y * 256 + threadIdx.x;
+    if (n >= N) return;
+    const int r0 = blockIdx.x * rows_per, r1 = min(M, r0 + rows_per);
+    float s = 0.0f;
+    for (int r = r0; r < r1; ++r) s += G[(size_t)r * N + n];
+    part[(size_t)blockIdx.x * N + n] = s;
+}
+
+// Global average pooling over P positions: G[n][c] = mean_p Y[n][p][c].
+__global__ __launch_bounds__(256) void k_gap_fwd(const bf16* __restrict__ Y, int Nb, int P, int C, bf16* __restrict__ G)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Nb * C) return;
+    const int n = i / C, c = i - n * C;
+    float s = 0.0f;
+    for (int p = 0; p < P; ++p) s += bf2f(Y[((size_t)n * P + p) * C + c]);
+    G[i] = f2bf(s / (float)P);
+}
+
+__global__ __launch_bounds__(256) void k_gap_bwd(const float* __restrict__ dG, int Nb, int P, int C,
+                                                 float* __restrict__ dY)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Nb * P * C) return;
+    const int c = i % C, n = i / (P * C);
+    dY[i] = dG[n * C + c] / (float)P;
+}
+
+// Weighted squared error over the first D of each row of O (stride ldo):
+// loss_n = sum_j w_j (O - T)^2; dO = 2 w_j (O - T) / N as bf16 (and fp32).
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ O, int ldo, const float* __restrict__ T,
+                                              const float* __restrict__ wts, int Nb, int D, float* __restrict__ loss_n,
+                                              bf16* __restrict__ dO, float* __restrict__ dOf)
+{
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= Nb) return;
+    float l = 0.0f;
+    for (int j = 0; j < ldo; ++j) {
+        float g = 0.0f;
+        if (j < D) {
+            const float e = O[n * ldo + j] - T[n * D + j];
+            l += wts[j] * e * e;
+            g = 2.0f * wts[j] * e / (float)Nb;
+        }
+        dO[n * ldo + j] = f2bf(g);
+        dOf[n * ldo + j] = g;
+    }
+    loss_n[n] = l;
+}
+
+// Adam with decoupled weight decay on fp32 master weights; writes the bf16
+// copy the GEMMs read.  Entries with mask 0 (reduction padding) stay 0.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, bf16* __restrict__ pb,
+                                              const uint8_t* __restrict__ mask, size_t n, float lr, float b1, float b2,
+                                              float eps, float bc1, float bc2, float wd)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (mask && !mask[i]) {
+        p[i] = 0.0f;
+        pb[i] = 0;
+        return;
+    }
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    const float pi = p[i] * (1.0f - lr * wd) - lr * upd;
+    p[i] = pi;
+    pb[i] = f2bf(pi);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ a, size_t n, bf16* __restrict__ b)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+// Network input from frame data at twice the input resolution (2W x 2H):
+// each channel is the mean of the valid values of a 2x2 block (depth valid
+// when positive and finite), depth in metres / 5 clamped to [0, 1], intensity
+// / 255.  Channels: model depth (float metres), model intensity, current
+// depth (uint16 millimetres), current intensity.
+__global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ dA, const uint8_t* __restrict__ iA,
+                                                 const uint16_t* __restrict__ dB, const uint8_t* __restrict__ iB,
+                                                 int Nb, int W, int H, size_t frame_stride, bf16* __restrict__ X)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Nb * W * H) return;
+    const int n = i / (W * H), p = i - n * (W * H);
+    const int y = p / W, x = p - y * W;
+    const int W2 = 2 * W;
+    const size_t base = (size_t)n * frame_stride;
+    float sa = 0.f, sb = 0.f, ia = 0.f, ib = 0.f;
+    int ca = 0, cb = 0;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const size_t q = base + (size_t)(2 * y + dy) * W2 + (2 * x + dx);
+            const float za = dA[q];
+            if (za > 0.0f && za < 1e30f) { sa += za; ++ca; }
+            const float zb = (float)dB[q] * 0.001f;
+            if (zb > 0.0f) { sb += zb; ++cb; }
+            ia += (float)iA[q];
+            ib += (float)iB[q];
+        }
+    float v[4];
+    v[0] = ca ? fminf(sa / (float)ca / 5.0f, 1.0f) : 0.0f;
+    v[1] = ia / (4.0f * 255.0f);
+    v[2] = cb ? fminf(sb / (float)cb / 5.0f, 1.0f) : 0.0f;
+    v[3] = ib / (4.0f * 255.0f);
+    for (int c = 0; c < 4; ++c) X[(size_t)i * 4 + c] = f2bf(v[c]);
+}
+
+}  // namespace cnn
+
+namespace slam {
+void cnn_launch_prepare(const float* depth_a, const uint8_t* image_a, const uint16_t* depth_b, const uint8_t* image_b,
+                        int n, size_t frame_stride, void* x, hipStream_t s)
+{
+    hipLaunchKernelGGL(cnn::k_prepare, dim3((unsigned)((n * 80 * 60 + 255) / 256)), dim3(256), 0, s, depth_a, image_a,
+                       depth_b, image_b, n, 80, 60, frame_stride, (cnn::bf16*)x);
+}
+}  // namespace slam
+
+namespace cnn {
+
+// ============================================================= network
+struct ConvSpec {
+    int cin, cout, hi, wi, ho, wo;
+};
+constexpr int kConvs = 6;
+const ConvSpec kSpec[kConvs] = {{4, 32, 60, 80, 30, 40},  {32, 64, 30, 40, 15, 20},  {64, 96, 15, 20, 8, 10},
+                                {96, 128, 8, 10, 4, 5},   {128, 128, 4, 5, 2, 3},  {128, 128, 2, 3, 1, 2}};
+constexpr int kFc1In = 128, kFc1Out = 256, kOut = 6, kOutPad = 8;
+
+struct Param {
+    size_t off, n;
+};
+
+}  // namespace cnn
+
+using namespace cnn;
+
+struct cnn_net_s {
+    int max_batch = 0;
+    hipStream_t stream = nullptr;    // null: the legacy default stream
+    // parameters (flat): per conv W [Kp][Cout] + b [Cout]; fc1 W [128][256] + b; fc2 W [256][8] + b [8]
+    Param convW[kConvs], convB[kConvs], fc1W, fc1B, fc2W, fc2B;
+    int Kp[kConvs];
+    size_t nparam = 0;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    bf16* pb = nullptr;
+    uint8_t* mask = nullptr;
+    int step = 0;
+    // activations of the last forward (for backward), sized for max_batch
+    bf16* xcol[kConvs] = {};
+    bf16* y[kConvs] = {};
+    bf16 *gap = nullptr, *h1 = nullptr;
+    float* out = nullptr;            // [N][8]
+    // backward scratch
+    float *dy = nullptr, *dx = nullptr, *dcol = nullptr, *split = nullptr, *colpart = nullptr;
+    bf16 *dz = nullptr, *dob = nullptr;
+    float *dof = nullptr, *lossn = nullptr, *wts = nullptr;
+    size_t split_elems = 0, colpart_elems = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(hipError_t e, const char* what)
+{
+    fprintf(stderr, "slam_cnn: %s failed: %s\n", what, hipGetErrorString(e));
+    return CNN_ERR_HIP;
+}
+
+#define CK(x)                                \
+    do {                                     \
+        hipError_t e_ = (x);                 \
+        if (e_ != hipSuccess) return fail(e_, #x); \
+    } while (0)
+
+template <class T>
+int dalloc(cnn_net_s* c, T** p, size_t n)
+{
+    CK(hipMalloc((void**)p, n * sizeof(T) + 16));
+    c->allocs.push_back(*p);
+    return CNN_OK;
+}
+
+int launch_check(const char* what)
+{
+    hipError_t e = hipPeekAtLastError();
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(e, what);
+    }
+    return CNN_OK;
+}
+
+inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// C = op(A) op(B) with the conventions of k_gemm; splits > 1 reduces through
+// c->split (fp32 [splits][M][ldc]) into Cf.
+int gemm(cnn_net_s* c, bool ta, const bf16* A, int lda, bool tb, const bf16* B, int ldb, float* Cf, bf16* Cb,
+         int ldc, int M, int N, int K, const float* bias, bool relu, int splits)
+{
+    if (splits < 1) splits = 1;
+    const int kchunk = ((cdiv(K, splits) + BK - 1) / BK) * BK;
+    splits = cdiv(K, kchunk);
+    if (splits > 1 && (size_t)splits * M * ldc > c->split_elems) return CNN_ERR_ARG;
+    dim3 grid(cdiv(M, BM), cdiv(N, BN), splits);
+    float* out = splits > 1 ? c->split : Cf;
+    bf16* outb = splits > 1 ? nullptr : Cb;
+    const float* bb = splits > 1 ? nullptr : bias;
+    const int rl = (splits > 1) ? 0 : (relu ? 1 : 0);
+    if (!ta && !tb)
+        hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, out, outb, ldc, M, N, K,
+                           kchunk, bb, rl);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, out, outb, ldc, M, N, K,
+                           kchunk, bb, rl);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, out, outb, ldc, M, N, K,
+                           kchunk, bb, rl);
+    else
+        hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, out, outb, ldc, M, N, K,
+                           kchunk, bb, rl);
+    if (splits > 1) {
+        const size_t n = (size_t)M * ldc;
+        hipLaunchKernelGGL(k_split_sum, g1(n), dim3(256), 0, c->stream, c->split, splits, n, Cf);
+    }
+    return launch_check("gemm");
+}
+
+int colsum(cnn_net_s* c, const float* G, int M, int N, float* out)
+{
+    const int chunks = std::min(256, std::max(1, M / 64));
+    const int rows_per = cdiv(M, chunks);
+    if ((size_t)chunks * N > c->colpart_elems) return CNN_ERR_ARG;
+    hipLaunchKernelGGL(k_colsum_part, dim3(chunks, cdiv(N, 256)), dim3(256), 0, c->stream, G, M, N, rows_per,
+                       c->colpart);
+    hipLaunchKernelGGL(k_split_sum, g1(N), dim3(256), 0, c->stream, c->colpart, chunks, (size_t)N, out);
+    return launch_check("colsum");
+}
+
+int splits_for(int K, int MN)
+{
+    // enough blocks along the reduction when the output tile count is small
+    const int tiles = std::max(1, MN / (BM * BN));
+    int s = std::max(1, 512 / tiles);
+    s = std::min(s, std::max(1, K / 512));
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cnn_create(int max_batch, cnn_net* out)
+{
+    if (!out || max_batch < 1 || max_batch > 4096) return CNN_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CNN_ERR_NODEV;
+    cnn_net_s* c = new cnn_net_s();
+    c->max_batch = max_batch;
+    size_t off = 0;
+    auto take = [&](Param& p, size_t n) {
+        p.off = off;
+        p.n = n;
+        off += (n + 63) & ~(size_t)63;
+    };
+    for (int l = 0; l < kConvs; ++l) {
+        c->Kp[l] = ((9 * kSpec[l].cin + BK - 1) / BK) * BK;
+        take(c->convW[l], (size_t)c->Kp[l] * kSpec[l].cout);
+        take(c->convB[l], kSpec[l].cout);
+    }
+    take(c->fc1W, (size_t)kFc1In * kFc1Out);
+    take(c->fc1B, kFc1Out);
+    take(c->fc2W, (size_t)kFc1Out * kOutPad);
+    take(c->fc2B, kOutPad);
+    c->nparam = off;
+    const int N = max_batch;
+    size_t big = 0, split = 0;
+    int rc = CNN_OK;
+    rc = rc ? rc : dalloc(c, &c->p, off);
+    rc = rc ? rc : dalloc(c, &c->g, off);
+    rc = rc ? rc : dalloc(c, &c->m, off);
+    rc = rc ? rc : dalloc(c, &c->v, off);
+    rc = rc ? rc : dalloc(c, &c->pb, off);
+    rc = rc ? rc : dalloc(c, &c->mask, off);
+    for (int l = 0; l < kConvs && !rc; ++l) {
+        const size_t M = (size_t)N * kSpec[l].ho * kSpec[l].wo;
+        rc = dalloc(c, &c->xcol[l], M * c->Kp[l]);
+        rc = rc ? rc : dalloc(c, &c->y[l], M * kSpec[l].cout);
+        big = std::max(big, M * (size_t)c->Kp[l]);
+        big = std::max(big, (size_t)N * kSpec[l].hi * kSpec[l].wi * kSpec[l].cin);
+        split = std::max(split, (size_t)c->Kp[l] * kSpec[l].cout);
+    }
+    split = std::max(split, (size_t)kFc1In * kFc1Out);
+    c->split_elems = split * 64;
+    c->colpart_elems = (size_t)256 * 256;
+    rc = rc ? rc : dalloc(c, &c->gap, (size_t)N * kFc1In);
+    rc = rc ? rc : dalloc(c, &c->h1, (size_t)N * kFc1Out);
+    rc = rc ? rc : dalloc(c, &c->out, (size_t)N * kOutPad);
+    rc = rc ? rc : dalloc(c, &c->dy, big);
+    rc = rc ? rc : dalloc(c, &c->dx, big);
+    rc = rc ? rc : dalloc(c, &c->dcol, big);
+    rc = rc ? rc : dalloc(c, &c->dz, big);
+    rc = rc ? rc : dalloc(c, &c->split, c->split_elems);
+    rc = rc ? rc : dalloc(c, &c->colpart, c->colpart_elems);
+    rc = rc ? rc : dalloc(c, &c->dob, (size_t)N * kOutPad);
+    rc = rc ? rc : dalloc(c, &c->dof, (size_t)N * kOutPad);
+    rc = rc ? rc : dalloc(c, &c->lossn, (size_t)N);
+    rc = rc ? rc : dalloc(c, &c->wts, (size_t)kOutPad);
+    if (rc) {
+        cnn_destroy(c);
+        return rc;
+    }
+    // parameter mask: 0 on the reduction padding rows of each conv weight and
+    // on the padding columns of fc2
+    std::vector<uint8_t> mk(off, 1);
+    for (int l = 0; l < kConvs; ++l)
+        for (int k = 9 * kSpec[l].cin; k < c->Kp[l]; ++k)
+            for (int n = 0; n < kSpec[l].cout; ++n) mk[c->convW[l].off + (size_t)k * kSpec[l].cout + n] = 0;
+    for (int k = 0; k < kFc1Out; ++k)
+        for (int n = kOut; n < kOutPad; ++n) mk[c->fc2W.off + (size_t)k * kOutPad + n] = 0;
+    for (int n = kOut; n < kOutPad; ++n) mk[c->fc2B.off + n] = 0;
+    if (hipMemcpy(c->mask, mk.data(), off, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->p, 0, off * 4) != hipSuccess || hipMemset(c->g, 0, off * 4) != hipSuccess ||
+        hipMemset(c->m, 0, off * 4) != hipSuccess || hipMemset(c->v, 0, off * 4) != hipSuccess ||
+        hipMemset(c->pb, 0, off * 2) != hipSuccess) {
+        cnn_destroy(c);
+        return CNN_ERR_HIP;
+    }
+    *out = c;
+    return CNN_OK;
+}
+
+int cnn_destroy(cnn_net c)
+{
+    if (!c) return CNN_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    for (void* p : c->allocs) (void)hipFree(p);
+    delete c;
+    return CNN_OK;
+}
+
+int cnn_set_stream(cnn_net c, void* stream)
+{
+    if (!c) return CNN_ERR_ARG;
+    c->stream = (hipStream_t)stream;
+    return CNN_OK;
+}
+
+int cnn_num_params(cnn_net c, size_t* n)
+{
+    if (!c || !n) return CNN_ERR_ARG;
+    *n = c->nparam;
+    return CNN_OK;
+}
+
+int cnn_param_layout(cnn_net c, size_t* offsets, size_t* sizes, int cap)
+{
+    if (!c || !offsets || !sizes || cap < 2 * kConvs + 4) return CNN_ERR_ARG;
+    int q = 0;
+    for (int l = 0; l < kConvs; ++l) {
+        offsets[q] = c->convW[l].off; sizes[q++] = c->convW[l].n;
+        offsets[q] = c->convB[l].off; sizes[q++] = c->convB[l].n;
+    }
+    offsets[q] = c->fc1W.off; sizes[q++] = c->fc1W.n;
+    offsets[q] = c->fc1B.off; sizes[q++] = c->fc1B.n;
+    offsets[q] = c->fc2W.off; sizes[q++] = c->fc2W.n;
+    offsets[q] = c->fc2B.off; sizes[q++] = c->fc2B.n;
+    return q;
+}
+
+int cnn_set_params(cnn_net c, const float* host, size_t n)
+{
+    if (!c || !host || n != c->nparam) return CNN_ERR_ARG;
+    CK(hipMemcpy(c->p, host, n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_to_bf16, g1(n), dim3(256), 0, c->stream, c->p, n, c->pb);
+    CK(hipMemset(c->m, 0, n * 4));
+    CK(hipMemset(c->v, 0, n * 4));
+    c->step = 0;
+    CK(hipDeviceSynchronize());
+    return launch_check("set_params");
+}
+
+int cnn_get_params(cnn_net c, float* host, size_t n)
+{
+    if (!c || !host || n != c->nparam) return CNN_ERR_ARG;
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(host, c->p, n * 4, hipMemcpyDeviceToHost));
+    return CNN_OK;
+}
+
+int cnn_get_grads(cnn_net c, float* host, size_t n)
+{
+    if (!c || !host || n != c->nparam) return CNN_ERR_ARG;
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(host, c->g, n * 4, hipMemcpyDeviceToHost));
+    return CNN_OK;
+}
+
+int cnn_prepare_input(cnn_net c, const float* depth_a, const uint8_t* image_a, const uint16_t* depth_b,
+                      const uint8_t* image_b, int n, size_t frame_stride, void* x)
+{
+    if (!c || !depth_a || !image_a || !depth_b || !image_b || !x || n < 1) return CNN_ERR_ARG;
+    hipLaunchKernelGGL(k_prepare, g1((size_t)n * 80 * 60), dim3(256), 0, c->stream, depth_a, image_a, depth_b,
+                       image_b, n, 80, 60, frame_stride, (bf16*)x);
+    return launch_check("prepare_input");
+}
+
+// Forward of N samples (x: NHWC bf16 [N][60][80][4], device); out (device,
+// fp32 [N][6]) may be null.  Keeps the activations for cnn_backward.
+int cnn_forward(cnn_net c, const void* x, int N, float* out)
+{
+    if (!c || !x || N < 1 || N > c->max_batch) return CNN_ERR_ARG;
+    const bf16* in = (const bf16*)x;
+    int rc;
+    for (int l = 0; l < kConvs; ++l) {
+        const ConvSpec& s = kSpec[l];
+        const int M = N * s.ho * s.wo;
+        const size_t tot = (size_t)M * c->Kp[l];
+        hipLaunchKernelGGL(k_im2col, g1(tot), dim3(256), 0, c->stream, in, N, s.hi, s.wi, s.cin, s.ho, s.wo, c->Kp[l],
+                           c->xcol[l]);
+        if ((rc = gemm(c, false, c->xcol[l], c->Kp[l], false, c->pb + c->convW[l].off, s.cout, nullptr, c->y[l], s.cout,
+                       M, s.cout, c->Kp[l], c->p + c->convB[l].off, true, 1)))
+            return rc;
+        in = c->y[l];
+    }
+    hipLaunchKernelGGL(k_gap_fwd, g1((size_t)N * kFc1In), dim3(256), 0, c->stream, c->y[kConvs - 1], N, 2, kFc1In,
+                       c->gap);
+    if ((rc = gemm(c, false, c->gap, kFc1In, false, c->pb + c->fc1W.off, kFc1Out, nullptr, c->h1, kFc1Out, N, kFc1Out,
+                   kFc1In, c->p + c->fc1B.off, true, 1)))
+        return rc;
+    if ((rc = gemm(c, false, c->h1, kFc1Out, false, c->pb + c->fc2W.off, kOutPad, c->out, nullptr, kOutPad, N, kOutPad,
+                   kFc1Out, c->p + c->fc2B.off, false, 1)))
+        return rc;
+    if (out) CK(hipMemcpy2DAsync(out, kOut * 4, c->out, kOutPad * 4, kOut * 4, N, hipMemcpyDeviceToDevice, c->stream));
+    return launch_check("forward");
+}
+
+// Gradients of the weighted squared error against target (device fp32
+// [N][6]) with weights w (host, 6) for the last forward of N samples: loss
+// per sample into loss_n (device, optional), parameter gradients into the
+// net's gradient buffer.
+int cnn_backward(cnn_net c, int N, const float* target, const float* w, float* loss_n)
+{
+    if (!c || !target || !w || N < 1 || N > c->max_batch) return CNN_ERR_ARG;
+    int rc;
+    float wp[kOutPad] = {0};
+    for (int j = 0; j < kOut; ++j) wp[j] = w[j];
+    CK(hipMemcpyAsync(c->wts, wp, sizeof wp, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_loss, g1(N), dim3(256), 0, c->stream, c->out, kOutPad, target, c->wts, N, kOut, c->lossn,
+                       c->dob, c->dof);
+    if (loss_n) CK(hipMemcpyAsync(loss_n, c->lossn, N * 4, hipMemcpyDeviceToDevice, c->stream));
+    // fc2: dW2 = H1^T dO, db2 = colsum dO, dH1 = dO W2^T
+    if ((rc = gemm(c, true, c->h1, kFc1Out, false, c->dob, kOutPad, c->g + c->fc2W.off, nullptr, kOutPad, kFc1Out,
+                   kOutPad, N, nullptr, false, splits_for(N, kFc1Out * kOutPad))))
+        return rc;
+    if ((rc = colsum(c, c->dof, N, kOutPad, c->g + c->fc2B.off))) return rc;
+    if ((rc = gemm(c, false, c->dob, kOutPad, true, c->pb + c->fc2W.off, kOutPad, c->dy, nullptr, kFc1Out, N, kFc1Out,
+                   kOutPad, nullptr, false, 1)))
+        return rc;
+    // fc1 (ReLU): dZ1 = dH1 * (H1 > 0); dW1 = G^T dZ1; db1; dG = dZ1 W1^T
+    hipLaunchKernelGGL(k_relu_grad, g1((size_t)N * kFc1Out), dim3(256), 0, c->stream, c->dy, c->h1,
+                       (size_t)N * kFc1Out, c->dz, c->dx);
+    if ((rc = gemm(c, true, c->gap, kFc1In, false, c->dz, kFc1Out, c->g + c->fc1W.off, nullptr, kFc1Out, kFc1In,
+                   kFc1Out, N, nullptr, false, splits_for(N, kFc1In * kFc1Out))))
+        return rc;
+    if ((rc = colsum(c, c->dx, N, kFc1Out, c->g + c->fc1B.off))) return rc;
+    if ((rc = gemm(c, false, c->dz, kFc1Out, true, c->pb + c->fc1W.off, kFc1Out, c->dcol, nullptr, kFc1In, N, kFc1In,
+                   kFc1Out, nullptr, false, 1)))
+        return rc;
+    // GAP backward into dy of the last conv
+    hipLaunchKernelGGL(k_gap_bwd, g1((size_t)N * 2 * kFc1In), dim3(256), 0, c->stream, c->dcol, N, 2, kFc1In, c->dy);
+    for (int l = kConvs - 1; l >= 0; --l) {
+        const ConvSpec& s = kSpec[l];
+        const int M = N * s.ho * s.wo;
+        const size_t ny = (size_t)M * s.cout;
+        // dZ = dY * (Y > 0); fp32 copy in dx for the bias sums
+        hipLaunchKernelGGL(k_relu_grad, g1(ny), dim3(256), 0, c->stream, c->dy, c->y[l], ny, c->dz, c->dx);
+        if ((rc = gemm(c, true, c->xcol[l], c->Kp[l], false, c->dz, s.cout, c->g + c->convW[l].off, nullptr, s.cout,
+                       c->Kp[l], s.cout, M, nullptr, false, splits_for(M, c->Kp[l] * s.cout))))
+            return rc;
+        if ((rc = colsum(c, c->dx, M, s.cout, c->g + c->convB[l].off))) return rc;
+        if (l == 0) break;
+        // dXcol = dZ W^T, then col2im into dy of the layer below
+        if ((rc = gemm(c, false, c->dz, s.cout, true, c->pb + c->convW[l].off, s.cout, c->dcol, nullptr, c->Kp[l], M,
+                       c->Kp[l], s.cout, nullptr, false, 1)))
+            return rc;
+        const size_t nx = (size_t)N * s.hi * s.wi * s.cin;
+        hipLaunchKernelGGL(k_col2im, g1(nx), dim3(256), 0, c->stream, c->dcol, N, s.hi, s.wi, s.cin, s.ho, s.wo,
+                           c->Kp[l], c->dy);
+    }
+    return launch_check("backward");
+}
+
+int cnn_adam_step(cnn_net c, float lr, float wd)
+{
+    if (!c || !(lr >= 0.0f)) return CNN_ERR_ARG;
+    c->step++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.0f - powf(b1, (float)c->step), bc2 = 1.0f - powf(b2, (float)c->step);
+    hipLaunchKernelGGL(k_adam, g1(c->nparam), dim3(256), 0, c->stream, c->p, c->g, c->m, c->v, c->pb, c->mask,
+                       c->nparam, lr, b1, b2, eps, bc1, bc2, wd);
+    return launch_check("adam");
+}
+
+}  // extern "C"

@@ -92,6 +92,11 @@ struct TrackState {
     float* pose_log;                 // optional device ring of [R|t] results
     int pose_log_cap;
     int frame_count;                 // track calls completed
+    // warm start: the increment the seeded call starts from (resultRt
+    // convention), whether the prediction was kept over the plain start, and
+    // counters of seeded calls and kept predictions
+    double seedRt[16];
+    int seed_used, seed_calls, seed_kept;
 };
 
 // Uniform options of one track call.
@@ -102,6 +107,7 @@ struct TrackParams {
     float sobelScale;
     float maxDepthDelta;
     int iters[kLevels];
+    int seeded;                      // start from TrackState::seedRt (warm start)
 };
 
 // Per pair: level maps, state, scratch.

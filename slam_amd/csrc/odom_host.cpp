@@ -676,6 +676,40 @@ int odom_track(odom_handle c, float trans[3], float rot[9], int rgbOnly, float i
     return odom_get_pose(c, trans, rot);
 }
 
+int odom_get_stream(odom_handle c, void** stream)
+{
+    if (!c || !stream) return ODOM_ERR_ARG;
+    *stream = (void*)c->stream;
+    return ODOM_OK;
+}
+
+int odom_init_warmstart_input(odom_handle c, void* x)
+{
+    if (!c || !x) return ODOM_ERR_ARG;
+    if (c->w[2] != 2 * 80 || c->h[2] != 2 * 60) return ODOM_ERR_ARG;  // 640x480 frames
+    if (!(c->icp_cur && c->rgb_last && c->rgb_next)) return ODOM_ERR_STATE;
+    cnn_launch_prepare(c->lastDepth[2], c->lastImage[2], c->depth[2], next_img(c, 2), 1, (size_t)c->n[2], x,
+                       c->stream);
+    return check_launch(c);
+}
+
+int odom_track_seeded(odom_handle c, const float* pred, float t_scale, float r_scale, const int iters[3], int rgbOnly,
+                      float icpWeight, int select)
+{
+    if (!c || !pred || !iters || !(t_scale > 0.0f) || !(r_scale > 0.0f)) return ODOM_ERR_ARG;
+    TrackParams P = make_params(c, rgbOnly, icpWeight, 1, 0, 0);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    for (int l = 0; l < kLevels; ++l) {
+        if (iters[l] < 0 || iters[l] > 63) return ODOM_ERR_ARG;
+        P.iters[l] = iters[l];
+    }
+    P.seeded = 1;
+    if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
+    if (select && !(c->icp_cur && c->icp_model)) return ODOM_ERR_STATE;
+    launch_seed_select(c->d_pair[c->parity], 1, pred, t_scale, r_scale, select, P, c->stream);
+    return enqueue_track(c, P);
+}
+
 int odom_get_stats(odom_handle c, odom_stats* s)
 {
     if (!c || !s) return ODOM_ERR_ARG;
@@ -692,6 +726,9 @@ int odom_get_stats(odom_handle c, odom_stats* s)
     s->gn_iterations = t.iters_done;
     s->degenerate = t.degenerate;
     s->reverted = t.reverted;
+    s->seed_used = t.seed_used;
+    s->seed_calls = t.seed_calls;
+    s->seed_kept = t.seed_kept;
     return ODOM_OK;
 }
 

@@ -90,6 +90,13 @@ struct StepLaunch {
 // SO3 pre-alignment; max_px_l2: largest level-2 pixel count over the pairs
 void launch_so3(const PairDesc* pairs, int npairs, int max_px_l2, hipStream_t s);
 void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
+// warm start: seed from the network prediction (6 floats per pair, device),
+// optionally kept only when the level-2 ICP association is not worse than plain
+void launch_seed_select(const PairDesc* pairs, int npairs, const float* pred, float t_scale, float r_scale,
+                        int select, const TrackParams& P, hipStream_t s);
+// network input (cnn_kernels.hip) from 160x120 level-2 buffers
+void cnn_launch_prepare(const float* depth_a, const uint8_t* image_a, const uint16_t* depth_b, const uint8_t* image_b,
+                        int n, size_t frame_stride, void* x, hipStream_t s);
 // Gauss-Newton iteration k (global index over the call) at (level, iter).
 // mode 0: tracking (the prologue finishes iteration k-1, which ran at
 // prev_level); mode 1: parity pass with the constants of state slot 0,

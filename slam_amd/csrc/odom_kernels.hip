@@ -1651,6 +1651,118 @@ void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackP
 // RGBDOdometryef.cpp:278-282, 386-401: pose prior and its inverse, and the
 // state before the first iteration (the increment seeded with the SO3
 // rotation) in slot 1.
+// current = [Rprev | tprev] * resultRt^-1 with resultRt taken to float
+// (RGBDOdometryef.cpp:563-575), as the solve's pose update.
+__device__ void pose_from_result(const double* Rt, const float* Rprev, const float* tprev, float* Rcurr,
+                                 float* tcurr)
+{
+    float Rf[9], tf[3], Ri[9], ti[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = (float)Rt[r * 4 + c];
+        tf[r] = (float)Rt[r * 4 + 3];
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Ri[r * 3 + c] = Rf[c * 3 + r];
+    for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r * 3] * tf[0] + Ri[r * 3 + 1] * tf[1] + Ri[r * 3 + 2] * tf[2]);
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            Rcurr[r * 3 + c] = Rprev[r * 3] * Ri[c] + Rprev[r * 3 + 1] * Ri[3 + c] + Rprev[r * 3 + 2] * Ri[6 + c];
+        tcurr[r] = Rprev[r * 3] * ti[0] + Rprev[r * 3 + 1] * ti[1] + Rprev[r * 3 + 2] * ti[2] + tprev[r];
+    }
+}
+
+// Warm start.  The predicted camera motion (pred: scaled translation and
+// rotation vector, 6 floats per pair) becomes the seed resultRt = M^-1.  With
+// `select`, the prediction is scored against the plain start (identity
+// increment) by the ICP association at level 2 -- inliers and mean squared
+// point-to-plane residual -- and kept only when it is not worse on either
+// (1 % slack), else the call starts plain.  One workgroup per pair.
+__global__ __launch_bounds__(256) void k_seed_select(const PairDesc* __restrict__ pairs,
+                                                     const float* __restrict__ pred, float t_scale, float r_scale,
+                                                     int select, TrackParams P)
+{
+    __shared__ double sSeed[16];
+    __shared__ float sRc[2][9], sTc[2][3], sRpi[9], sTp[3];
+    __shared__ float sRed[4][4];
+    const PairDesc& D = pairs[blockIdx.x];
+    TrackState* st = D.st;
+    if (threadIdx.x == 0) {
+        float Rprev[9], tprev[3];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) Rprev[r * 3 + c] = st->pose_out[r * 4 + c];
+            tprev[r] = st->pose_out[r * 4 + 3];
+        }
+        inv3f(Rprev, sRpi);
+        for (int r = 0; r < 3; ++r) sTp[r] = tprev[r];
+        const float* q = pred + 6 * blockIdx.x;
+        double w[3], R[9], t[3];
+        for (int r = 0; r < 3; ++r) {
+            t[r] = (double)q[r] * (double)t_scale;
+            w[r] = (double)q[3 + r] * (double)r_scale;
+        }
+        rodrigues(w, R);
+        // M = [R | t] (prev -> current camera); seed = M^-1 = [R^T | -R^T t]
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) sSeed[r * 4 + c] = R[c * 3 + r];
+            sSeed[r * 4 + 3] = -(R[0 * 3 + r] * t[0] + R[1 * 3 + r] * t[1] + R[2 * 3 + r] * t[2]);
+        }
+        sSeed[12] = sSeed[13] = sSeed[14] = 0.0;
+        sSeed[15] = 1.0;
+        for (int a = 0; a < 9; ++a) sRc[0][a] = Rprev[a];
+        for (int a = 0; a < 3; ++a) sTc[0][a] = tprev[a];
+        pose_from_result(sSeed, Rprev, tprev, sRc[1], sTc[1]);
+    }
+    __syncthreads();
+    int keep = 1;
+    if (select) {
+        const LevelDesc& L = D.lv[2];
+        const int N = L.w * L.h;
+        const size_t P3 = (size_t)N;
+        float cnt[2] = {0.f, 0.f}, r2[2] = {0.f, 0.f};
+        for (int cand = 0; cand < 2; ++cand) {
+            IcpC ic;
+            for (int a = 0; a < 9; ++a) { ic.Rc[a] = sRc[cand][a]; ic.Rpi[a] = sRpi[a]; }
+            for (int a = 0; a < 3; ++a) { ic.tc[a] = sTc[cand][a]; ic.tp[a] = sTp[a]; }
+            ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
+            ic.dist = P.distThres; ic.angle = P.angleThres;
+            for (int i = threadIdx.x; i < N; i += 256) {
+                Px6 src, dst;
+                float vg[3], vcp[3], row[7];
+                icp_load(L, i, P3, src);
+                const int j = icp_project(L, ic, src, vg, vcp);
+                if (j < 0) continue;
+                icp_gather(L, j, P3, dst);
+                if (icp_row(ic, src, vg, vcp, dst, row)) {
+                    cnt[cand] += 1.0f;
+                    r2[cand] += row[6] * row[6];
+                }
+            }
+        }
+        float v[4] = {cnt[0], r2[0], cnt[1], r2[1]};
+        for (int q = 0; q < 4; ++q) v[q] = wave_total_f(v[q]);
+        if ((threadIdx.x & 63) == 63)
+            for (int q = 0; q < 4; ++q) sRed[threadIdx.x >> 6][q] = v[q];
+        __syncthreads();
+        float tot[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int wv = 0; wv < 4; ++wv)
+            for (int q = 0; q < 4; ++q) tot[q] += sRed[wv][q];
+        const float m0 = tot[1] / fmaxf(tot[0], 1.0f), m1 = tot[3] / fmaxf(tot[2], 1.0f);
+        keep = (tot[2] >= 0.99f * tot[0]) && (m1 <= 1.01f * m0);
+    }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 16; ++a) st->seedRt[a] = keep ? sSeed[a] : ((a % 5 == 0) ? 1.0 : 0.0);
+        st->seed_used = keep;
+        st->seed_calls++;
+        st->seed_kept += keep;
+    }
+}
+
+void launch_seed_select(const PairDesc* pairs, int npairs, const float* pred, float t_scale, float r_scale,
+                        int select, const TrackParams& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seed_select, dim3(npairs), dim3(256), 0, s, pairs, pred, t_scale, r_scale, select, P);
+}
+
 __global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1668,6 +1780,12 @@ __global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, Tr
     if (P.so3)
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) S.resultRt[a * 4 + b] = st->so3R[a * 3 + b];
+    if (P.seeded) {
+        // the warm start: resultRt = seed, and (unlike the SO3 seed of the
+        // reference) the current estimate prev * seed^-1 from the first pass
+        for (int a = 0; a < 16; ++a) S.resultRt[a] = st->seedRt[a];
+        pose_from_result(S.resultRt, st->Rprev, st->tprev, S.Rcurr, S.tcurr);
+    }
     for (int a = 0; a < 36; ++a) S.lastA[a] = st->lastA[a];
     for (int a = 0; a < 6; ++a) S.lastb[a] = st->lastb[a];
     S.lastICPError = st->lastICPError;

@@ -46,7 +46,8 @@ class Stats(ctypes.Structure):
                 ("last_rgb_error", ctypes.c_float), ("last_rgb_count", ctypes.c_float),
                 ("last_so3_error", ctypes.c_float), ("last_so3_count", ctypes.c_float),
                 ("so3_iterations", ctypes.c_int), ("gn_iterations", ctypes.c_int),
-                ("degenerate", ctypes.c_int), ("reverted", ctypes.c_int)]
+                ("degenerate", ctypes.c_int), ("reverted", ctypes.c_int), ("seed_used", ctypes.c_int),
+                ("seed_calls", ctypes.c_int), ("seed_kept", ctypes.c_int)]
 
 
 # Every symbol include/slam_odom.h declares.
@@ -61,6 +62,7 @@ EXPORTS = [
     "odom_batch_create", "odom_batch_destroy", "odom_track_batch", "odom_batch_synchronize",
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
+    "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded",
 ]
 
 _V = ctypes.c_void_p
@@ -114,6 +116,10 @@ def lib() -> ctypes.CDLL:
     L.odom_batch_set_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.odom_batch_set_graph.argtypes = [_V, ctypes.c_int]
     L.odom_batch_set_split_solve.argtypes = [_V, ctypes.c_int]
+    L.odom_get_stream.argtypes = [_V, _P(_V)]
+    L.odom_init_warmstart_input.argtypes = [_V, _V]
+    L.odom_track_seeded.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _V, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_int]
     L.odom_batch_autotune.argtypes = [_V, ctypes.c_int, ctypes.c_float]
     L.odom_batch_get_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]
     L.odom_batch_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
@@ -265,6 +271,27 @@ class RGBDOdometry:
                                               int(fastOdom), int(so3)))
         return t, R.reshape(3, 3)
 
+    @property
+    def stream(self) -> int:
+        """The handle's HIP stream (as an integer pointer)."""
+        st = _V()
+        _check("odom_get_stream", lib().odom_get_stream(self._h, ctypes.byref(st)))
+        return st.value or 0
+
+    def initWarmStartInput(self, x) -> None:
+        """Enqueue the warm-start network input (torch bf16 [1][60][80][4] on
+        the GPU) from this handle's model and current level-2 buffers."""
+        _check("odom_init_warmstart_input", lib().odom_init_warmstart_input(self._h, x.data_ptr()))
+
+    def track_seeded(self, pred, iters=(5, 2, 2), rgbOnly=False, icpWeight=10.0, select=True,
+                     t_scale=0.05, r_scale=0.05) -> None:
+        """Asynchronous tracking call starting from a predicted motion (device
+        tensor of 6 floats), with an explicit schedule and no SO3 pass."""
+        it = np.ascontiguousarray(iters, dtype=np.int32)
+        _check("odom_track_seeded", lib().odom_track_seeded(self._h, pred.data_ptr(), float(t_scale), float(r_scale),
+                                                            it.ctypes.data, int(rgbOnly), float(icpWeight),
+                                                            int(select)))
+
     def track_async(self, rgbOnly=False, icpWeight=10.0, pyramid=True, fastOdom=False, so3=True) -> None:
         _check("odom_track_async", lib().odom_track_async(self._h, int(rgbOnly), float(icpWeight), int(pyramid),
                                                           int(fastOdom), int(so3)))
@@ -298,7 +325,8 @@ class RGBDOdometry:
                 "lastRGBError": s.last_rgb_error, "lastRGBCount": s.last_rgb_count,
                 "lastSO3Error": s.last_so3_error, "lastSO3Count": s.last_so3_count,
                 "so3Iterations": s.so3_iterations, "gnIterations": s.gn_iterations,
-                "degenerate": s.degenerate, "reverted": s.reverted}
+                "degenerate": s.degenerate, "reverted": s.reverted, "seedUsed": s.seed_used,
+                "seedCalls": s.seed_calls, "seedKept": s.seed_kept}
 
     @property
     def lastA(self) -> np.ndarray:

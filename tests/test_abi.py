@@ -12,12 +12,13 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "slam_odom.h")
+CNN_HEADER = os.path.join(ROOT, "include", "slam_cnn.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(path=HEADER, prefix="odom_"):
+    text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"^\s*int\s+(odom_[a-z0-9_]+)\s*\(", text, flags=re.M))
+    return set(re.findall(r"^\s*int\s+(" + prefix + r"[a-z0-9_]+)\s*\(", text, flags=re.M))
 
 
 @pytest.fixture(scope="module")
@@ -36,8 +37,13 @@ def test_header_and_binding_agree():
     assert names == set(odometry.EXPORTS)
 
 
+def test_cnn_header_and_binding_agree():
+    from slam_amd import warmstart
+    assert declared(CNN_HEADER, "cnn_") == set(warmstart.EXPORTS)
+
+
 def test_library_exports_every_declared_symbol(L):
-    for name in declared():
+    for name in declared() | declared(CNN_HEADER, "cnn_"):
         assert hasattr(L, name), name
     # and they are real dynamic symbols of the shared object
     from slam_amd import _native

@@ -1,0 +1,97 @@
+"""The warm-start network's HIP kernels (bf16 MFMA GEMMs, fp32 accumulation)
+against the FP64 oracle (oracle/cnn_oracle.py), and a short training run.
+
+The GPU rounds weights, inputs, the stored activations of every layer and the
+gradients it feeds its GEMMs to bf16.  Against the oracle evaluated with the
+same rounding points (everything else FP64) each tensor must agree to a
+relative L2 error of 1e-2 (fp32 accumulation order, and the rare bf16 value
+that rounds to a neighbour); against the exact FP64 network, forward and the
+last layers' gradients to 4e-2.  A transposed operand, a wrong im2col index or
+a missing ReLU mask gives errors of order 1.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    from oracle import cnn_oracle as co
+    from slam_amd.warmstart import WarmStartNet
+    net = WarmStartNet(max_batch=8)
+    net.init_params(seed=5)
+    flat = net.get_params()
+    # the GEMMs read bf16 weights (biases stay fp32)
+    p = co.unpack(flat.astype(np.float64))
+    for k in p:
+        if k.endswith(".W"):
+            p[k] = bf16_round(p[k]).astype(np.float64)
+    rng = np.random.default_rng(6)
+    x = bf16_round(rng.uniform(0, 1, size=(4, 60, 80, 4)))
+    xt = torch.from_numpy(x).to(torch.bfloat16).cuda().contiguous()
+    target = rng.normal(size=(4, 6)).astype(np.float32)
+    return dict(net=net, co=co, p=p, x=x, xt=xt, target=target)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def cos(a, b):
+    return float(np.dot(a.ravel(), b.ravel()) / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle(setup):
+    s = setup
+    out = s["net"].forward(s["xt"]).cpu().numpy()
+    ref, _ = s["co"].forward(s["p"], s["x"])
+    assert out.shape == (4, 6)
+    assert rel(out, ref) < 4e-2 and cos(out, ref) > 0.998, (rel(out, ref), cos(out, ref))
+
+
+def test_forward_matches_rounded_oracle(setup):
+    s = setup
+    out = s["net"].forward(s["xt"]).cpu().numpy()
+    ref, _ = s["co"].forward(s["p"], s["x"], rnd=s["co"].bf16)
+    assert rel(out, ref) < 1e-2, rel(out, ref)
+
+
+def test_backward_matches_oracle(setup):
+    import torch
+    s = setup
+    net, co = s["net"], s["co"]
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0], np.float32)
+    net.forward(s["xt"])
+    loss = net.backward(4, torch.from_numpy(s["target"]).cuda(), w).cpu().numpy()
+    G = co.unpack(net.get_grads().astype(np.float64))
+    ref_loss, g = co.loss_and_grads(s["p"], s["x"], s["target"], w, rnd=co.bf16)
+    assert abs(loss.mean() - ref_loss) < 1e-2 * ref_loss
+    for name in g:
+        a, b = G[name], g[name]
+        assert rel(a, b) < 1e-2 and cos(a, b) > 0.9999, (name, rel(a, b), cos(a, b))
+    # against the exact network: the layers next to the loss
+    _, ge = co.loss_and_grads(s["p"], s["x"], s["target"], w)
+    for name in ("fc2.W", "fc2.b", "fc1.W", "fc1.b", "conv5.W"):
+        assert rel(G[name], ge[name]) < 4e-2, (name, rel(G[name], ge[name]))
+
+
+def test_training_reduces_loss():
+    """A few hundred rendered pairs, a few epochs on the GPU: the validation
+    error of the predicted motion falls well below the zero-motion guess."""
+    from slam_amd import warmstart as ws
+    net = ws.WarmStartNet(max_batch=64)
+    net.init_params(seed=1)
+    tr = ws.to_device(ws.render_pairs(768, seed=11))
+    va = ws.to_device(ws.render_pairs(128, seed=12))
+    hist = ws.train(net, tr, va, epochs=12, batch=64, lr=2e-3)
+    first, last = hist[0], hist[-1]
+    assert last["train_loss"] < 0.5 * first["train_loss"]
+    assert last["val_trans_err_mm"] < 0.8 * last["zero_guess_trans_mm"]
+    assert last["val_rot_err_deg"] < 0.8 * last["zero_guess_rot_deg"]
