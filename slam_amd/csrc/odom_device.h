@@ -72,12 +72,23 @@ struct IterState {
     int pad_;
 };
 
+// State of the SO3 pre-alignment loop (RGBDOdometryef.cpp:294-380) between
+// its launches; iteration k lives in slot k & 1.
+struct So3Slot {
+    double resultR[9], lastResultR[9];
+    float R_lr[9];
+    float lastError, lastCount;      // the loop's comparison values
+    float err, cnt;                  // lastSO3Error / lastSO3Count so far
+    int iters, done;
+};
+
 // Device-resident tracking state of one pair (one getIncrementalTransformation
 // call).
 struct TrackState {
     float Rprev[9], tprev[3], Rprev_inv[9];
     float pose_out[12];              // [R | t] row-major result of the call
     double so3R[9];                  // SO3 pre-alignment
+    So3Slot so3s[2];
     IterState it[2];
     // results of the call, copied by the end kernel (RGBDOdometryef.h:62-70)
     double lastA[36], lastb[6];

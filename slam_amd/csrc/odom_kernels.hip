@@ -1938,103 +1938,136 @@ constexpr int kSO3Threads = 512;
 
 // With LDS, both level-2 images are staged in LDS (dynamic shared memory of
 // 2 * max(w*h) bytes) before the iterations.
-template <bool LDS>
-__global__ __launch_bounds__(kSO3Threads) void k_so3(const PairDesc* __restrict__ pairs)
+// The SO3 loop as up to 11 launches of kSO3Blocks workgroups per pair.
+// Launch `it` (< 10) first evaluates iteration it-1 from its block partials
+// -- FP64 totals in a fixed order, the convergence / divergence tests and the
+// 3x3 solve of the loop body, identically in every block -- then, unless the
+// loop has ended, sums iteration it over its share of the level-2 pixels.
+// Launch 10 only evaluates iteration 9 and stores the result.  Block 0 of a
+// pair stores the loop state of each launch (slot it & 1).
+constexpr int kSO3Blocks = 16;
+
+__device__ void so3_evaluate(So3Slot& S, const double* tot)
 {
-    extern __shared__ uint8_t s_img[];
-    __shared__ float s_red[kSO3Threads / 64][kSO3];
-    __shared__ double s_tot[kSO3];
-    __shared__ float sH[9], sKi[9], sKR[9];
-    __shared__ int s_done;
-    const PairDesc& D = pairs[blockIdx.x];
-    TrackState* st = D.st;
-    const LevelDesc& L = D.lv[2];
-    const int w = L.w, h = L.h, n = w * h;
-    const uint8_t* last = D.so3_last;
-    const uint8_t* next = D.so3_next;
-    if (LDS) {
-        for (int i = threadIdx.x; i < n; i += kSO3Threads) {
-            s_img[i] = last[i];
-            s_img[n + i] = next[i];
-        }
-        last = s_img;
-        next = s_img + n;
+    ++S.iters;
+    const float res0 = (float)tot[9], res1 = (float)tot[10];
+    const float err = sqrtf(res0) / res1;
+    S.err = err;
+    S.cnt = res1;
+    if (err < S.lastError && S.lastCount == res1) {
+        S.done = 1;
+    } else if (err > S.lastError + 0.001f) {
+        S.err = S.lastError;
+        S.cnt = S.lastCount;
+        for (int k = 0; k < 9; ++k) S.resultR[k] = S.lastResultR[k];
+        S.done = 1;
     }
-    double K[9], Ki[9];
-    for (int k = 0; k < 9; ++k) { K[k] = L.K[k]; Ki[k] = L.Ki[k]; }
-    // loop state, meaningful in thread 0
-    float R_lr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    double resultR[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    double lastResultR[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    float lastError = FLT_MAX / 2, lastCount = FLT_MAX / 2;
-    int iters = 0;
-    for (int it = 0; it < 10; ++it) {
-        if (threadIdx.x == 0) {
-            double KR[9], Hd[9];
-            mm3d(K, resultR, KR);
-            mm3d(KR, Ki, Hd);
-            for (int k = 0; k < 9; ++k) { sH[k] = (float)Hd[k]; sKi[k] = (float)Ki[k]; sKR[k] = (float)KR[k]; }
-        }
-        __syncthreads();
-        float Hr[9], Kir[9], KRr[9];
-        for (int k = 0; k < 9; ++k) { Hr[k] = sH[k]; Kir[k] = sKi[k]; KRr[k] = sKR[k]; }
-        float acc[kSO3];
-#pragma unroll
-        for (int k = 0; k < kSO3; ++k) acc[k] = 0.0f;
-        for (int idx = threadIdx.x; idx < n; idx += kSO3Threads) {
-            const int y = idx / w, x = idx - y * w;
-            float row[4];
-            if (so3_pixel(last, next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
-        }
-        so3_block_total<kSO3Threads>(acc, s_red, s_tot);
-        if (threadIdx.x == 0) {
-            ++iters;
-            const float res0 = (float)s_tot[9], res1 = (float)s_tot[10];
-            const float err = sqrtf(res0) / res1;
-            st->lastSO3Error = err;
-            st->lastSO3Count = res1;
-            int done = 0;
-            if (err < lastError && lastCount == res1) {
-                done = 1;
-            } else if (err > lastError + 0.001f) {
-                st->lastSO3Error = lastError;
-                st->lastSO3Count = lastCount;
-                for (int k = 0; k < 9; ++k) resultR[k] = lastResultR[k];
-                done = 1;
-            }
-            if (!done) {
-                lastError = err;
-                lastCount = res1;
-                for (int k = 0; k < 9; ++k) lastResultR[k] = resultR[k];
-                double A3[9], b3[3], delta[3], rot[9];
-                unpack_so3(s_tot, A3, b3);
-                ldlt_solve<3>(A3, b3, delta);
-                rodrigues(delta, rot);
-                float rotf[9], nR[9];
-                for (int k = 0; k < 9; ++k) rotf[k] = (float)rot[k];
-                mm3f(rotf, R_lr, nR);
-                for (int k = 0; k < 9; ++k) { R_lr[k] = nR[k]; resultR[k] = (double)nR[k]; }
-            }
-            s_done = done;
-        }
-        __syncthreads();
-        if (s_done) break;
-    }
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 9; ++k) st->so3R[k] = resultR[k];
-        st->so3_iters = iters;
+    if (S.done) return;
+    S.lastError = err;
+    S.lastCount = res1;
+    for (int k = 0; k < 9; ++k) S.lastResultR[k] = S.resultR[k];
+    double A3[9], b3[3], delta[3], rot[9];
+    unpack_so3(tot, A3, b3);
+    ldlt_solve<3>(A3, b3, delta);
+    rodrigues(delta, rot);
+    float rotf[9], nR[9];
+    for (int k = 0; k < 9; ++k) rotf[k] = (float)rot[k];
+    mm3f(rotf, S.R_lr, nR);
+    for (int k = 0; k < 9; ++k) {
+        S.R_lr[k] = nR[k];
+        S.resultR[k] = (double)nR[k];
     }
 }
 
-constexpr int kSO3MaxLds = 60 * 1024;
+__global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __restrict__ pairs, int it)
+{
+    __shared__ So3Slot S;
+    __shared__ double s_tot[kSO3];
+    __shared__ float s_red[kSO3Threads / 64][kSO3];
+    __shared__ float sH[9], sKi[9], sKR[9];
+    const int p = blockIdx.x / kSO3Blocks, lb = blockIdx.x - p * kSO3Blocks;
+    const PairDesc& D = pairs[p];
+    TrackState* st = D.st;
+    const LevelDesc& L = D.lv[2];
+    if (it == 0) {
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 9; ++k) {
+                S.resultR[k] = S.lastResultR[k] = (k % 4 == 0) ? 1.0 : 0.0;
+                S.R_lr[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+            }
+            S.lastError = S.lastCount = FLT_MAX / 2;
+            S.err = S.cnt = 0.0f;
+            S.iters = 0;
+            S.done = 0;
+        }
+        __syncthreads();
+    } else {
+        // issue the partial loads with the state copy (one memory round trip)
+        const float* part = D.part_a[(it - 1) & 1];
+        float pv[kSO3Blocks] = {};
+        const int col = threadIdx.x;
+        if (col < kSO3) {
+#pragma unroll
+            for (int r = 0; r < kSO3Blocks; ++r) pv[r] = part[r * kPartStride + col];
+        }
+        copy_words<kSO3Threads>(&S, &st->so3s[(it - 1) & 1]);
+        __syncthreads();
+        if (!S.done) {
+            if (col < kSO3) {
+                double t = 0.0;
+#pragma unroll
+                for (int r = 0; r < kSO3Blocks; ++r) t += (double)pv[r];
+                s_tot[col] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) so3_evaluate(S, s_tot);
+            __syncthreads();
+        }
+    }
+    if (lb == 0) {
+        copy_words<kSO3Threads>(&st->so3s[it & 1], &S);
+        if (it == 10 && threadIdx.x == 0) {
+            for (int k = 0; k < 9; ++k) st->so3R[k] = S.resultR[k];
+            st->so3_iters = S.iters;
+            st->lastSO3Error = S.err;
+            st->lastSO3Count = S.cnt;
+        }
+    }
+    if (S.done || it == 10) return;
+    if (threadIdx.x == 0) {
+        double KR[9], Hd[9];
+        mm3d(L.K, S.resultR, KR);
+        mm3d(KR, L.Ki, Hd);
+        for (int k = 0; k < 9; ++k) {
+            sH[k] = (float)Hd[k];
+            sKi[k] = (float)L.Ki[k];
+            sKR[k] = (float)KR[k];
+        }
+    }
+    __syncthreads();
+    float Hr[9], Kir[9], KRr[9];
+    for (int k = 0; k < 9; ++k) {
+        Hr[k] = uni(sH[k]);
+        Kir[k] = uni(sKi[k]);
+        KRr[k] = uni(sKR[k]);
+    }
+    const int w = L.w, h = L.h, n = w * h;
+    float acc[kSO3];
+#pragma unroll
+    for (int k = 0; k < kSO3; ++k) acc[k] = 0.0f;
+    for (int idx = lb * kSO3Threads + threadIdx.x; idx < n; idx += kSO3Blocks * kSO3Threads) {
+        const int y = idx / w, x = idx - y * w;
+        float row[4];
+        if (so3_pixel(D.so3_last, D.so3_next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
+    }
+    block_partial<kSO3, kSO3Threads>(acc, D.part_a[it & 1] + (size_t)lb * kPartStride, s_red);
+}
 
 void launch_so3(const PairDesc* pairs, int npairs, int max_px_l2, hipStream_t s)
 {
-    const int lds = 2 * max_px_l2;
-    if (lds <= kSO3MaxLds)
-        hipLaunchKernelGGL((k_so3<true>), dim3(npairs), dim3(kSO3Threads), lds, s, pairs);
-    else
-        hipLaunchKernelGGL((k_so3<false>), dim3(npairs), dim3(kSO3Threads), 0, s, pairs);
+    (void)max_px_l2;
+    for (int it = 0; it <= 10; ++it)
+        hipLaunchKernelGGL(k_so3_iter, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, it);
 }
 
 // One SO3 step with explicit constants (H, K^-1, K R as 27 floats): sums in
