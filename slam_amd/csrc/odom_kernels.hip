@@ -1032,7 +1032,8 @@ __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const d
 #pragma unroll
         for (int q = 0; q < 6; ++q) S.lastb[q] = b[q];
     }
-    const int zeros = ldlt_solve<6>(A, b, x);
+    int zeros = 0;
+    if (!ldlt_spd<6>(A, b, x)) zeros = ldlt_solve<6>(A, b, x);
     double rot[9];
     rodrigues(x + 3, rot);
     double old[16], nrt[16];
@@ -1181,13 +1182,14 @@ __device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int
 }
 
 // FP64 totals of the loaded partials in a fixed order: per thread over its
-// records, then over the record groups in LDS.  tot[0..28] pass A, tot[29..57]
+// records, then per column over the record groups in LDS (each wave a slice
+// of the groups, then the waves' sums).  tot[0..28] pass A, tot[29..57]
 // pass B.
 template <int NT>
 __device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int nb, double* tot,
-                                             double (*red)[NT / 8])
+                                             double (*red)[64], double (*fin)[64])
 {
-    constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT;
+    constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT, G = NT / 64, SL = RG / G;
     const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -1201,22 +1203,27 @@ __device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int 
             s2 += ok ? (double)L.v[a][i].z : 0.0;
             s3 += ok ? (double)L.v[a][i].w : 0.0;
         }
-        red[a * 32 + 4 * c][rg] = s0;
-        red[a * 32 + 4 * c + 1][rg] = s1;
-        red[a * 32 + 4 * c + 2][rg] = s2;
-        red[a * 32 + 4 * c + 3][rg] = s3;
+        red[rg][a * 32 + 4 * c] = s0;
+        red[rg][a * 32 + 4 * c + 1] = s1;
+        red[rg][a * 32 + 4 * c + 2] = s2;
+        red[rg][a * 32 + 4 * c + 3] = s3;
+    }
+    __syncthreads();
+    {
+        const int col = threadIdx.x & 63, w = threadIdx.x >> 6;
+        double t = 0.0;
+#pragma unroll
+        for (int g = 0; g < SL; ++g) t += red[w * SL + g][col];
+        fin[w][col] = t;
     }
     __syncthreads();
     if (threadIdx.x < 64) {
         const int a = threadIdx.x >> 5, k = threadIdx.x & 31;
         if (k < kSE3) {
-            double e = 0.0, o = 0.0;
-#pragma unroll 8
-            for (int g = 0; g < RG; g += 2) {
-                e += red[threadIdx.x][g];
-                o += red[threadIdx.x][g + 1];
-            }
-            tot[a * kSE3 + k] = e + o;
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < G; ++w) t += fin[w][threadIdx.x];
+            tot[a * kSE3 + k] = t;
         }
     }
     __syncthreads();
@@ -1226,7 +1233,8 @@ template <int NT>
 struct Prolog {
     IterState S;
     double fin[2 * kSE3][NT / 64];
-    double red[64][NT / 8];
+    double red[NT / 8][64];
+    double fin2[NT / 64][64];
     double tot[2 * kSE3];
     unsigned int redi[NT / 64][2];
     unsigned int toti[2];
@@ -1265,7 +1273,7 @@ __device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, 
     __syncthreads();
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
     if (run) {
-        partials_sum<NT>(PL, na, nb, W.tot, W.red);
+        partials_sum<NT>(PL, na, nb, W.tot, W.red, W.fin2);
         if (threadIdx.x < 64) {
             if (threadIdx.x == 0) {
                 if (!P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);

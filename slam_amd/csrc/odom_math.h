@@ -184,6 +184,53 @@ ODOM_HD int ldlt_solve(const double* Ain, const double* b, double* x)
     return zeros;
 }
 
+// The same solve for a well-conditioned symmetric positive definite A: LDL^T
+// without pivoting (in exact arithmetic the same x as ldlt_solve; ~6x fewer
+// operations, no row swaps).  Returns false, leaving x unspecified, when a
+// pivot is not above 1e-9 of the largest diagonal entry: the caller then uses
+// ldlt_solve, whose pivoting and zero-pivot rule decide singular systems.
+template <int N>
+ODOM_HD bool ldlt_spd(const double* A, const double* b, double* x)
+{
+    double L[N][N], d[N], id[N], y[N];
+    double amax = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) amax = fmax(amax, fabs(A[i * N + i]));
+    const double tol = 1e-9 * amax;
+    bool ok = amax > 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double dj = A[j * N + j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj = dj - L[j][k] * L[j][k] * d[k];
+        ok = ok && (dj > tol);
+        d[j] = dj;
+        id[j] = 1.0 / dj;
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            double v = A[i * N + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v = v - L[i][k] * L[j][k] * d[k];
+            L[i][j] = v * id[j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double v = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v = v - L[i][k] * y[k];
+        y[i] = v;
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        double v = y[i] * id[i];
+#pragma unroll
+        for (int k = i + 1; k < N; ++k) v = v - L[k][i] * x[k];
+        x[i] = v;
+    }
+    return ok;
+}
+
 // OdometryProvider::rodrigues, utils.h:16-52
 ODOM_HD void rodrigues(const double* src, double* R)
 {
