@@ -304,6 +304,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
     ap.add_argument("--burst-frames", type=int, default=150)
+    ap.add_argument("--no-graph", action="store_true", help="launch tracking calls kernel by kernel")
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--batch-oracle-pairs", type=int, default=6,
                     help="pairs of the batch checked against the FP64 oracle on the host")
@@ -351,6 +352,7 @@ def main() -> int:
     L = lib()
     odo = RGBDOdometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
     h = odo.handle
+    odo.set_graph(not args.no_graph)
     by = ctypes.byref
 
     def reset():

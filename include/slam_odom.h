@@ -121,6 +121,9 @@ int odom_track_async(odom_handle h, int rgb_only, float icp_weight, int pyramid,
 int odom_set_pose(odom_handle h, const float trans[3], const float rot[9]);
 int odom_get_pose(odom_handle h, float trans[3], float rot[9]);
 int odom_synchronize(odom_handle h);
+/* Replay each distinct tracking call (options, schedule and image parity)
+ * from a hipGraph captured on its first use; 0 launches kernel by kernel. */
+int odom_set_graph(odom_handle h, int enable);
 /* Record every call's resulting pose ([R|t], 12 floats row-major) into a
  * device ring of `capacity` entries, slot = call index % capacity; the call
  * index restarts at 0.  capacity 0 turns the log off. */

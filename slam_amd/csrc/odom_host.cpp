@@ -83,6 +83,10 @@ struct odom_ctx {
     // vtmp: the model vertex buffer that initRGB/initRGBModel read
     // (vmaps_tmp, RGBDOdometryef.cpp:208-235) has been filled.
     bool icp_cur = false, icp_model = false, rgb_next = false, rgb_last = false, vtmp = false;
+    // captured tracking calls (odom_set_graph)
+    bool use_graph = false;
+    struct Graph { std::vector<int> key; hipGraphExec_t exec; };
+    std::vector<Graph> graphs;
     hipError_t last_hip = hipSuccess;
 };
 
@@ -114,8 +118,11 @@ int check_launch(odom_ctx* c)
 uint8_t* next_img(odom_ctx* c, int l) { return c->img[c->parity][l]; }
 uint8_t* last_next_img(odom_ctx* c, int l) { return c->img[c->parity ^ 1][l]; }
 
+void drop_graphs(odom_ctx* c);
+
 int upload_pair(odom_ctx* c)
 {
+    drop_graphs(c);
     for (int q = 0; q < 2; ++q) {
         PairDesc& D = c->h_pair[q];
         for (int l = 0; l < kLevels; ++l) {
@@ -325,7 +332,7 @@ void prep_levels(odom_ctx* c, PrepLevels& a)
 // iteration (the solve of iteration k runs in the prologue of pass A of
 // iteration k+1), end, image swap.  With `ev`, an event pair brackets every
 // step launch (ev[2*i], ev[2*i+1] for launch i, A and B alternating).
-int enqueue_track(odom_ctx* c, const TrackParams& P, hipEvent_t* ev = nullptr)
+void track_launches(odom_ctx* c, const TrackParams& P, hipEvent_t* ev)
 {
     if (P.rgb) {
         PrepLevels a{};
@@ -353,6 +360,46 @@ int enqueue_track(odom_ctx* c, const TrackParams& P, hipEvent_t* ev = nullptr)
     }
     const int K = (int)sched.size();
     launch_track_end(dp, 1, P, K, K > 0 ? sched[K - 1] / 64 : 0, c->stream);
+}
+
+std::vector<int> track_key(const odom_ctx* c, const TrackParams& P)
+{
+    int w;
+    memcpy(&w, &P.icpWeight, 4);
+    return {P.icp, P.rgb, P.rgbOnly, P.so3, P.seeded, P.iters[0], P.iters[1], P.iters[2], w, c->parity};
+}
+
+void drop_graphs(odom_ctx* c)
+{
+    if (!c->graphs.empty()) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// One getIncrementalTransformation's launches, replayed from a hipGraph when
+// the handle has graphs on (one graph per option set and image parity,
+// captured on first use), else launched one by one.
+int enqueue_track(odom_ctx* c, const TrackParams& P, hipEvent_t* ev = nullptr)
+{
+    if (c->use_graph && !ev) {
+        const std::vector<int> key = track_key(c, P);
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : c->graphs)
+            if (g.key == key) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            track_launches(c, P, nullptr);
+            HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(c, e);
+            c->graphs.push_back({key, exec});
+        }
+        HIPCHK(c, hipGraphLaunch(exec, c->stream));
+    } else {
+        track_launches(c, P, ev);
+    }
     // std::swap(lastNextImage[i], nextImage[i]) (RGBDOdometryef.cpp:585-591):
     // later launches take the other descriptor
     if (P.so3) c->parity ^= 1;
@@ -506,6 +553,7 @@ int odom_destroy(odom_handle c)
 {
     if (!c) return ODOM_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     if (c->arena) (void)hipFree(c->arena);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -674,6 +722,15 @@ int odom_track(odom_handle c, float trans[3], float rot[9], int rgbOnly, float i
     rc = odom_track_async(c, rgbOnly, icpWeight, pyramid, fastOdom, so3);
     if (rc) return rc;
     return odom_get_pose(c, trans, rot);
+}
+
+int odom_set_graph(odom_handle c, int enable)
+{
+    if (!c) return ODOM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return ODOM_OK;
 }
 
 int odom_get_stream(odom_handle c, void** stream)

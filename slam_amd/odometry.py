@@ -62,7 +62,7 @@ EXPORTS = [
     "odom_batch_create", "odom_batch_destroy", "odom_track_batch", "odom_batch_synchronize",
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
-    "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded",
+    "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph",
 ]
 
 _V = ctypes.c_void_p
@@ -117,6 +117,7 @@ def lib() -> ctypes.CDLL:
     L.odom_batch_set_graph.argtypes = [_V, ctypes.c_int]
     L.odom_batch_set_split_solve.argtypes = [_V, ctypes.c_int]
     L.odom_get_stream.argtypes = [_V, _P(_V)]
+    L.odom_set_graph.argtypes = [_V, ctypes.c_int]
     L.odom_init_warmstart_input.argtypes = [_V, _V]
     L.odom_track_seeded.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _V, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_int]
@@ -270,6 +271,10 @@ class RGBDOdometry:
         _check("odom_track", lib().odom_track(self._h, _p(t), _p(R), int(rgbOnly), float(icpWeight), int(pyramid),
                                               int(fastOdom), int(so3)))
         return t, R.reshape(3, 3)
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay tracking calls from captured hipGraphs."""
+        _check("odom_set_graph", lib().odom_set_graph(self._h, int(enable)))
 
     @property
     def stream(self) -> int:

@@ -233,6 +233,35 @@ def test_sequence_matches_oracle(orc):
             assert np.array_equal(gpu.read_buffer("last_next_image", l), ref.buf("lastNextImage", l))
 
 
+@pytest.mark.parametrize("so3", [True, False])
+def test_graph_replays_match_eager(orc, so3):
+    """Six frames tracked by a handle replaying captured graphs and by one
+    launching eagerly, with the frames' own init calls and a pose read-back
+    (synchronisation) between the replays: every replay equals the eager
+    result bit for bit (with so3 the two image parities use two graphs)."""
+    cam = synth.CAM_320
+    scene = synth.Scene(23)
+    poses = synth.spline_trajectory(7, seed=6)
+    frames = [scene.render(T, cam, noise_seed=200 + i) for i, T in enumerate(poses)]
+    eager, graph = mk(cam), mk(cam)
+    graph.set_graph(True)
+    for odo in (eager, graph):
+        odo.set_pose(poses[0][:3, 3], poses[0][:3, :3])
+        odo.initFirstRGB(frames[0].rgb)
+    for k in range(1, len(frames)):
+        filt = orc.bilateral(frames[k].depth, MAXD)
+        out = []
+        for odo in (eager, graph):
+            t, R = odo.get_pose()
+            odo.initICPModel(frames[k - 1].vert4, frames[k - 1].norm4, CUTOFF, pose_of(t, R).astype(np.float32))
+            odo.initRGBModel(frames[k - 1].rgba)
+            odo.initICP(filt, CUTOFF)
+            odo.initRGB(frames[k].rgb)
+            odo.track_async(False, 10.0, True, False, so3)
+            out.append(odo.get_pose())
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), k
+
+
 def test_degenerate_all_invalid_depth(orc):
     """No valid depth: zero information, identity update, no abort."""
     cam = synth.CAM_320
