@@ -204,41 +204,47 @@ def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain
                                 it.ctypes.data, 0, ctypes.c_float(ICP_WEIGHT), 1)
 
     nsteps = args.steps
-    reset()
-    run_learned(1, 1 + args.warmup, views)
-    odo.synchronize()
-    log_t = torch.zeros((nsteps + 1, 12), dtype=torch.float32, device="cuda")
-    odo.set_pose_log(log_t.data_ptr(), nsteps + 1)
-    reset()
-    torch.cuda.synchronize()
-    odo.synchronize()
-    s0 = odo.stats()
-    t0 = time.perf_counter()
-    run_learned(1, 1 + nsteps, views)
-    odo.synchronize()
-    el = time.perf_counter() - t0
-    s1 = odo.stats()
-    fps = nsteps / el
-    est = log_t.cpu().numpy()[:nsteps]
 
     def pose12(p):
         T = np.eye(4)
         T[:3, :4] = p.reshape(3, 4)
         return T
 
-    kept = (s1["seedKept"] - s0["seedKept"]) / max(1, s1["seedCalls"] - s0["seedCalls"])
-    out = {"workload": "same 640x480 sequence, network init + seeded ICP+RGB, schedule "
-                       f"{iters[0]}/{iters[1]}/{iters[2]}, no SO3, fallback to the previous pose when worse",
-           "frames_per_s": round(fps, 2), "speedup_vs_plain": round(fps / plain_fps, 3),
-           "prediction_kept_frac": round(kept, 3)}
-    if ref64 is not None:
-        n = len(ref64)
-        e = [synth.pose_error(pose12(est[k]), ref64[k]) for k in range(n)]
-        out["vs_fp64_oracle_mean_mm"] = round(1e3 * float(np.mean([q[0] for q in e])), 4)
-        out["vs_fp64_oracle_max_mm"] = round(1e3 * float(np.max([q[0] for q in e])), 4)
-        out["vs_fp64_oracle_mean_deg"] = round(float(np.mean([q[1] for q in e])), 5)
-    dt_l = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, nsteps + 1)]
-    out["drift_vs_truth_mean_mm"] = round(1e3 * float(np.mean([d[0] for d in dt_l])), 3)
+    def timed(schedule) -> dict:
+        it[:] = schedule
+        reset()
+        run_learned(1, 1 + args.warmup, views)
+        odo.synchronize()
+        log_t = torch.zeros((nsteps + 1, 12), dtype=torch.float32, device="cuda")
+        odo.set_pose_log(log_t.data_ptr(), nsteps + 1)
+        reset()
+        torch.cuda.synchronize()
+        odo.synchronize()
+        s0 = odo.stats()
+        t0 = time.perf_counter()
+        run_learned(1, 1 + nsteps, views)
+        odo.synchronize()
+        el = time.perf_counter() - t0
+        s1 = odo.stats()
+        fps = nsteps / el
+        est = log_t.cpu().numpy()[:nsteps]
+        kept = (s1["seedKept"] - s0["seedKept"]) / max(1, s1["seedCalls"] - s0["seedCalls"])
+        r = {"schedule": list(schedule), "frames_per_s": round(fps, 2), "speedup_vs_plain": round(fps / plain_fps, 3),
+             "prediction_kept_frac": round(kept, 3)}
+        if ref64 is not None:
+            n = len(ref64)
+            e = [synth.pose_error(pose12(est[k]), ref64[k]) for k in range(n)]
+            r["vs_fp64_oracle_mean_mm"] = round(1e3 * float(np.mean([q[0] for q in e])), 4)
+            r["vs_fp64_oracle_max_mm"] = round(1e3 * float(np.max([q[0] for q in e])), 4)
+            r["vs_fp64_oracle_mean_deg"] = round(float(np.mean([q[1] for q in e])), 5)
+        dt_l = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, nsteps + 1)]
+        r["drift_vs_truth_mean_mm"] = round(1e3 * float(np.mean([d[0] for d in dt_l])), 3)
+        return r
+
+    out = {"workload": "same 640x480 sequence, network init + seeded ICP+RGB, no SO3, fallback to the previous "
+                       "pose when the level-2 association from the prediction is worse",
+           "short_schedule": timed(iters), "full_schedule": timed([10, 5, 4])}
+    it[:] = iters
 
     # fast motion: bursts up to 8 cm / 6 deg per frame, both forms vs truth
     nb = args.burst_frames
@@ -305,6 +311,8 @@ def main() -> int:
     ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
     ap.add_argument("--burst-frames", type=int, default=150)
     ap.add_argument("--no-graph", action="store_true", help="launch tracking calls kernel by kernel")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 4 device copies of 256 MiB (the reference kernel of counter runs)")
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--batch-oracle-pairs", type=int, default=6,
                     help="pairs of the batch checked against the FP64 oracle on the host")
@@ -323,6 +331,14 @@ def main() -> int:
 
     name, arch = device_info()
     log(f"device {name} {arch}")
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 4 x 256 MiB device copies")
+        del src, dst
     cam = synth.CAM_640
     nframes = max(args.steps, args.warmup) + 1
     scene = synth.Scene(args.seed)

@@ -44,6 +44,20 @@ __host__ __device__ inline float bf2f(bf16 h) { return __builtin_bit_cast(float,
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ============================================================= GEMM
+// Eight consecutive bf16 from p[i..i+7] when `row_ok`, of which the first
+// `avail` exist (zero-filled beyond); one 16-byte load when all eight do.
+// Every operand's leading dimension and base offset are multiples of 8
+// elements, so full segments are 16-byte aligned.
+__device__ __forceinline__ void load8(const bf16* __restrict__ p, size_t i, bool row_ok, int avail, bf16 (&v)[8])
+{
+    if (row_ok && avail >= 8) {
+        *(uint4*)v = *(const uint4*)(p + i);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = (row_ok && q < avail) ? p[i + q] : (bf16)0;
+}
+
 // C[m][n] = sum_k A(m, k) B(k, n); A(m, k) = TA ? A[k][m] : A[m][k],
 // B(k, n) = TB ? B[n][k] : B[k][n].  64x64 block tile, 32-deep k tiles, four
 // waves each owning a 32x32 quarter as 2x2 MFMA 16x16 tiles.  Both operands
@@ -78,22 +92,13 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
             const int r = t >> 2, ks = (t & 3) * 8;
             const int m = m0 + r;
             bf16 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int k = k0 + ks + q;
-                v[q] = (m < M && k < kend) ? A[(size_t)m * lda + k] : (bf16)0;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[r][ks + q] = v[q];
+            load8(A, (size_t)m * lda + k0 + ks, m < M, kend - (k0 + ks), v);
+            *(uint4*)&As[r][ks] = *(const uint4*)v;
         } else {
             const int kr = t >> 3, ms = (t & 7) * 8;
             const int k = k0 + kr;
             bf16 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int m = m0 + ms + q;
-                v[q] = (m < M && k < kend) ? A[(size_t)k * lda + m] : (bf16)0;
-            }
+            load8(A, (size_t)k * lda + m0 + ms, k < kend, M - (m0 + ms), v);
 #pragma unroll
             for (int q = 0; q < 8; ++q) As[ms + q][kr] = v[q];
         }
@@ -102,24 +107,15 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
             const int kr = t >> 3, ns = (t & 7) * 8;
             const int k = k0 + kr;
             bf16 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int n = n0 + ns + q;
-                v[q] = (n < N && k < kend) ? B[(size_t)k * ldb + n] : (bf16)0;
-            }
+            load8(B, (size_t)k * ldb + n0 + ns, k < kend, N - (n0 + ns), v);
 #pragma unroll
             for (int q = 0; q < 8; ++q) Bs[ns + q][kr] = v[q];
         } else {
             const int r = t >> 2, ks = (t & 3) * 8;
             const int n = n0 + r;
             bf16 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int k = k0 + ks + q;
-                v[q] = (n < N && k < kend) ? B[(size_t)n * ldb + k] : (bf16)0;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[r][ks + q] = v[q];
+            load8(B, (size_t)n * ldb + k0 + ks, n < N, kend - (k0 + ks), v);
+            *(uint4*)&Bs[r][ks] = *(const uint4*)v;
         }
         __syncthreads();
 #pragma unroll
@@ -170,6 +166,23 @@ __global__ __launch_bounds__(256) void k_split_sum(const float* __restrict__ par
     float s = 0.0f;
     for (int z = 0; z < splits; ++z) s += part[(size_t)z * n + i];
     out[i] = s;
+}
+
+// The same with the GEMM epilogue: + bias[col], ReLU, stored as fp32 or bf16.
+__global__ __launch_bounds__(256) void k_split_sum_epi(const float* __restrict__ part, int splits, int M, int N,
+                                                       int ldc, const float* __restrict__ bias, int relu,
+                                                       float* __restrict__ Cf, bf16* __restrict__ Cb)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= M * N) return;
+    const int m = i / N, n = i - m * N;
+    const size_t o = (size_t)m * ldc + n;
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z) s += part[(size_t)z * M * ldc + o];
+    if (bias) s += bias[n];
+    if (relu) s = fmaxf(s, 0.0f);
+    if (Cb) Cb[o] = f2bf(s);
+    else Cf[o] = s;
 }
 
 // ============================================================= conv helpers
@@ -470,8 +483,13 @@ int gemm(cnn_net_s* c, bool ta, const bf16* A, int lda, bool tb, const bf16* B, 
         hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, out, outb, ldc, M, N, K,
                            kchunk, bb, rl);
     if (splits > 1) {
-        const size_t n = (size_t)M * ldc;
-        hipLaunchKernelGGL(k_split_sum, g1(n), dim3(256), 0, c->stream, c->split, splits, n, Cf);
+        if (!bias && !relu && Cf) {
+            const size_t n = (size_t)M * ldc;
+            hipLaunchKernelGGL(k_split_sum, g1(n), dim3(256), 0, c->stream, c->split, splits, n, Cf);
+        } else {
+            hipLaunchKernelGGL(k_split_sum_epi, g1((size_t)M * N), dim3(256), 0, c->stream, c->split, splits, M, N,
+                               ldc, bias, relu ? 1 : 0, Cf, Cb);
+        }
     }
     return launch_check("gemm");
 }
@@ -485,6 +503,16 @@ int colsum(cnn_net_s* c, const float* G, int M, int N, float* out)
                        c->colpart);
     hipLaunchKernelGGL(k_split_sum, g1(N), dim3(256), 0, c->stream, c->colpart, chunks, (size_t)N, out);
     return launch_check("colsum");
+}
+
+// Forward GEMMs with few output tiles (small batches) split the reduction so
+// more than a handful of workgroups share it: at most K / 128 chunks, aiming
+// at 64 workgroups.
+int fwd_splits(int M, int N, int K)
+{
+    const int tiles = cdiv(M, BM) * cdiv(N, BN);
+    if (tiles >= 64) return 1;
+    return std::max(1, std::min(64 / tiles, K / 128));
 }
 
 int splits_for(int K, int MN)
@@ -670,17 +698,17 @@ int cnn_forward(cnn_net c, const void* x, int N, float* out)
         hipLaunchKernelGGL(k_im2col, g1(tot), dim3(256), 0, c->stream, in, N, s.hi, s.wi, s.cin, s.ho, s.wo, c->Kp[l],
                            c->xcol[l]);
         if ((rc = gemm(c, false, c->xcol[l], c->Kp[l], false, c->pb + c->convW[l].off, s.cout, nullptr, c->y[l], s.cout,
-                       M, s.cout, c->Kp[l], c->p + c->convB[l].off, true, 1)))
+                       M, s.cout, c->Kp[l], c->p + c->convB[l].off, true, fwd_splits(M, s.cout, c->Kp[l]))))
             return rc;
         in = c->y[l];
     }
     hipLaunchKernelGGL(k_gap_fwd, g1((size_t)N * kFc1In), dim3(256), 0, c->stream, c->y[kConvs - 1], N, 2, kFc1In,
                        c->gap);
     if ((rc = gemm(c, false, c->gap, kFc1In, false, c->pb + c->fc1W.off, kFc1Out, nullptr, c->h1, kFc1Out, N, kFc1Out,
-                   kFc1In, c->p + c->fc1B.off, true, 1)))
+                   kFc1In, c->p + c->fc1B.off, true, fwd_splits(N, kFc1Out, kFc1In))))
         return rc;
     if ((rc = gemm(c, false, c->h1, kFc1Out, false, c->pb + c->fc2W.off, kOutPad, c->out, nullptr, kOutPad, N, kOutPad,
-                   kFc1Out, c->p + c->fc2B.off, false, 1)))
+                   kFc1Out, c->p + c->fc2B.off, false, fwd_splits(N, kOutPad, kFc1Out))))
         return rc;
     if (out) CK(hipMemcpy2DAsync(out, kOut * 4, c->out, kOutPad * 4, kOut * 4, N, hipMemcpyDeviceToDevice, c->stream));
     return launch_check("forward");

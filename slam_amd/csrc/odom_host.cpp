@@ -760,7 +760,7 @@ int odom_track_seeded(odom_handle c, const float* pred, float t_scale, float r_s
         if (iters[l] < 0 || iters[l] > 63) return ODOM_ERR_ARG;
         P.iters[l] = iters[l];
     }
-    P.seeded = 1;
+    P.seeded = select ? 2 : 1;
     if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     if (select && !(c->icp_cur && c->icp_model)) return ODOM_ERR_STATE;
     launch_seed_select(c->d_pair[c->parity], 1, pred, t_scale, r_scale, select, P, c->stream);

@@ -1680,19 +1680,22 @@ __device__ void pose_from_result(const double* Rt, const float* Rprev, const flo
 }
 
 // Warm start.  The predicted camera motion (pred: scaled translation and
-// rotation vector, 6 floats per pair) becomes the seed resultRt = M^-1.  With
-// `select`, the prediction is scored against the plain start (identity
-// increment) by the ICP association at level 2 -- inliers and mean squared
-// point-to-plane residual -- and kept only when it is not worse on either
-// (1 % slack), else the call starts plain.  One workgroup per pair.
-__global__ __launch_bounds__(256) void k_seed_select(const PairDesc* __restrict__ pairs,
-                                                     const float* __restrict__ pred, float t_scale, float r_scale,
-                                                     int select, TrackParams P)
+// rotation vector, 6 floats per pair) becomes the candidate seed
+// resultRt = M^-1, stored by block 0 of the pair.  With `select`, both starts
+// -- the prediction and the plain one (identity increment) -- are scored by
+// the ICP association at level 2 (inlier count and sum of squared
+// point-to-plane residuals), each of kSeedBlocks workgroups per pair storing
+// its 4 sums as a block partial; k_track_begin decides.
+constexpr int kSeedBlocks = 32;
+
+__global__ __launch_bounds__(256) void k_seed_eval(const PairDesc* __restrict__ pairs, const float* __restrict__ pred,
+                                                   float t_scale, float r_scale, int select, TrackParams P)
 {
     __shared__ double sSeed[16];
     __shared__ float sRc[2][9], sTc[2][3], sRpi[9], sTp[3];
     __shared__ float sRed[4][4];
-    const PairDesc& D = pairs[blockIdx.x];
+    const int p = blockIdx.x / kSeedBlocks, lb = blockIdx.x - p * kSeedBlocks;
+    const PairDesc& D = pairs[p];
     TrackState* st = D.st;
     if (threadIdx.x == 0) {
         float Rprev[9], tprev[3];
@@ -1702,7 +1705,7 @@ __global__ __launch_bounds__(256) void k_seed_select(const PairDesc* __restrict_
         }
         inv3f(Rprev, sRpi);
         for (int r = 0; r < 3; ++r) sTp[r] = tprev[r];
-        const float* q = pred + 6 * blockIdx.x;
+        const float* q = pred + 6 * p;
         double w[3], R[9], t[3];
         for (int r = 0; r < 3; ++r) {
             t[r] = (double)q[r] * (double)t_scale;
@@ -1719,56 +1722,62 @@ __global__ __launch_bounds__(256) void k_seed_select(const PairDesc* __restrict_
         for (int a = 0; a < 9; ++a) sRc[0][a] = Rprev[a];
         for (int a = 0; a < 3; ++a) sTc[0][a] = tprev[a];
         pose_from_result(sSeed, Rprev, tprev, sRc[1], sTc[1]);
+        if (lb == 0)
+            for (int a = 0; a < 16; ++a) st->seedRt[a] = sSeed[a];
     }
     __syncthreads();
-    int keep = 1;
-    if (select) {
-        const LevelDesc& L = D.lv[2];
-        const int N = L.w * L.h;
-        const size_t P3 = (size_t)N;
-        float cnt[2] = {0.f, 0.f}, r2[2] = {0.f, 0.f};
-        for (int cand = 0; cand < 2; ++cand) {
-            IcpC ic;
-            for (int a = 0; a < 9; ++a) { ic.Rc[a] = sRc[cand][a]; ic.Rpi[a] = sRpi[a]; }
-            for (int a = 0; a < 3; ++a) { ic.tc[a] = sTc[cand][a]; ic.tp[a] = sTp[a]; }
-            ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
-            ic.dist = P.distThres; ic.angle = P.angleThres;
-            for (int i = threadIdx.x; i < N; i += 256) {
-                Px6 src, dst;
-                float vg[3], vcp[3], row[7];
-                icp_load(L, i, P3, src);
-                const int j = icp_project(L, ic, src, vg, vcp);
-                if (j < 0) continue;
-                icp_gather(L, j, P3, dst);
-                if (icp_row(ic, src, vg, vcp, dst, row)) {
-                    cnt[cand] += 1.0f;
-                    r2[cand] += row[6] * row[6];
-                }
+    if (!select) return;
+    const LevelDesc& L = D.lv[2];
+    const int N = L.w * L.h;
+    const size_t P3 = (size_t)N;
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int cand = 0; cand < 2; ++cand) {
+        IcpC ic;
+        for (int a = 0; a < 9; ++a) { ic.Rc[a] = sRc[cand][a]; ic.Rpi[a] = sRpi[a]; }
+        for (int a = 0; a < 3; ++a) { ic.tc[a] = sTc[cand][a]; ic.tp[a] = sTp[a]; }
+        ic.fx = L.fx; ic.fy = L.fy; ic.cx = L.cx; ic.cy = L.cy;
+        ic.dist = P.distThres; ic.angle = P.angleThres;
+        for (int i = lb * 256 + threadIdx.x; i < N; i += kSeedBlocks * 256) {
+            Px6 src, dst;
+            float vg[3], vcp[3], row[7];
+            icp_load(L, i, P3, src);
+            const int j = icp_project(L, ic, src, vg, vcp);
+            if (j < 0) continue;
+            icp_gather(L, j, P3, dst);
+            if (icp_row(ic, src, vg, vcp, dst, row)) {
+                v[2 * cand] += 1.0f;
+                v[2 * cand + 1] += row[6] * row[6];
             }
         }
-        float v[4] = {cnt[0], r2[0], cnt[1], r2[1]};
-        for (int q = 0; q < 4; ++q) v[q] = wave_total_f(v[q]);
-        if ((threadIdx.x & 63) == 63)
-            for (int q = 0; q < 4; ++q) sRed[threadIdx.x >> 6][q] = v[q];
-        __syncthreads();
-        float tot[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int wv = 0; wv < 4; ++wv)
-            for (int q = 0; q < 4; ++q) tot[q] += sRed[wv][q];
-        const float m0 = tot[1] / fmaxf(tot[0], 1.0f), m1 = tot[3] / fmaxf(tot[2], 1.0f);
-        keep = (tot[2] >= 0.99f * tot[0]) && (m1 <= 1.01f * m0);
     }
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 16; ++a) st->seedRt[a] = keep ? sSeed[a] : ((a % 5 == 0) ? 1.0 : 0.0);
-        st->seed_used = keep;
-        st->seed_calls++;
-        st->seed_kept += keep;
+    for (int q = 0; q < 4; ++q) v[q] = wave_total_f(v[q]);
+    if ((threadIdx.x & 63) == 63)
+        for (int q = 0; q < 4; ++q) sRed[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float t = 0.0f;
+        for (int wv = 0; wv < 4; ++wv) t += sRed[wv][threadIdx.x];
+        D.part_b[0][(size_t)lb * kPartStride + threadIdx.x] = t;
     }
 }
 
 void launch_seed_select(const PairDesc* pairs, int npairs, const float* pred, float t_scale, float r_scale,
                         int select, const TrackParams& P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_seed_select, dim3(npairs), dim3(256), 0, s, pairs, pred, t_scale, r_scale, select, P);
+    hipLaunchKernelGGL(k_seed_eval, dim3(npairs * kSeedBlocks), dim3(256), 0, s, pairs, pred, t_scale, r_scale,
+                       select, P);
+}
+
+// The keep-or-fall-back rule: the prediction is kept when its level-2
+// association has at least 99 % of the plain start's inliers and a mean
+// squared residual at most 1 % above it.
+__device__ int seed_keep(const PairDesc& D)
+{
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kSeedBlocks; ++b)
+        for (int q = 0; q < 4; ++q) t[q] += (double)D.part_b[0][(size_t)b * kPartStride + q];
+    const double m0 = t[1] / fmax(t[0], 1.0), m1 = t[3] / fmax(t[2], 1.0);
+    return (t[2] >= 0.99 * t[0]) && (m1 <= 1.01 * m0);
 }
 
 __global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
@@ -1789,10 +1798,15 @@ __global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, Tr
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) S.resultRt[a * 4 + b] = st->so3R[a * 3 + b];
     if (P.seeded) {
-        // the warm start: resultRt = seed, and (unlike the SO3 seed of the
-        // reference) the current estimate prev * seed^-1 from the first pass
-        for (int a = 0; a < 16; ++a) S.resultRt[a] = st->seedRt[a];
+        // the warm start (seeded 2: only if the level-2 scores keep it):
+        // resultRt = seed, and (unlike the SO3 seed of the reference) the
+        // current estimate prev * seed^-1 from the first pass
+        const int keep = (P.seeded == 2) ? seed_keep(pairs[p]) : 1;
+        for (int a = 0; a < 16; ++a) S.resultRt[a] = keep ? st->seedRt[a] : ((a % 5 == 0) ? 1.0 : 0.0);
         pose_from_result(S.resultRt, st->Rprev, st->tprev, S.Rcurr, S.tcurr);
+        st->seed_used = keep;
+        st->seed_calls++;
+        st->seed_kept += keep;
     }
     for (int a = 0; a < 36; ++a) S.lastA[a] = st->lastA[a];
     for (int a = 0; a < 6; ++a) S.lastb[a] = st->lastb[a];

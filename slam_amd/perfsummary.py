@@ -1,0 +1,154 @@
+"""Summaries of rocprofv3 output for the committed profiles.
+
+    python -m slam_amd.perfsummary stats  DIR [--top N] [--cmd "..."]
+        markdown table of the kernel statistics (DIR/run_kernel_stats.csv)
+    python -m slam_amd.perfsummary frame  DIR
+        kernels of one mid-run tracking call (DIR/run_kernel_trace.csv)
+    python -m slam_amd.perfsummary traffic FETCH_DIR WRITE_DIR [--out JSON]
+        HBM bytes per launch of the ICP/association pass (k_step_a) per
+        pyramid level, from FETCH_SIZE and WRITE_SIZE counter runs (KiB per
+        dispatch), each scaled by what the counter reads for the 256 MiB
+        device copies of ``bench.py --calibrate-copy``.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import os
+import re
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 256 << 20
+LEVEL_PX = (640 * 480, 320 * 240, 160 * 120)
+MAX_BLK = 256
+
+
+def short(name: str) -> str:
+    n = name.split("(")[0].replace("void ", "").replace("slam::", "").strip()
+    return n
+
+
+def cmd_stats(d: str, top: int, cmd: str | None) -> str:
+    rows = list(csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))))
+    out = []
+    if cmd:
+        out.append(f"Command: `{cmd}`\n")
+    out.append("| kernel | calls | total ms | avg us | min us | max us | % |")
+    out.append("|---|---:|---:|---:|---:|---:|---:|")
+    for r in rows[:top]:
+        out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                   f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                   f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def cmd_frame(d: str, mode: str = "plain") -> str:
+    tr = list(csv.DictReader(open(os.path.join(d, "run_kernel_trace.csv"))))
+    tr.sort(key=lambda r: int(r["Start_Timestamp"]))
+    ends = [i for i, r in enumerate(tr) if "k_track_end" in r["Kernel_Name"]]
+    frames = []
+    for a, b in zip(ends[:-1], ends[1:]):
+        names = " ".join(r["Kernel_Name"] for r in tr[a + 1:b + 1])
+        if "k_batch" in names or b - a > 400:
+            continue
+        seeded = "k_seed" in names
+        so3 = "k_so3_iter" in names
+        if mode == "any" or (mode == "plain" and so3 and not seeded) or (mode == "learned" and seeded):
+            frames.append((a, b))
+    i0, i1 = frames[len(frames) // 2]
+    t0 = int(tr[i0]["End_Timestamp"])
+    agg = defaultdict(lambda: [0, 0.0])
+    for r in tr[i0 + 1:i1 + 1]:
+        a = agg[short(r["Kernel_Name"])]
+        a[0] += 1
+        a[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    total = (int(tr[i1]["End_Timestamp"]) - t0) / 1e3
+    out = ["| kernel | launches | us per frame | avg us |", "|---|---:|---:|---:|"]
+    for k, (n, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        out.append(f"| `{k}` | {n} | {us:.1f} | {us / n:.2f} |")
+    out.append(f"\nOne tracking call (frame) end to end: {total:.1f} us, {i1 - i0} launches.")
+    return "\n".join(out)
+
+
+def _level(name: str, grid: int, wg: int) -> int | None:
+    m = re.search(r"k_step_a<(\d+), (\d+)", name)
+    if not m:
+        return None
+    threads, ppt = int(m.group(1)), int(m.group(2))
+    blocks = grid // wg
+    for lv, n in enumerate(LEVEL_PX):
+        if min(MAX_BLK, -(-n // (threads * ppt))) == blocks:
+            return lv
+    return None
+
+
+def _per_level(path: str):
+    rows = list(csv.DictReader(open(path)))
+    copy = [float(r["Counter_Value"]) for r in rows
+            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    lv = defaultdict(list)
+    for r in rows:
+        if "k_step_a" not in r["Kernel_Name"] or "true>" not in r["Kernel_Name"]:
+            continue  # the fused pass of tracking calls only
+        level = _level(r["Kernel_Name"], int(r["Grid_Size"]), int(r["Workgroup_Size"]))
+        if level is not None:
+            lv[level].append(float(r["Counter_Value"]))
+    return copy, {k: sum(v) / len(v) for k, v in lv.items()}, {k: len(v) for k, v in lv.items()}
+
+
+def cmd_traffic(fetch_dir: str, write_dir: str) -> dict:
+    fc, fl, fn = _per_level(os.path.join(fetch_dir, "run_counter_collection.csv"))
+    wc, wl, wn = _per_level(os.path.join(write_dir, "run_counter_collection.csv"))
+    f_scale = COPY_BYTES / (1024.0 * (sum(fc) / len(fc)))
+    w_scale = COPY_BYTES / (1024.0 * (sum(wc) / len(wc)))
+    levels = {}
+    for l in sorted(set(fl) & set(wl)):
+        fb = fl[l] * 1024.0 * f_scale
+        wb = wl[l] * 1024.0 * w_scale
+        levels[str(l)] = {"fetch_kib": round(fl[l], 1), "write_kib": round(wl[l], 1), "dispatches": fn[l],
+                          "fetch_bytes": int(fb), "write_bytes": int(wb), "traffic_bytes": int(fb + wb),
+                          "algorithmic_bytes_48B": 48 * LEVEL_PX[l]}
+    iters = (10, 5, 4)
+    per_launch = sum(iters[int(l)] * v["traffic_bytes"] for l, v in levels.items()) / sum(
+        iters[int(l)] for l in levels)
+    return {"kernel": "k_step_a (fused pass of tracking calls)", "counter_unit": "KiB per dispatch",
+            "calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": round(sum(fc) / len(fc), 1),
+                            "write_kib": round(sum(wc) / len(wc), 1), "fetch_scale": round(f_scale, 4),
+                            "write_scale": round(w_scale, 4)},
+            "levels": levels, "traffic_bytes_per_launch_10_5_4": int(per_launch)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--top", type=int, default=20)
+    s.add_argument("--command", default=None, help="the command that made the profile")
+    f = sub.add_parser("frame")
+    f.add_argument("dir")
+    f.add_argument("--mode", choices=("plain", "learned", "any"), default="plain",
+                   help="plain: calls with the SO3 pass; learned: seeded calls")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--out", default=None)
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        print(cmd_stats(a.dir, a.top, a.command))
+    elif a.cmd == "frame":
+        print(cmd_frame(a.dir, a.mode))
+    else:
+        res = cmd_traffic(a.fetch_dir, a.write_dir)
+        text = json.dumps(res, indent=1)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(text + "\n")
+        print(text)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
