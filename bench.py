@@ -305,7 +305,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-frames", type=int, default=20, help="frames of the CPU baseline / FP64 oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-pair measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
@@ -420,13 +420,17 @@ def main() -> int:
     drift_t = float(np.mean([d[0] for d in drift]))
     drift_r = float(np.mean([d[1] for d in drift]))
 
-    # roofline of the step kernels: HIP events around every step launch on
-    # the library's stream, over tracking calls of the last frame's inputs
+    # roofline of the step kernels: per level, the last iteration of a
+    # completed call relaunched back to back between one HIP event pair on the
+    # library's stream (odom_profile_steps); the call itself with an event
+    # pair around every launch gives the end-to-end call time
     iters = [10, 5, 4]
     npx = [(cam.width >> l) * (cam.height >> l) for l in range(3)]
-    profs = [odo.profile_track(False, ICP_WEIGHT, True, False, True) for _ in range(args.profile_reps)]
-    us = {0: [float(np.mean([p["us_a"][l] for p in profs])) for l in range(3)],
-          1: [float(np.mean([p["us_b"][l] for p in profs])) for l in range(3)]}
+    odo.track_async(False, ICP_WEIGHT, True, False, True)
+    odo.synchronize()
+    steps = odo.profile_steps(False, ICP_WEIGHT, reps=args.profile_reps)
+    us = {0: steps["us_a"], 1: steps["us_b"]}
+    profs = [odo.profile_track(False, ICP_WEIGHT, True, False, True) for _ in range(3)]
     track_us = float(np.mean([p["us_total"] for p in profs]))
     tot_t = {k: sum(iters[l] * us[k][l] for l in range(3)) for k in (0, 1)}
     dom = 0 if tot_t[0] >= tot_t[1] else 1
@@ -437,13 +441,18 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     joint_bytes = sum(iters[l] * (BYTES_ICP + BYTES_RGB) * npx[l] for l in range(3))
     joint_gbs = joint_bytes / ((tot_t[0] + tot_t[1]) * 1e-6) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_step_a_traffic_640x480.json")
+    if dom == 0 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = int(json.load(fh)["traffic_bytes_per_launch_10_5_4"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_step_a (ICP products + photometric association)" if dom == 0 else
                 "k_step_b (photometric products)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_us, 2),
                 "per_level_us": {"k_step_a": [round(x, 2) for x in us[0]], "k_step_b": [round(x, 2) for x in us[1]]},
-                "joint_iteration_gbs": round(joint_gbs, 1), "track_call_us": round(track_us, 1)}
+                "joint_iteration_gbs": round(joint_gbs, 1), "track_call_us_events_per_launch": round(track_us, 1)}
     log(f"roofline {roofline}")
 
     cpu = None

@@ -174,6 +174,11 @@ int odom_get_step_shape(odom_handle h, int kernel, int level, int *threads, int 
  * per level, and the whole call's duration. */
 int odom_profile_track(odom_handle h, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3,
                        float us_a[3], float us_b[3], int n_a[3], int n_b[3], float *us_total);
+/* Mean microseconds of the two step kernels per level as they run inside a
+ * tracking call (ICP/association pass us_a, photometric pass us_b): after a
+ * completed call with the same options, each level's last iteration is
+ * relaunched reps times back to back between one event pair. */
+int odom_profile_steps(odom_handle h, int rgb_only, float icp_weight, int reps, float us_a[3], float us_b[3]);
 /* Times the supported shapes on the handle's current inputs and keeps the
  * fastest per kernel and level. */
 int odom_autotune(odom_handle h, int rgb_only, float icp_weight);

@@ -1025,6 +1025,51 @@ int odom_profile_track(odom_handle c, int rgbOnly, float icpWeight, int pyramid,
     return rc;
 }
 
+// Mean duration of the step kernels as they run inside tracking calls, per
+// level: after a completed call with the same options, the call's last
+// iteration at each level is launched `reps` times back to back between one
+// event pair.  A relaunch of iteration k reads state slot (k-1) & 1 and the
+// partials of parity (k-1) & 1 and writes the other parity, so every relaunch
+// repeats the same work.
+int odom_profile_steps(odom_handle c, int rgbOnly, float icpWeight, int reps, float us_a[3], float us_b[3])
+{
+    if (!c || !us_a || !us_b || reps < 1 || reps > 10000) return ODOM_ERR_ARG;
+    const TrackParams P = make_params(c, rgbOnly, icpWeight, 1, 0, 1);
+    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
+    if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
+    std::vector<int> sched;
+    for (int l = kLevels - 1; l >= 0; --l)
+        for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int rc = ODOM_OK;
+    for (int level = 0; level < kLevels && rc == ODOM_OK; ++level) {
+        int k = -1;
+        for (int q = 0; q < (int)sched.size(); ++q)
+            if (sched[q] / 64 == level) k = q;
+        us_a[level] = us_b[level] = 0.0f;
+        if (k <= 0) continue;
+        const int iter = sched[k] % 64, prev = sched[k - 1] / 64;
+        for (int pass = 0; pass < (P.rgb ? 2 : 1); ++pass) {
+            for (int r = 0; r < reps + 2; ++r) {
+                if (r == 2) HIPCHK(c, hipEventRecord(e0, c->stream));
+                if (pass == 0) launch_step_a(step_launch(c, 0, level), level, iter, k, prev, P, 0, nullptr, c->stream);
+                else launch_step_b(step_launch(c, 1, level), level, iter, k, P, 0, c->stream);
+            }
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            (pass == 0 ? us_a : us_b)[level] = ms * 1000.0f / reps;
+        }
+        rc = check_launch(c);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
 {
     if (!c) return ODOM_ERR_ARG;

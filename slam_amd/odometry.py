@@ -62,7 +62,7 @@ EXPORTS = [
     "odom_batch_create", "odom_batch_destroy", "odom_track_batch", "odom_batch_synchronize",
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
-    "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph",
+    "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
 ]
 
 _V = ctypes.c_void_p
@@ -118,6 +118,7 @@ def lib() -> ctypes.CDLL:
     L.odom_batch_set_split_solve.argtypes = [_V, ctypes.c_int]
     L.odom_get_stream.argtypes = [_V, _P(_V)]
     L.odom_set_graph.argtypes = [_V, ctypes.c_int]
+    L.odom_profile_steps.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, _V, _V]
     L.odom_init_warmstart_input.argtypes = [_V, _V]
     L.odom_track_seeded.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _V, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_int]
@@ -419,6 +420,14 @@ class RGBDOdometry:
                                                               _p(nb), ctypes.byref(tot)))
         return {"us_a": ua.tolist(), "us_b": ub.tolist(), "n_a": na.tolist(), "n_b": nb.tolist(),
                 "us_total": tot.value}
+
+    def profile_steps(self, rgbOnly=False, icpWeight=10.0, reps=50) -> dict:
+        """Mean microseconds per launch of the two step kernels per level, as
+        they run inside a tracking call (needs a completed call)."""
+        ua, ub = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        _check("odom_profile_steps", lib().odom_profile_steps(self._h, int(rgbOnly), float(icpWeight), int(reps),
+                                                              _p(ua), _p(ub)))
+        return {"us_a": ua.tolist(), "us_b": ub.tolist()}
 
     def autotune(self, rgbOnly=False, icpWeight=10.0) -> None:
         _check("odom_autotune", lib().odom_autotune(self._h, int(rgbOnly), float(icpWeight)))
