@@ -22,7 +22,6 @@ int step_shape_supported(StepShape sh);
 // ---- preprocessing (a1-a13) ----
 void launch_bilateral(const uint16_t* src, int src_pitch, uint16_t* dst, int w, int h, float maxD,
                       hipStream_t s);
-void launch_pyrdown_u16(const uint16_t* src, int w, int h, uint16_t* dst, hipStream_t s);
 // depth pyramid -> vertex+normal maps of all levels in one launch
 struct VNLevels {
     const uint16_t* depth[kLevels];
@@ -48,11 +47,6 @@ struct ModelLevels {
     const float* pose_dev;          // if set: [R|t] (3x4 row-major) read on the device
 };
 void launch_model_maps(const ModelLevels& a, hipStream_t s);
-void launch_vertices_to_depth(const float* vz, int w, int h, float cutoff, float* depth, hipStream_t s);
-void launch_pyrdown_f32(const float* src, int w, int h, float* dst, hipStream_t s);
-void launch_intensity(const uint8_t* rgb, int pitch_bytes, int channels, int w, int h, uint8_t* dst,
-                      hipStream_t s);
-void launch_pyrdown_u8(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
 // Levels 1 and 2 of up to two pyramids in one launch.  kind: 0 = float
 // Gaussian (a9), 1 = uint8 Gaussian (a11), 2 = uint16 bilateral-gated (a2).
 struct Pyr12 {
@@ -114,8 +108,6 @@ void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, i
 void launch_reduce_dbg(const PairDesc* pairs, int which, int level, hipStream_t s);
 void launch_so3_step_dbg(const PairDesc* pairs, const float* consts27, double* out11, uint8_t* mask,
                          hipStream_t s);
-void launch_copy_u8(const uint8_t* src, uint8_t* dst, size_t n, hipStream_t s);
-void launch_swap_u8(uint8_t* a, uint8_t* b, size_t n, hipStream_t s);
 
 // Constant tables of the kernels (the bilateral's spatial factors), per device.
 void init_device_tables();

@@ -120,36 +120,6 @@ void launch_bilateral(const uint16_t* src, int pitch, uint16_t* dst, int w, int 
     hipLaunchKernelGGL(k_bilateral, grid, dim3(256), 0, s, src, pitch, dst, w, h, maxv);
 }
 
-// ================================================================ a2 pyrDown
-// utils.cu:57-94, sigma_color 30
-__global__ __launch_bounds__(256) void k_pyrdown_u16(const uint16_t* __restrict__ src, int sw, int sh,
-                                                     uint16_t* __restrict__ dst, int dw, int dh)
-{
-    const int x = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int y = blockIdx.y * 8 + (threadIdx.x >> 5);
-    if (x >= dw || y >= dh) return;
-    const float wts[3] = {0.375f, 0.25f, 0.0625f};
-    const int center = src[(size_t)(2 * y) * sw + 2 * x];
-    const int x_mi = max(0, 2 * x - 2) - 2 * x, y_mi = max(0, 2 * y - 2) - 2 * y;
-    const int x_ma = min(sw, 2 * x + 3) - 2 * x, y_ma = min(sh, 2 * y + 3) - 2 * y;
-    float sum = 0.0f, wall = 0.0f;
-    for (int yi = y_mi; yi < y_ma; ++yi)
-        for (int xi = x_mi; xi < x_ma; ++xi) {
-            const int val = src[(size_t)(2 * y + yi) * sw + 2 * x + xi];
-            if ((float)abs(val - center) < 90.0f) {
-                sum = sum + (float)val * wts[abs(xi)] * wts[abs(yi)];
-                wall = wall + wts[abs(xi)] * wts[abs(yi)];
-            }
-        }
-    dst[(size_t)y * dw + x] = (uint16_t)(int)(sum / wall);
-}
-
-void launch_pyrdown_u16(const uint16_t* src, int w, int h, uint16_t* dst, hipStream_t s)
-{
-    const int dw = w / 2, dh = h / 2;
-    hipLaunchKernelGGL(k_pyrdown_u16, dim3(cdiv(dw, 32), cdiv(dh, 8)), dim3(256), 0, s, src, w, h, dst, dw, dh);
-}
-
 // Level segments of a one-dimensional grid: blocks [b0[l], b0[l+1]) cover level l.
 struct Seg3 {
     int b0[kLevels + 1];
@@ -335,23 +305,6 @@ void launch_model_maps(const ModelLevels& a, hipStream_t s)
     hipLaunchKernelGGL(k_model_maps, dim3(g.b0[kLevels]), dim3(256), 0, s, a, g);
 }
 
-// ================================================== a8 verticesToDepth
-// utils.cu:526-537 (input: the z plane of the last vertex map)
-__global__ __launch_bounds__(256) void k_vertices_to_depth(const float* __restrict__ vz, int n, float cutoff,
-                                                           float* __restrict__ depth)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float z = vz[i];
-    depth[i] = (z > cutoff || z <= 0) ? QNAN : z;
-}
-
-void launch_vertices_to_depth(const float* vz, int w, int h, float cutoff, float* depth, hipStream_t s)
-{
-    const int n = w * h;
-    hipLaunchKernelGGL(k_vertices_to_depth, dim3(cdiv(n, 256)), dim3(256), 0, s, vz, n, cutoff, depth);
-}
-
 // ============================================== a9 + a11 Gaussian pyrDown
 // utils.cu:332-363 and 470-500: loop bound min(2x+3, cols-1) and the
 // reversed weight index (ty-cy-1)*5+(tx-cx-1) are kept.
@@ -362,93 +315,41 @@ __device__ __forceinline__ float gauss5(int k)
     return k == 2 ? 6.0f : ((k & 1) ? 4.0f : 1.0f);
 }
 
-__global__ __launch_bounds__(256) void k_pyrdown_f32(const float* __restrict__ src, int sw, int sh,
-                                                     float* __restrict__ dst, int dw, int dh)
-{
-    const int x = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int y = blockIdx.y * 8 + (threadIdx.x >> 5);
-    if (x >= dw || y >= dh) return;
-    const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
-    float sum = 0.0f;
-    int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const float v = src[(size_t)cy * sw + cx];
-            if (!isnan(v)) {
-                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + v * gk;
-                count = (int)((float)count + gk);
-            }
-        }
-    dst[(size_t)y * dw + x] = sum / (float)count;
-}
-
-void launch_pyrdown_f32(const float* src, int w, int h, float* dst, hipStream_t s)
-{
-    const int dw = w / 2, dh = h / 2;
-    hipLaunchKernelGGL(k_pyrdown_f32, dim3(cdiv(dw, 32), cdiv(dh, 8)), dim3(256), 0, s, src, w, h, dst, dw, dh);
-}
-
-__global__ __launch_bounds__(256) void k_pyrdown_u8(const uint8_t* __restrict__ src, int sw, int sh,
-                                                    uint8_t* __restrict__ dst, int dw, int dh)
-{
-    const int x = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int y = blockIdx.y * 8 + (threadIdx.x >> 5);
-    if (x >= dw || y >= dh) return;
-    const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
-    float sum = 0.0f;
-    int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const uint8_t v = src[(size_t)cy * sw + cx];
-            if (v > 0) {
-                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + (float)v * gk;
-                count = (int)((float)count + gk);
-            }
-        }
-    const float q = sum / (float)count;
-    dst[(size_t)y * dw + x] = (count == 0) ? (uint8_t)0 : (uint8_t)q;
-}
-
-void launch_pyrdown_u8(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s)
-{
-    const int dw = w / 2, dh = h / 2;
-    hipLaunchKernelGGL(k_pyrdown_u8, dim3(cdiv(dw, 32), cdiv(dh, 8)), dim3(256), 0, s, src, w, h, dst, dw, dh);
-}
-
-// ====================================================== a10 intensity
-// utils.cu:550-563: 0.114/0.299/0.587 on bytes 0/1/2, truncated
-__global__ __launch_bounds__(256) void k_intensity(const uint8_t* __restrict__ rgb, int pitch, int ch, int w,
-                                                   int h, uint8_t* __restrict__ dst)
-{
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= w || y >= h) return;
-    const uint8_t* p = rgb + (size_t)y * pitch + (size_t)x * ch;
-    const int value = (int)((float)p[0] * 0.114f + (float)p[1] * 0.299f + (float)p[2] * 0.587f);
-    dst[(size_t)y * w + x] = (uint8_t)value;
-}
-
-void launch_intensity(const uint8_t* rgb, int pitch, int ch, int w, int h, uint8_t* dst, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_intensity, dim3(cdiv(w, 64), cdiv(h, 4)), dim3(256), 0, s, rgb, pitch, ch, w, h, dst);
-}
-
 // ===================================================== pyramid launches
-// One pyramid level of up to two pyramids per launch.
+// One pyramid level of up to two pyramids per launch.  Each output reads its
+// 5x5 window as 25 independent loads (coordinates clamped into the image, so
+// every address is valid) that are all in flight together; taps outside the
+// window of the reference's loops are then skipped in the accumulation, which
+// visits the rest in the loops' row-major order, so every sum is the value
+// the loops produce.
+template <class T, class Get>
+__device__ __forceinline__ void window5(Get get, int sw, int sh, int x, int y, T (&v)[25])
+{
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const int cy = min(max(2 * y - 2 + dy, 0), sh - 1);
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) v[dy * 5 + dx] = get(min(max(2 * x - 2 + dx, 0), sw - 1), cy);
+    }
+}
+
 template <class Get>
 __device__ __forceinline__ float gauss_f32_at(Get get, int sw, int sh, int x, int y)
 {
     const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
+    float v[25];
+    window5(get, sw, sh, x, y, v);
     float sum = 0.0f;
     int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const float v = get(cx, cy);
-            if (!isnan(v)) {
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
+            const float val = v[dy * 5 + dx];
+            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx && !isnan(val)) {
                 const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + v * gk;
+                sum = sum + val * gk;
                 count = (int)((float)count + gk);
             }
         }
@@ -459,14 +360,19 @@ template <class Get>
 __device__ __forceinline__ uint8_t gauss_u8_at(Get get, int sw, int sh, int x, int y)
 {
     const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
+    uint8_t v[25];
+    window5(get, sw, sh, x, y, v);
     float sum = 0.0f;
     int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy)
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const uint8_t v = get(cx, cy);
-            if (v > 0) {
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
+            const uint8_t val = v[dy * 5 + dx];
+            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx && val > 0) {
                 const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + (float)v * gk;
+                sum = sum + (float)val * gk;
                 count = (int)((float)count + gk);
             }
         }
@@ -474,21 +380,26 @@ __device__ __forceinline__ uint8_t gauss_u8_at(Get get, int sw, int sh, int x, i
     return (count == 0) ? (uint8_t)0 : (uint8_t)q;
 }
 
-// utils.cu:57-94 (sigma_color 30)
+// utils.cu:57-94 (sigma_color 30): taps at offsets -2..2 inside the image,
+// weighted {0.0625, 0.25, 0.375, 0.25, 0.0625} per axis, kept when within
+// 3 sigma of the centre value.
 template <class Get>
 __device__ __forceinline__ uint16_t pyr_u16_at(Get get, int sw, int sh, int x, int y)
 {
-    const float wts[3] = {0.375f, 0.25f, 0.0625f};
-    const int center = get(2 * x, 2 * y);
-    const int x_mi = max(0, 2 * x - 2) - 2 * x, y_mi = max(0, 2 * y - 2) - 2 * y;
-    const int x_ma = min(sw, 2 * x + 3) - 2 * x, y_ma = min(sh, 2 * y + 3) - 2 * y;
+    const float wts[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    int v[25];
+    window5(get, sw, sh, x, y, v);
+    const int center = v[12];
     float sum = 0.0f, wall = 0.0f;
-    for (int yi = y_mi; yi < y_ma; ++yi)
-        for (int xi = x_mi; xi < x_ma; ++xi) {
-            const int val = get(2 * x + xi, 2 * y + yi);
-            if ((float)abs(val - center) < 90.0f) {
-                sum = sum + (float)val * wts[abs(xi)] * wts[abs(yi)];
-                wall = wall + wts[abs(xi)] * wts[abs(yi)];
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
+            const int val = v[dy * 5 + dx];
+            if (cy >= 0 && cy < sh && cx >= 0 && cx < sw && (float)abs(val - center) < 90.0f) {
+                sum = sum + (float)val * wts[dx] * wts[dy];
+                wall = wall + wts[dx] * wts[dy];
             }
         }
     return (uint16_t)(int)(sum / wall);
@@ -645,9 +556,16 @@ __global__ __launch_bounds__(256) void k_rgb_prep(PrepLevels a, Seg3 g)
     a.dxy[l][k] = make_short2(sx, sy);
     float z = QNAN;
     if (x < w - 5 && y < h - 1) {
+        // rows y-2..y+1 and columns x-2..x+1 inside the image (the upper
+        // bounds hold here), all 16 loads issued before any test
         bool valid = true;
-        for (int u = max(y - 2, 0); u < min(y + 2, h); u++)
-            for (int v = max(x - 2, 0); v < min(x + 2, w); v++) valid = valid && (img[(size_t)u * w + v] > 0);
+#pragma unroll
+        for (int du = -2; du < 2; ++du)
+#pragma unroll
+            for (int dv = -2; dv < 2; ++dv) {
+                const uint8_t t = img[(size_t)max(y + du, 0) * w + max(x + dv, 0)];
+                valid = valid & ((y + du < 0) | (x + dv < 0) | (t > 0));
+            }
         if (valid) {
             const float mTwo = (float)((int)sx * (int)sx + (int)sy * (int)sy);
             if (mTwo >= a.minScale[l]) {
@@ -665,35 +583,6 @@ void launch_rgb_prep(const PrepLevels& a, hipStream_t s)
     for (int l = 0; l < kLevels; ++l) n[l] = a.w[l] * a.h[l];
     const Seg3 g = make_seg(n, 256);
     hipLaunchKernelGGL(k_rgb_prep, dim3(g.b0[kLevels]), dim3(256), 0, s, a, g);
-}
-
-__global__ __launch_bounds__(256) void k_copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
-                                                 size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
-void launch_copy_u8(const uint8_t* src, uint8_t* dst, size_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_copy_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
-}
-
-// std::swap(lastNextImage[i], nextImage[i]) (RGBDOdometryef.cpp:585-591) as a
-// swap of contents, so buffer addresses never change.
-__global__ __launch_bounds__(256) void k_swap_u8(uint8_t* __restrict__ a, uint8_t* __restrict__ b, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        const uint8_t t = a[i];
-        a[i] = b[i];
-        b[i] = t;
-    }
-}
-
-void launch_swap_u8(uint8_t* a, uint8_t* b, size_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_swap_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
 }
 
 // ============================================================ reductions
@@ -1840,25 +1729,32 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     TrackState* st = D.st;
     PrevPose pp;
     load_prev(st, pp);
+    // read before the prologue, so the stores below wait on no load
+    float* const plog = st->pose_log;
+    const int plog_cap = st->pose_log_cap;
+    const int frame = st->frame_count;
     finish_previous<256>(D, P, K, last_level, pp, W);
     if (threadIdx.x != 0) return;
     const IterState& S = W.S;
     float R[9], t[3];
     for (int a = 0; a < 9; ++a) R[a] = S.Rcurr[a];
     for (int a = 0; a < 3; ++a) t[a] = S.tcurr[a];
-    st->reverted = 0;
+    int reverted = 0;
     if (P.rgb) {
-        const float d0 = t[0] - st->tprev[0], d1 = t[1] - st->tprev[1], d2 = t[2] - st->tprev[2];
+        const float d0 = t[0] - pp.t[0], d1 = t[1] - pp.t[1], d2 = t[2] - pp.t[2];
         if (sqrtf(d0 * d0 + d1 * d1 + d2 * d2) > 0.3f) {
-            for (int a = 0; a < 9; ++a) R[a] = st->Rprev[a];
-            for (int a = 0; a < 3; ++a) t[a] = st->tprev[a];
-            st->reverted = 1;
+            for (int a = 0; a < 9; ++a) R[a] = pp.R[a];
+            for (int a = 0; a < 3; ++a) t[a] = pp.t[a];
+            reverted = 1;
         }
     }
+    st->reverted = reverted;
+    float pose[12];
     for (int a = 0; a < 3; ++a) {
-        for (int b = 0; b < 3; ++b) st->pose_out[a * 4 + b] = R[a * 3 + b];
-        st->pose_out[a * 4 + 3] = t[a];
+        for (int b = 0; b < 3; ++b) pose[a * 4 + b] = R[a * 3 + b];
+        pose[a * 4 + 3] = t[a];
     }
+    for (int a = 0; a < 12; ++a) st->pose_out[a] = pose[a];
     for (int a = 0; a < 36; ++a) st->lastA[a] = S.lastA[a];
     for (int a = 0; a < 6; ++a) st->lastb[a] = S.lastb[a];
     st->lastICPError = S.lastICPError;
@@ -1867,11 +1763,11 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     st->lastRGBCount = S.lastRGBCount;
     st->iters_done = S.iters_done;
     st->degenerate = S.degenerate;
-    if (st->pose_log && st->pose_log_cap > 0) {
-        float* dst = st->pose_log + (size_t)(st->frame_count % st->pose_log_cap) * 12;
-        for (int a = 0; a < 12; ++a) dst[a] = st->pose_out[a];
+    if (plog && plog_cap > 0) {
+        float* dst = plog + (size_t)(frame % plog_cap) * 12;
+        for (int a = 0; a < 12; ++a) dst[a] = pose[a];
     }
-    st->frame_count++;
+    st->frame_count = frame + 1;
 }
 
 void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s)
