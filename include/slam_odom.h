@@ -164,7 +164,8 @@ int odom_read_buffer(odom_handle h, const char *name, int level, void *dst, size
 
 /* Launch shapes of the two Gauss-Newton step kernels (the role of the
  * reference's GPUConfig.h table, GPUConfig.h:7-149).  kernel 0 = ICP /
- * association pass, 1 = photometric pass. */
+ * association pass, 1 = photometric pass; threads 256 or 512, 1 to 4 pixels
+ * per thread per tile. */
 int odom_set_step_shape(odom_handle h, int kernel, int level, int threads, int pixels_per_thread);
 int odom_get_step_shape(odom_handle h, int kernel, int level, int *threads, int *pixels_per_thread);
 /* One getIncrementalTransformation on the current inputs and pose with a

@@ -516,8 +516,7 @@ int odom_create(const odom_params* p, odom_handle* out)
     for (auto& r : reqs) total += align_up(r.bytes, 256);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->arena, total) != hipSuccess) {
-        if (c->stream) (void)hipStreamDestroy(c->stream);
-        delete c;
+        odom_destroy(c);
         return ODOM_ERR_HIP;
     }
     c->arena_bytes = total;
@@ -1077,7 +1076,7 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
     if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
     if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     const int threads[] = {256, 512};
-    const int ppts[] = {1, 2, 4};
+    const int ppts[] = {1, 2, 3, 4};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -1496,7 +1495,7 @@ int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
     const int threads[] = {256, 512};
-    const int ppts[] = {1, 2, 4};
+    const int ppts[] = {1, 2, 3, 4};
     int rc = ODOM_OK;
     for (int kernel = 0; kernel < 2 && rc == ODOM_OK; ++kernel) {
         if (kernel == 1 && !P.rgb) break;

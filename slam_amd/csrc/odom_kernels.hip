@@ -22,6 +22,28 @@ namespace slam {
 
 #define QNAN __builtin_nanf("")
 
+// Device accesses through pointers held in descriptors and state records.
+// The compiler cannot tell such a pointer's address space and would use flat
+// instructions, which also count against the LDS/scalar wait counter (so a
+// wait for a descriptor field or an LDS read would also wait for pixel loads
+// in flight) and take a 64-bit address per lane.  gp() makes them global-
+// memory accesses, with the uniform base in scalar registers.
+#ifdef __HIP_DEVICE_COMPILE__
+#define GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define GLOBAL_AS  // the host pass only parses kernels
+#endif
+template <class T>
+__device__ __forceinline__ const GLOBAL_AS T* gp(const T* p)
+{
+    return (const GLOBAL_AS T*)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T* gp(T* p)
+{
+    return (GLOBAL_AS T*)p;
+}
+
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 float bexpf_host(float x) { return bexpf(x); }
@@ -208,59 +230,78 @@ struct Cam6 {
     float v[3], n[3];
 };
 
+// The texels are loaded whole and the NaN rules applied by selects, so every
+// load of an output is issued before the first wait (with branches the
+// compiler splits the loads and waits for each texel's z first).
 __device__ __forceinline__ Cam6 model_l0(const ModelLevels& a, int x, int y)
 {
-    const float4 vv = a.v4[(size_t)y * a.v_pitch + x];
-    const float4 nn = a.n4[(size_t)y * a.n_pitch + x];
+    const float4 vv = gp(a.v4)[(size_t)y * a.v_pitch + x];
+    const float4 nn = gp(a.n4)[(size_t)y * a.n_pitch + x];
+    const bool ok = !(vv.z == 0);
     Cam6 c;
-    if (!(vv.z == 0)) {
-        c.v[0] = vv.x; c.v[1] = vv.y; c.v[2] = vv.z;
-        c.n[0] = nn.x; c.n[1] = nn.y; c.n[2] = nn.z;
-    } else {
-        c.v[0] = c.v[1] = c.v[2] = QNAN;
-        c.n[0] = c.n[1] = c.n[2] = QNAN;
-    }
+    c.v[0] = ok ? vv.x : QNAN;
+    c.v[1] = ok ? vv.y : QNAN;
+    c.v[2] = ok ? vv.z : QNAN;
+    c.n[0] = ok ? nn.x : QNAN;
+    c.n[1] = ok ? nn.y : QNAN;
+    c.n[2] = ok ? nn.z : QNAN;
     return c;
 }
 
 __device__ __forceinline__ void mean4(const float* a, const float* b, const float* c, const float* d,
                                       bool normalize, float* o)
 {
-    if (isnan(a[0]) || isnan(b[0]) || isnan(c[0]) || isnan(d[0])) {
-        o[0] = o[1] = o[2] = QNAN;
-        return;
-    }
-    for (int k = 0; k < 3; ++k) o[k] = (a[k] + b[k] + c[k] + d[k]) / 4.0f;
+    const bool bad = isnan(a[0]) || isnan(b[0]) || isnan(c[0]) || isnan(d[0]);
+    float m[3];
+    for (int k = 0; k < 3; ++k) m[k] = (a[k] + b[k] + c[k] + d[k]) / 4.0f;
     if (normalize) {
-        const float rn = 1.0f / sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-        o[0] = o[0] * rn; o[1] = o[1] * rn; o[2] = o[2] * rn;
+        const float rn = 1.0f / sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        m[0] = m[0] * rn; m[1] = m[1] * rn; m[2] = m[2] * rn;
     }
+    for (int k = 0; k < 3; ++k) o[k] = bad ? QNAN : m[k];
+}
+
+// 2x2 mean of four texels (x, y), (x+1, y), (x, y+1), (x+1, y+1).
+__device__ __forceinline__ Cam6 mean_cam(const Cam6& c00, const Cam6& c01, const Cam6& c10, const Cam6& c11)
+{
+    Cam6 o;
+    mean4(c00.v, c01.v, c10.v, c11.v, false, o.v);
+    mean4(c00.n, c01.n, c10.n, c11.n, true, o.n);
+    return o;
 }
 
 __device__ __forceinline__ Cam6 model_l1(const ModelLevels& a, int x, int y)
 {
     const Cam6 c00 = model_l0(a, 2 * x, 2 * y), c01 = model_l0(a, 2 * x + 1, 2 * y);
     const Cam6 c10 = model_l0(a, 2 * x, 2 * y + 1), c11 = model_l0(a, 2 * x + 1, 2 * y + 1);
-    Cam6 o;
-    mean4(c00.v, c01.v, c10.v, c11.v, false, o.v);
-    mean4(c00.n, c01.n, c10.n, c11.n, true, o.n);
-    return o;
+    return mean_cam(c00, c01, c10, c11);
 }
 
+// Level 2 from the 4x4 level-0 texels: the level-1 means of its 2x2
+// quadrants, then their mean.
 __device__ __forceinline__ Cam6 model_l2(const ModelLevels& a, int x, int y)
 {
-    const Cam6 c00 = model_l1(a, 2 * x, 2 * y), c01 = model_l1(a, 2 * x + 1, 2 * y);
-    const Cam6 c10 = model_l1(a, 2 * x, 2 * y + 1), c11 = model_l1(a, 2 * x + 1, 2 * y + 1);
-    Cam6 o;
-    mean4(c00.v, c01.v, c10.v, c11.v, false, o.v);
-    mean4(c00.n, c01.n, c10.n, c11.n, true, o.n);
-    return o;
+    Cam6 t[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[r][q] = model_l0(a, 4 * x + q, 4 * y + r);
+    Cam6 m[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            m[i][j] = mean_cam(t[2 * i][2 * j], t[2 * i][2 * j + 1], t[2 * i + 1][2 * j], t[2 * i + 1][2 * j + 1]);
+    return mean_cam(m[0][0], m[0][1], m[1][0], m[1][1]);
 }
 
 __global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, Seg3 g)
 {
-    const int l = seg_level(g, blockIdx.x);
-    const int i = (blockIdx.x - g.b0[l]) * 256 + threadIdx.x;
+    // segments in the order level 2, 1, 0: the blocks with the most loads
+    // per thread (a level-2 output reads 16 level-0 texels) start first
+    const int seg = seg_level(g, blockIdx.x);
+    const int l = 2 - seg;
+    const int i = (blockIdx.x - g.b0[seg]) * 256 + threadIdx.x;
     const int w = a.w[l], h = a.h[l];
     if (i >= w * h) return;
     const int y = i / w, x = i - y * w;
@@ -300,7 +341,7 @@ __global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, Seg3 g)
 void launch_model_maps(const ModelLevels& a, hipStream_t s)
 {
     int n[kLevels];
-    for (int l = 0; l < kLevels; ++l) n[l] = a.w[l] * a.h[l];
+    for (int l = 0; l < kLevels; ++l) n[l] = a.w[2 - l] * a.h[2 - l];
     const Seg3 g = make_seg(n, 256);
     hipLaunchKernelGGL(k_model_maps, dim3(g.b0[kLevels]), dim3(256), 0, s, a, g);
 }
@@ -613,26 +654,68 @@ __device__ __forceinline__ unsigned int wave_total_u(unsigned int v)
     return v;
 }
 
+// Wave totals of V <= 32 values per lane by reduce-scatter.  Each step pairs
+// every lane with a lane in the other half of its group of 64, 32, 16, 8 or 4
+// lanes; the lane keeps half of its values and adds the partner's copy of
+// that half.  The groups of 64 and 32 exchange by v_permlane32_swap and
+// v_permlane16_swap (one swap moves a value pair between the two halves), the
+// others by DPP row_mirror, row_half_mirror and quad_perm.  After a last add
+// across lanes 2c and 2c+1 both hold the total of value c.  About 70
+// operations for 32 values, where a scan per value takes 6 dependent ones.
 template <int V>
-__device__ __forceinline__ void wave_sum(float (&a)[V])
+__device__ __forceinline__ float wave_scatter(const float (&a)[V])
 {
+    static_assert(V <= 32, "one value per lane pair");
+    const int lane = threadIdx.x & 63;
+    float s16[16];
 #pragma unroll
-    for (int k = 0; k < V; ++k) a[k] = wave_total_f(a[k]);
+    for (int j = 0; j < 16; ++j) {
+        if (j >= V) {
+            s16[j] = 0.0f;
+            continue;
+        }
+        const float y = (j + 16 < V) ? a[j + 16] : 0.0f;
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a[j]), __float_as_uint(y), false, false);
+        s16[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    float s8[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s16[j]), __float_as_uint(s16[j + 8]), false,
+                                                        false);
+        s8[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const bool h8 = (lane & 8) != 0, h4 = (lane & 4) != 0, h2 = (lane & 2) != 0;
+    float s4[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float send = h8 ? s8[j] : s8[j + 4], keep = h8 ? s8[j + 4] : s8[j];
+        s4[j] = keep + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, send), 0x140, 0xf));
+    }
+    float s2[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float send = h4 ? s4[j] : s4[j + 2], keep = h4 ? s4[j + 2] : s4[j];
+        s2[j] = keep + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, send), 0x141, 0xf));
+    }
+    const float send = h2 ? s2[0] : s2[1], keep = h2 ? s2[1] : s2[0];
+    const float s1 = keep + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, send), 0x4e, 0xf));
+    return s1 + __builtin_bit_cast(float, ODOM_DPP(__builtin_bit_cast(int, s1), 0xb1, 0xf));
 }
 
-// Block partial of V floats: wave totals (lane 63), then the waves in order.
+// Block partial of V floats: wave totals (value c in lanes 2c, 2c+1), then
+// the waves in order.
 template <int V, int NT>
-__device__ __forceinline__ void block_partial(float (&a)[V], float* __restrict__ out, float (*red)[V])
+__device__ __forceinline__ void block_partial(const float (&a)[V], float* __restrict__ out, float (*red)[V])
 {
-    wave_sum<V>(a);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 63)
-        for (int k = 0; k < V; ++k) red[wid][k] = a[k];
+    const float t = wave_scatter<V>(a);
+    if ((lane & 1) == 0 && (lane >> 1) < V) red[wid][lane >> 1] = t;
     __syncthreads();
     if (threadIdx.x < V) {
         float s = 0.0f;
         for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
-        out[threadIdx.x] = s;
+        gp(out)[threadIdx.x] = s;
     }
 }
 
@@ -651,7 +734,7 @@ __device__ __forceinline__ void block_partial_int(unsigned int c0, unsigned int 
     if (threadIdx.x < 2) {
         unsigned int s = 0;
         for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
-        out[threadIdx.x] = s;
+        gp(out)[threadIdx.x] = s;
     }
 }
 
@@ -691,6 +774,27 @@ __device__ __forceinline__ void partials_total2(const float* __restrict__ pa, in
         double s = 0.0;
         for (int gi = 0; gi < G; ++gi) s += red[t][gi];
         tot[t] = s;
+    }
+    __syncthreads();
+}
+
+// Wrapping uint32 totals of one (c0, c1) pair per thread, already loaded;
+// every thread sees tot[2].
+template <int NT>
+__device__ __forceinline__ void int_total_reg(unsigned int c0, unsigned int c1, unsigned int* tot,
+                                              unsigned int (*red)[2])
+{
+    c0 = wave_total_u(c0);
+    c1 = wave_total_u(c1);
+    if ((threadIdx.x & 63) == 63) {
+        red[threadIdx.x >> 6][0] = c0;
+        red[threadIdx.x >> 6][1] = c1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        unsigned int s = 0;
+        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
+        tot[threadIdx.x] = s;
     }
     __syncthreads();
 }
@@ -735,12 +839,12 @@ struct Px6 {
 
 __device__ __forceinline__ void icp_load(const LevelDesc& L, int i, size_t P, Px6& s)
 {
-    s.v[0] = L.vcur[i];
-    s.v[1] = L.vcur[i + P];
-    s.v[2] = L.vcur[i + 2 * P];
-    s.n[0] = L.ncur[i];
-    s.n[1] = L.ncur[i + P];
-    s.n[2] = L.ncur[i + 2 * P];
+    s.v[0] = gp(L.vcur)[i];
+    s.v[1] = gp(L.vcur)[i + P];
+    s.v[2] = gp(L.vcur)[i + 2 * P];
+    s.n[0] = gp(L.ncur)[i];
+    s.n[1] = gp(L.ncur)[i + P];
+    s.n[2] = gp(L.ncur)[i + 2 * P];
 }
 
 // Projective association (reduce.cu:290-298).  Returns the gather index or
@@ -762,12 +866,12 @@ __device__ __forceinline__ int icp_project(const LevelDesc& L, const IcpC& c, co
 
 __device__ __forceinline__ void icp_gather(const LevelDesc& L, int j, size_t P, Px6& d)
 {
-    d.v[0] = L.vprev[j];
-    d.v[1] = L.vprev[j + P];
-    d.v[2] = L.vprev[j + 2 * P];
-    d.n[0] = L.nprev[j];
-    d.n[1] = L.nprev[j + P];
-    d.n[2] = L.nprev[j + 2 * P];
+    d.v[0] = gp(L.vprev)[j];
+    d.v[1] = gp(L.vprev)[j + P];
+    d.v[2] = gp(L.vprev)[j + 2 * P];
+    d.n[0] = gp(L.nprev)[j];
+    d.n[1] = gp(L.nprev)[j + P];
+    d.n[2] = gp(L.nprev)[j + 2 * P];
 }
 
 // reduce.cu:305-348: normal/distance tests and the point-to-plane row.
@@ -1008,13 +1112,23 @@ __device__ void icp_stats(IterState& S, const TrackParams& P, const double* icp_
     S.lastICPCount = S.res1;
 }
 
-// Word-by-word copy of a state record by all threads of the block.
+// Load of a state record from global memory into LDS, word per thread.
 template <int NT, class T>
-__device__ __forceinline__ void copy_words(T* dst, const T* src)
+__device__ __forceinline__ void copy_in(T* lds, const T* src)
 {
     static_assert(sizeof(T) % 4 == 0, "state records are word aligned");
-    const unsigned int* s = (const unsigned int*)src;
-    unsigned int* d = (unsigned int*)dst;
+    const GLOBAL_AS unsigned int* s = gp((const unsigned int*)src);
+    unsigned int* d = (unsigned int*)lds;
+    for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += NT) d[i] = s[i];
+}
+
+// Store of a record from LDS to global memory, word per thread.
+template <int NT, class T>
+__device__ __forceinline__ void copy_out(T* dst, const T* lds)
+{
+    static_assert(sizeof(T) % 4 == 0, "state records are word aligned");
+    const unsigned int* s = (const unsigned int*)lds;
+    GLOBAL_AS unsigned int* d = gp((unsigned int*)dst);
     for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += NT) d[i] = s[i];
 }
 
@@ -1025,8 +1139,9 @@ struct PrevPose {
     float R[9], t[3], Ri[9];
 };
 
-__device__ __forceinline__ void load_prev(const TrackState* st, PrevPose& pp)
+__device__ __forceinline__ void load_prev(const TrackState* st_, PrevPose& pp)
 {
+    const GLOBAL_AS TrackState* st = gp(st_);
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         pp.R[q] = st->Rprev[q];
@@ -1061,7 +1176,7 @@ __device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int
 #pragma unroll
             for (int i = 0; i < IT; ++i) {
                 const int r = rg + i * RG;
-                L.v[a][i] = *(const float4*)(part + (size_t)(r < n ? r : 0) * kPartStride + 4 * c);
+                L.v[a][i] = *gp((const float4*)(part + (size_t)(r < n ? r : 0) * kPartStride + 4 * c));
             }
         } else {
 #pragma unroll
@@ -1150,14 +1265,14 @@ __device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, 
 {
     TrackState* st = D.st;
     if (k == 0) {
-        copy_words<NT>(&W.S, &st->it[1]);
+        copy_in<NT>(&W.S, &st->it[1]);
         __syncthreads();
         return;
     }
     const int pb = (k - 1) & 1;
     const int na = D.nblk_a[prev_level], nb = D.nblk_b[prev_level];
     PartLoad<NT> PL;
-    copy_words<NT>(&W.S, &st->it[pb]);
+    copy_in<NT>(&W.S, &st->it[pb]);
     partials_issue<NT>(P.icp ? D.part_a[pb] : nullptr, na, P.rgb ? D.part_b[pb] : nullptr, nb, PL);
     __syncthreads();
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
@@ -1205,8 +1320,8 @@ __device__ __forceinline__ void load_tile_a(const LevelDesc& L, const TrackParam
         if (idx < N) {
             if (P.icp) icp_load(L, idx, P3, T.src[q]);
             if (P.rgb) {
-                T.zs[q] = L.zsrc[idx];
-                T.in[q] = L.inext[idx];
+                T.zs[q] = gp(L.zsrc)[idx];
+                T.in[q] = gp(L.inext)[idx];
             }
         }
     }
@@ -1252,8 +1367,8 @@ __device__ __forceinline__ void work_tile_a(const LevelDesc& L, const TrackParam
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
             if (jm[q] >= 0) {
-                d0[q] = L.zlast[jm[q]];
-                il[q] = L.ilast[jm[q]];
+                d0[q] = gp(L.zlast)[jm[q]];
+                il[q] = gp(L.ilast)[jm[q]];
             }
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
@@ -1299,9 +1414,9 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
         finish_previous<NT>(D, P, k, prev_level, pp, W);
         if (threadIdx.x == 0) enter_iteration(W.S, L, level, iter);
         __syncthreads();
-        if (lb == 0) copy_words<NT>(&st->it[k & 1], &W.S);
+        if (lb == 0) copy_out<NT>(&st->it[k & 1], &W.S);
     } else {
-        copy_words<NT>(&W.S, &st->it[k & 1]);
+        copy_in<NT>(&W.S, &st->it[k & 1]);
         __syncthreads();
     }
     if (W.S.skip) return;
@@ -1351,7 +1466,7 @@ __global__ __launch_bounds__(256) void k_solve(const PairDesc* __restrict__ pair
     finish_previous<256>(D, P, k, prev_level, pp, W);
     if (threadIdx.x == 0) enter_iteration(W.S, D.lv[level], level, iter);
     __syncthreads();
-    copy_words<256>(&st->it[k & 1], &W.S);
+    copy_out<256>(&st->it[k & 1], &W.S);
 }
 
 // ======================================================== step kernel B
@@ -1375,9 +1490,9 @@ __device__ __forceinline__ void load_tile_b(const LevelDesc& L, int base, int N,
         T.in[q] = 0;
         T.gr[q] = make_short2(0, 0);
         if (idx < N) {
-            T.zs[q] = L.zsrc[idx];
-            T.in[q] = L.inext[idx];
-            T.gr[q] = L.dxy[idx];
+            T.zs[q] = gp(L.zsrc)[idx];
+            T.in[q] = gp(L.inext)[idx];
+            T.gr[q] = gp(L.dxy)[idx];
         }
     }
 }
@@ -1397,18 +1512,34 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
     const int N = L.w * L.h;
     const int ntiles = (N + NT * PPT - 1) / (NT * PPT);
     const int pb = k & 1;
-    const IterState& Sg = st->it[mode == 0 ? pb : 0];
-    if (mode == 0 && Sg.skip) return;
+    const GLOBAL_AS IterState& Sg = *gp(&st->it[mode == 0 ? pb : 0]);
+    static_assert(NT >= kMaxBlk, "one count partial per thread");
 
+    // Every load of the prologue is issued before its first wait: the first
+    // tile, pass A's count partials and the state fields of the iteration.
     TileB<NT, PPT> T;
     load_tile_b<NT, PPT>(L, lb * NT * PPT + threadIdx.x, N, T);
+    unsigned int c0 = 0, c1 = 0;
+    if (mode == 0 && (int)threadIdx.x < D.nblk_a[level]) {
+        const uint2 c = gp((const uint2*)D.part_ai[pb])[threadIdx.x];
+        c0 = c.x;
+        c1 = c.y;
+    }
+    const int skip = Sg.skip;
+    const float err_cmp = Sg.rgb_err_cmp;
+    const float sigma_g = Sg.sigmaVal;
+    RgbC rc;
+    for (int q = 0; q < 9; ++q) rc.K[q] = uni(Sg.krkinv[q]);
+    for (int q = 0; q < 3; ++q) rc.kt[q] = uni(Sg.kt[q]);
+    rc.maxDepthDelta = P.maxDepthDelta;
+    if (mode == 0 && skip) return;
 
     float sigma;
     if (mode == 0) {
-        int_total<NT>(D.part_ai[pb], D.nblk_a[level], W.toti, W.redi);
+        int_total_reg<NT>(c0, c1, W.toti, W.redi);
         if (threadIdx.x == 0) {
             IterState& S = W.S;
-            assoc_rgb(S, P, Sg.rgb_err_cmp, (int)W.toti[0], (int)W.toti[1]);
+            assoc_rgb(S, P, err_cmp, (int)W.toti[0], (int)W.toti[1]);
             if (lb == 0) {
                 IterState* G = &st->it[pb];
                 G->rgb_count = S.rgb_count;
@@ -1426,13 +1557,8 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
         if (W.S.broke) return;
         sigma = W.S.sigmaVal;
     } else {
-        sigma = Sg.sigmaVal;
+        sigma = sigma_g;
     }
-
-    RgbC rc;
-    for (int q = 0; q < 9; ++q) rc.K[q] = uni(Sg.krkinv[q]);
-    for (int q = 0; q < 3; ++q) rc.kt[q] = uni(Sg.kt[q]);
-    rc.maxDepthDelta = P.maxDepthDelta;
 
     float acc[kSE3];
 #pragma unroll
@@ -1452,8 +1578,8 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
             if (jm[q] >= 0) {
-                d0[q] = L.zlast[jm[q]];
-                il[q] = L.ilast[jm[q]];
+                d0[q] = gp(L.zlast)[jm[q]];
+                il[q] = gp(L.ilast)[jm[q]];
             }
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
@@ -1491,7 +1617,7 @@ void launch_reduce_dbg(const PairDesc* pairs, int which, int level, hipStream_t 
 int step_shape_supported(StepShape sh)
 {
     const bool t = sh.threads == 256 || sh.threads == 512;
-    const bool p = sh.ppt == 1 || sh.ppt == 2 || sh.ppt == 4;
+    const bool p = sh.ppt >= 1 && sh.ppt <= 4;
     return t && p;
 }
 
@@ -1501,9 +1627,11 @@ int step_shape_supported(StepShape sh)
         switch (L.shape.threads * 100 + L.shape.ppt) {                                                   \
             case 25601: hipLaunchKernelGGL((KER<256, 1>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
             case 25602: hipLaunchKernelGGL((KER<256, 2>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
+            case 25603: hipLaunchKernelGGL((KER<256, 3>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
             case 25604: hipLaunchKernelGGL((KER<256, 4>), grid, dim3(256), 0, s, __VA_ARGS__); break;    \
             case 51201: hipLaunchKernelGGL((KER<512, 1>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
             case 51202: hipLaunchKernelGGL((KER<512, 2>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
+            case 51203: hipLaunchKernelGGL((KER<512, 3>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
             case 51204: hipLaunchKernelGGL((KER<512, 4>), grid, dim3(512), 0, s, __VA_ARGS__); break;    \
             default: break;                                                                              \
         }                                                                                                \
@@ -1515,9 +1643,11 @@ int step_shape_supported(StepShape sh)
         switch (L.shape.threads * 100 + L.shape.ppt) {                                                   \
             case 25601: hipLaunchKernelGGL((k_step_a<256, 1, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
             case 25602: hipLaunchKernelGGL((k_step_a<256, 2, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            case 25603: hipLaunchKernelGGL((k_step_a<256, 3, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
             case 25604: hipLaunchKernelGGL((k_step_a<256, 4, FUS>), grid, dim3(256), 0, s, __VA_ARGS__); break; \
             case 51201: hipLaunchKernelGGL((k_step_a<512, 1, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
             case 51202: hipLaunchKernelGGL((k_step_a<512, 2, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
+            case 51203: hipLaunchKernelGGL((k_step_a<512, 3, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
             case 51204: hipLaunchKernelGGL((k_step_a<512, 4, FUS>), grid, dim3(512), 0, s, __VA_ARGS__); break; \
             default: break;                                                                              \
         }                                                                                                \
@@ -1664,54 +1794,81 @@ __device__ int seed_keep(const PairDesc& D)
 {
     double t[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = 0; b < kSeedBlocks; ++b)
-        for (int q = 0; q < 4; ++q) t[q] += (double)D.part_b[0][(size_t)b * kPartStride + q];
+        for (int q = 0; q < 4; ++q) t[q] += (double)gp(D.part_b[0])[(size_t)b * kPartStride + q];
     const double m0 = t[1] / fmax(t[0], 1.0), m1 = t[3] / fmax(t[2], 1.0);
     return (t[2] >= 0.99 * t[0]) && (m1 <= 1.01 * m0);
 }
 
-__global__ void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
+__global__ __launch_bounds__(64) void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs) return;
-    TrackState* st = pairs[p].st;
+    GLOBAL_AS TrackState* st = gp(pairs[p].st);
+    // every load first, then the stores
+    float pose[12], errs[4];
+    double so3R[9], lastA[36], lastb[6], seed[16];
+    for (int a = 0; a < 12; ++a) pose[a] = st->pose_out[a];
+    for (int a = 0; a < 9; ++a) so3R[a] = P.so3 ? st->so3R[a] : 0.0;
+    for (int a = 0; a < 36; ++a) lastA[a] = st->lastA[a];
+    for (int a = 0; a < 6; ++a) lastb[a] = st->lastb[a];
+    for (int a = 0; a < 16; ++a) seed[a] = P.seeded ? st->seedRt[a] : 0.0;
+    errs[0] = st->lastICPError;
+    errs[1] = st->lastICPCount;
+    errs[2] = st->lastRGBError;
+    errs[3] = st->lastRGBCount;
+    const int keep = (P.seeded == 2) ? seed_keep(pairs[p]) : 1;
+
+    float Rprev[9], tprev[3], Rprev_inv[9];
     for (int a = 0; a < 3; ++a) {
-        for (int b = 0; b < 3; ++b) st->Rprev[a * 3 + b] = st->pose_out[a * 4 + b];
-        st->tprev[a] = st->pose_out[a * 4 + 3];
+        for (int b = 0; b < 3; ++b) Rprev[a * 3 + b] = pose[a * 4 + b];
+        tprev[a] = pose[a * 4 + 3];
     }
-    inv3f(st->Rprev, st->Rprev_inv);
-    IterState& S = st->it[1];
-    for (int a = 0; a < 9; ++a) S.Rcurr[a] = st->Rprev[a];
-    for (int a = 0; a < 3; ++a) S.tcurr[a] = st->tprev[a];
-    for (int a = 0; a < 16; ++a) S.resultRt[a] = (a % 5 == 0) ? 1.0 : 0.0;
+    inv3f(Rprev, Rprev_inv);
+    double resultRt[16];
+    float Rcurr[9], tcurr[3];
+    for (int a = 0; a < 9; ++a) Rcurr[a] = Rprev[a];
+    for (int a = 0; a < 3; ++a) tcurr[a] = tprev[a];
+    for (int a = 0; a < 16; ++a) resultRt[a] = (a % 5 == 0) ? 1.0 : 0.0;
     if (P.so3)
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) S.resultRt[a * 4 + b] = st->so3R[a * 3 + b];
+            for (int b = 0; b < 3; ++b) resultRt[a * 4 + b] = so3R[a * 3 + b];
     if (P.seeded) {
         // the warm start (seeded 2: only if the level-2 scores keep it):
         // resultRt = seed, and (unlike the SO3 seed of the reference) the
         // current estimate prev * seed^-1 from the first pass
-        const int keep = (P.seeded == 2) ? seed_keep(pairs[p]) : 1;
-        for (int a = 0; a < 16; ++a) S.resultRt[a] = keep ? st->seedRt[a] : ((a % 5 == 0) ? 1.0 : 0.0);
-        pose_from_result(S.resultRt, st->Rprev, st->tprev, S.Rcurr, S.tcurr);
-        st->seed_used = keep;
-        st->seed_calls++;
-        st->seed_kept += keep;
+        for (int a = 0; a < 16; ++a) resultRt[a] = keep ? seed[a] : ((a % 5 == 0) ? 1.0 : 0.0);
+        pose_from_result(resultRt, Rprev, tprev, Rcurr, tcurr);
     }
-    for (int a = 0; a < 36; ++a) S.lastA[a] = st->lastA[a];
-    for (int a = 0; a < 6; ++a) S.lastb[a] = st->lastb[a];
-    S.lastICPError = st->lastICPError;
-    S.lastICPCount = st->lastICPCount;
-    S.lastRGBError = st->lastRGBError;
-    S.lastRGBCount = st->lastRGBCount;
-    S.res0 = 0.0f;
-    S.res1 = 0.0f;
-    S.skip = 0;
-    S.break_level = -1;
-    S.pending = 0;
-    S.broke = 0;
-    S.degenerate = 0;
-    S.iters_done = 0;
-    S.rgb_err_cmp = FLT_MAX;
+
+    for (int a = 0; a < 9; ++a) {
+        st->Rprev[a] = Rprev[a];
+        st->Rprev_inv[a] = Rprev_inv[a];
+    }
+    for (int a = 0; a < 3; ++a) st->tprev[a] = tprev[a];
+    if (P.seeded) {
+        st->seed_used = keep;
+        st->seed_calls = st->seed_calls + 1;
+        st->seed_kept = st->seed_kept + keep;
+    }
+    GLOBAL_AS IterState& G = st->it[1];
+    for (int a = 0; a < 16; ++a) G.resultRt[a] = resultRt[a];
+    for (int a = 0; a < 9; ++a) G.Rcurr[a] = Rcurr[a];
+    for (int a = 0; a < 3; ++a) G.tcurr[a] = tcurr[a];
+    for (int a = 0; a < 36; ++a) G.lastA[a] = lastA[a];
+    for (int a = 0; a < 6; ++a) G.lastb[a] = lastb[a];
+    G.lastICPError = errs[0];
+    G.lastICPCount = errs[1];
+    G.lastRGBError = errs[2];
+    G.lastRGBCount = errs[3];
+    G.res0 = 0.0f;
+    G.res1 = 0.0f;
+    G.skip = 0;
+    G.break_level = -1;
+    G.pending = 0;
+    G.broke = 0;
+    G.degenerate = 0;
+    G.iters_done = 0;
+    G.rgb_err_cmp = FLT_MAX;
 }
 
 void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s)
@@ -1730,11 +1887,12 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     PrevPose pp;
     load_prev(st, pp);
     // read before the prologue, so the stores below wait on no load
-    float* const plog = st->pose_log;
-    const int plog_cap = st->pose_log_cap;
-    const int frame = st->frame_count;
+    float* const plog = gp(st)->pose_log;
+    const int plog_cap = gp(st)->pose_log_cap;
+    const int frame = gp(st)->frame_count;
     finish_previous<256>(D, P, K, last_level, pp, W);
     if (threadIdx.x != 0) return;
+    GLOBAL_AS TrackState* g = gp(st);
     const IterState& S = W.S;
     float R[9], t[3];
     for (int a = 0; a < 9; ++a) R[a] = S.Rcurr[a];
@@ -1748,26 +1906,26 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
             reverted = 1;
         }
     }
-    st->reverted = reverted;
+    g->reverted = reverted;
     float pose[12];
     for (int a = 0; a < 3; ++a) {
         for (int b = 0; b < 3; ++b) pose[a * 4 + b] = R[a * 3 + b];
         pose[a * 4 + 3] = t[a];
     }
-    for (int a = 0; a < 12; ++a) st->pose_out[a] = pose[a];
-    for (int a = 0; a < 36; ++a) st->lastA[a] = S.lastA[a];
-    for (int a = 0; a < 6; ++a) st->lastb[a] = S.lastb[a];
-    st->lastICPError = S.lastICPError;
-    st->lastICPCount = S.lastICPCount;
-    st->lastRGBError = S.lastRGBError;
-    st->lastRGBCount = S.lastRGBCount;
-    st->iters_done = S.iters_done;
-    st->degenerate = S.degenerate;
+    for (int a = 0; a < 12; ++a) g->pose_out[a] = pose[a];
+    for (int a = 0; a < 36; ++a) g->lastA[a] = S.lastA[a];
+    for (int a = 0; a < 6; ++a) g->lastb[a] = S.lastb[a];
+    g->lastICPError = S.lastICPError;
+    g->lastICPCount = S.lastICPCount;
+    g->lastRGBError = S.lastRGBError;
+    g->lastRGBCount = S.lastRGBCount;
+    g->iters_done = S.iters_done;
+    g->degenerate = S.degenerate;
     if (plog && plog_cap > 0) {
-        float* dst = plog + (size_t)(frame % plog_cap) * 12;
+        GLOBAL_AS float* dst = gp(plog) + (size_t)(frame % plog_cap) * 12;
         for (int a = 0; a < 12; ++a) dst[a] = pose[a];
     }
-    st->frame_count = frame + 1;
+    g->frame_count = frame + 1;
 }
 
 void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s)
@@ -1776,11 +1934,16 @@ void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, i
 }
 
 // ================================================================= SO3
-// a17 SO3Reduction (reduce.cu:937-1075) and the host loop of
-// RGBDOdometryef.cpp:294-380, one workgroup per pair on pyramid level 2: all
-// (at most 10) iterations, the FP64 3x3 solve and the convergence tests run
-// inside one launch.
-__device__ __forceinline__ void so3_grad(const uint8_t* img, int w, int x, int y, float* g)
+// a17 SO3Reduction (reduce.cu:937-1075) per level-2 pixel, split into the
+// part that does not depend on the rotation (the last image's gradient and
+// value at the pixel, loaded before a launch's prologue), the warp, and the
+// part at the warped pixel of the next image.
+struct So3Src {
+    float g[2], v;
+};
+
+template <class Ptr>
+__device__ __forceinline__ void so3_grad(Ptr img, int w, int x, int y, float* g)
 {
     const float actu = (float)img[(size_t)y * w + x];
     float back = (float)img[(size_t)y * w + x - 1];
@@ -1791,22 +1954,42 @@ __device__ __forceinline__ void so3_grad(const uint8_t* img, int w, int x, int y
     g[1] = ((back + actu) / 2.0f) - ((fore + actu) / 2.0f);
 }
 
-__device__ __forceinline__ bool so3_pixel(const uint8_t* __restrict__ last, const uint8_t* __restrict__ next,
-                                          int w, int h, int x, int y, const float* H, const float* Ki,
-                                          const float* KR, float* row)
+__device__ __forceinline__ bool so3_interior(int w, int h, int x, int y)
+{
+    return x >= 1 && x < w - 1 && y >= 1 && y < h - 1;
+}
+
+template <class Ptr>
+__device__ __forceinline__ void so3_src(Ptr last, int w, int h, int x, int y, So3Src& s)
+{
+    s.g[0] = s.g[1] = s.v = 0.0f;
+    if (so3_interior(w, h, x, y)) {
+        so3_grad(last, w, x, y, s.g);
+        s.v = (float)last[(size_t)y * w + x];
+    }
+}
+
+// The warped pixel (wx, wy) of (x, y); false where the reference skips it.
+__device__ __forceinline__ bool so3_warp(const float* H, int w, int h, int x, int y, int& wx, int& wy)
 {
     const float xf = (float)x, yf = (float)y;
     const float wp0 = H[0] * xf + H[1] * yf + H[2] * 1.0f;
     const float wp1 = H[3] * xf + H[4] * yf + H[5] * 1.0f;
     const float wp2 = H[6] * xf + H[7] * yf + H[8] * 1.0f;
-    const int wx = f2i_rn(wp0 / wp2), wy = f2i_rn(wp1 / wp2);
-    if (!(wx >= 1 && wx < w - 1 && wy >= 1 && wy < h - 1 && x >= 1 && x < w - 1 && y >= 1 && y < h - 1))
-        return false;
-    float gn[2], gl[2];
+    wx = f2i_rn(wp0 / wp2);
+    wy = f2i_rn(wp1 / wp2);
+    return wx >= 1 && wx < w - 1 && wy >= 1 && wy < h - 1 && so3_interior(w, h, x, y);
+}
+
+template <class Ptr>
+__device__ __forceinline__ void so3_row(Ptr next, int w, int wx, int wy, const So3Src& src, int x, int y,
+                                        const float* Ki, const float* KR, float* row)
+{
+    const float xf = (float)x, yf = (float)y;
+    float gn[2];
     so3_grad(next, w, wx, wy, gn);
-    so3_grad(last, w, x, y, gl);
-    const float gx = (gn[0] + gl[0]) / 2.0f;
-    const float gy = (gn[1] + gl[1]) / 2.0f;
+    const float gx = (gn[0] + src.g[0]) / 2.0f;
+    const float gy = (gn[1] + src.g[1]) / 2.0f;
     const float pt0 = Ki[0] * xf + Ki[1] * yf + Ki[2] * 1.0f;
     const float pt1 = Ki[3] * xf + Ki[4] * yf + Ki[5] * 1.0f;
     const float pt2 = Ki[6] * xf + Ki[7] * yf + Ki[8] * 1.0f;
@@ -1818,7 +2001,19 @@ __device__ __forceinline__ bool so3_pixel(const uint8_t* __restrict__ last, cons
     row[0] = lp1 * pt2 - lp2 * pt1;
     row[1] = lp2 * pt0 - lp0 * pt2;
     row[2] = lp0 * pt1 - lp1 * pt0;
-    row[3] = -((float)next[(size_t)wy * w + wx] - (float)last[(size_t)y * w + x]);
+    row[3] = -((float)next[(size_t)wy * w + wx] - src.v);
+}
+
+// One pixel in full (the parity-check kernel).
+template <class Ptr>
+__device__ __forceinline__ bool so3_pixel(Ptr last, Ptr next, int w, int h, int x, int y, const float* H,
+                                          const float* Ki, const float* KR, float* row)
+{
+    int wx, wy;
+    if (!so3_warp(H, w, h, x, y, wx, wy)) return false;
+    So3Src src;
+    so3_src(last, w, h, x, y, src);
+    so3_row(next, w, wx, wy, src, x, y, Ki, KR, row);
     return true;
 }
 
@@ -1840,9 +2035,9 @@ __device__ __forceinline__ void acc_row4(float (&acc)[kSO3], const float* row)
 template <int NT>
 __device__ __forceinline__ void so3_block_total(float (&acc)[kSO3], float (*red)[kSO3], double* tot)
 {
-    wave_sum<kSO3>(acc);
-    if ((threadIdx.x & 63) == 63)
-        for (int k = 0; k < kSO3; ++k) red[threadIdx.x >> 6][k] = acc[k];
+    const int lane = threadIdx.x & 63;
+    const float t = wave_scatter<kSO3>(acc);
+    if ((lane & 1) == 0 && (lane >> 1) < kSO3) red[threadIdx.x >> 6][lane >> 1] = t;
     __syncthreads();
     if (threadIdx.x < kSO3) {
         double s = 0.0;
@@ -1854,8 +2049,6 @@ __device__ __forceinline__ void so3_block_total(float (&acc)[kSO3], float (*red)
 
 constexpr int kSO3Threads = 512;
 
-// With LDS, both level-2 images are staged in LDS (dynamic shared memory of
-// 2 * max(w*h) bytes) before the iterations.
 // The SO3 loop as up to 11 launches of kSO3Blocks workgroups per pair.
 // Launch `it` (< 10) first evaluates iteration it-1 from its block partials
 // -- FP64 totals in a fixed order, the convergence / divergence tests and the
@@ -1864,6 +2057,9 @@ constexpr int kSO3Threads = 512;
 // Launch 10 only evaluates iteration 9 and stores the result.  Block 0 of a
 // pair stores the loop state of each launch (slot it & 1).
 constexpr int kSO3Blocks = 16;
+// pixels per thread whose rotation-independent part is loaded before the
+// prologue (all of a level 2 of up to 3 * 16 * 512 pixels)
+constexpr int kSO3Pre = 3;
 
 __device__ void so3_evaluate(So3Slot& S, const double* tot)
 {
@@ -1907,6 +2103,24 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
     const PairDesc& D = pairs[p];
     TrackState* st = D.st;
     const LevelDesc& L = D.lv[2];
+    const int w = L.w, h = L.h, n = w * h;
+    constexpr int stride = kSO3Blocks * kSO3Threads;
+    const int first = lb * kSO3Threads + threadIdx.x;
+    const GLOBAL_AS uint8_t* last = gp(D.so3_last);
+    const GLOBAL_AS uint8_t* next = gp(D.so3_next);
+
+    // the rotation-independent loads of this thread's pixels overlap the prologue
+    So3Src src[kSO3Pre];
+    if (it < 10) {
+#pragma unroll
+        for (int j = 0; j < kSO3Pre; ++j) {
+            const int idx = first + j * stride;
+            if (idx < n) {
+                const int y = idx / w, x = idx - y * w;
+                so3_src(last, w, h, x, y, src[j]);
+            }
+        }
+    }
     if (it == 0) {
         if (threadIdx.x == 0) {
             for (int k = 0; k < 9; ++k) {
@@ -1921,14 +2135,14 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
         __syncthreads();
     } else {
         // issue the partial loads with the state copy (one memory round trip)
-        const float* part = D.part_a[(it - 1) & 1];
+        const GLOBAL_AS float* part = gp(D.part_a[(it - 1) & 1]);
         float pv[kSO3Blocks] = {};
         const int col = threadIdx.x;
         if (col < kSO3) {
 #pragma unroll
             for (int r = 0; r < kSO3Blocks; ++r) pv[r] = part[r * kPartStride + col];
         }
-        copy_words<kSO3Threads>(&S, &st->so3s[(it - 1) & 1]);
+        copy_in<kSO3Threads>(&S, &st->so3s[(it - 1) & 1]);
         __syncthreads();
         if (!S.done) {
             if (col < kSO3) {
@@ -1943,12 +2157,13 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
         }
     }
     if (lb == 0) {
-        copy_words<kSO3Threads>(&st->so3s[it & 1], &S);
+        copy_out<kSO3Threads>(&st->so3s[it & 1], &S);
         if (it == 10 && threadIdx.x == 0) {
-            for (int k = 0; k < 9; ++k) st->so3R[k] = S.resultR[k];
-            st->so3_iters = S.iters;
-            st->lastSO3Error = S.err;
-            st->lastSO3Count = S.cnt;
+            GLOBAL_AS TrackState* g = gp(st);
+            for (int k = 0; k < 9; ++k) g->so3R[k] = S.resultR[k];
+            g->so3_iters = S.iters;
+            g->lastSO3Error = S.err;
+            g->lastSO3Count = S.cnt;
         }
     }
     if (S.done || it == 10) return;
@@ -1969,14 +2184,32 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
         Kir[k] = uni(sKi[k]);
         KRr[k] = uni(sKR[k]);
     }
-    const int w = L.w, h = L.h, n = w * h;
     float acc[kSO3];
 #pragma unroll
     for (int k = 0; k < kSO3; ++k) acc[k] = 0.0f;
-    for (int idx = lb * kSO3Threads + threadIdx.x; idx < n; idx += kSO3Blocks * kSO3Threads) {
+    // the preloaded pixels: all warps, then all next-image loads, then the
+    // rows in pixel order
+    int wx[kSO3Pre], wy[kSO3Pre];
+    bool ok[kSO3Pre];
+#pragma unroll
+    for (int j = 0; j < kSO3Pre; ++j) {
+        const int idx = first + j * stride;
+        const int y = idx / w, x = idx - y * w;
+        ok[j] = idx < n && so3_warp(Hr, w, h, x, y, wx[j], wy[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kSO3Pre; ++j)
+        if (ok[j]) {
+            const int idx = first + j * stride;
+            const int y = idx / w, x = idx - y * w;
+            float row[4];
+            so3_row(next, w, wx[j], wy[j], src[j], x, y, Kir, KRr, row);
+            acc_row4(acc, row);
+        }
+    for (int idx = first + kSO3Pre * stride; idx < n; idx += stride) {
         const int y = idx / w, x = idx - y * w;
         float row[4];
-        if (so3_pixel(D.so3_last, D.so3_next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
+        if (so3_pixel(last, next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
     }
     block_partial<kSO3, kSO3Threads>(acc, D.part_a[it & 1] + (size_t)lb * kPartStride, s_red);
 }
@@ -2007,7 +2240,7 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_step_dbg(const PairDesc* __
     for (int idx = threadIdx.x; idx < w * h; idx += kSO3Threads) {
         const int y = idx / w, x = idx - y * w;
         float row[4];
-        const bool f = so3_pixel(D.so3_last, D.so3_next, w, h, x, y, H, Ki, KR, row);
+        const bool f = so3_pixel(gp(D.so3_last), gp(D.so3_next), w, h, x, y, H, Ki, KR, row);
         if (f) acc_row4(acc, row);
         if (mask) mask[idx] = (uint8_t)f;
     }

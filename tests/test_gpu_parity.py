@@ -331,7 +331,7 @@ def test_step_shapes_agree(pair):
     Rc, tc, Rpi, tp = _constants(pair["T0"], T, 0, cam)
     base, mb = gpu.icp_step(0, Rc, tc, Rpi, tp)
     for t in (256, 512):
-        for q in (1, 2, 4):
+        for q in (1, 2, 3, 4):
             gpu.set_step_shape(0, 0, t, q)
             out, m = gpu.icp_step(0, Rc, tc, Rpi, tp)
             assert np.array_equal(m, mb)
@@ -355,3 +355,46 @@ def test_device_memory_inputs(orc):
     a = host.getIncrementalTransformation(T0[:3, 3], T0[:3, :3])
     b = dev.getIncrementalTransformation(T0[:3, 3], T0[:3, :3])
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_async_device_sequence_equals_synchronous(orc):
+    """Six frames enqueued from device buffers with no host wait between them
+    (raw depth filtered on the device, models at the device-resident pose,
+    graphs on, poses to the device log): the logged poses equal, bit for bit,
+    those of the same frames fed from host buffers one synchronous call at a
+    time."""
+    torch = pytest.importorskip("torch")
+    cam = synth.CAM_640
+    scene = synth.Scene(25)
+    poses = synth.spline_trajectory(7, seed=8)
+    frames = [scene.render(T, cam, noise_seed=300 + i) for i, T in enumerate(poses)]
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    dev = [{k: d(getattr(f, k)) for k in ("depth", "rgb", "rgba", "vert4", "norm4")} for f in frames]
+    torch.cuda.synchronize()
+    sync, asy = mk(cam), mk(cam)
+    asy.set_graph(True)
+    log = torch.zeros((len(frames), 12), dtype=torch.float32, device="cuda")
+    asy.set_pose_log(log.data_ptr(), len(frames))
+    for odo in (sync, asy):
+        odo.set_pose(poses[0][:3, 3], poses[0][:3, :3])
+    sync.initFirstRGB(frames[0].rgb)
+    asy.initFirstRGB(dev[0]["rgb"])
+    expected = []
+    for k in range(1, len(frames)):
+        sync.initICPModelCurrent(frames[k - 1].vert4, frames[k - 1].norm4, CUTOFF)
+        sync.initRGBModel(frames[k - 1].rgba)
+        sync.initICPRaw(frames[k].depth, MAXD, CUTOFF)
+        sync.initRGB(frames[k].rgb)
+        sync.track_async(False, 10.0, True, False, True)
+        expected.append(sync.get_pose())
+        asy.initICPModelCurrent(dev[k - 1]["vert4"], dev[k - 1]["norm4"], CUTOFF)
+        asy.initRGBModel(dev[k - 1]["rgba"])
+        asy.initICPRaw(dev[k]["depth"], MAXD, CUTOFF)
+        asy.initRGB(dev[k]["rgb"])
+        asy.track_async(False, 10.0, True, False, True)
+    asy.synchronize()
+    got = log.cpu().numpy()
+    for k, (t, R) in enumerate(expected):
+        Rt = got[k].reshape(3, 4)
+        assert np.array_equal(Rt[:, 3], np.asarray(t, np.float32).reshape(3)), k
+        assert np.array_equal(Rt[:, :3], np.asarray(R, np.float32).reshape(3, 3)), k
