@@ -403,11 +403,13 @@ def main() -> int:
     odo.synchronize()
     t0 = time.perf_counter()
     run(1, 1 + args.steps)
+    enqueued = time.perf_counter() - t0
     odo.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
-    log(f"timed {args.steps} frames in {elapsed * 1e3:.1f} ms: {fps:.1f} frames/s")
+    log(f"timed {args.steps} frames in {elapsed * 1e3:.1f} ms: {fps:.1f} frames/s "
+        f"(host enqueue {enqueued * 1e3:.1f} ms)")
     est = pose_log.cpu().numpy()[: args.steps]
     stats = odo.stats()
 

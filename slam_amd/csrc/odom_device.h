@@ -89,6 +89,8 @@ struct TrackState {
     float pose_out[12];              // [R | t] row-major result of the call
     double so3R[9];                  // SO3 pre-alignment
     So3Slot so3s[2];
+    So3Slot so3_final;               // the SO3 loop state once it has ended
+    int so3_done;                    // set by the launch that ended the loop
     IterState it[2];
     // results of the call, copied by the end kernel (RGBDOdometryef.h:62-70)
     double lastA[36], lastb[6];

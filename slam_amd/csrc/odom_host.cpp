@@ -340,11 +340,11 @@ void track_launches(odom_ctx* c, const TrackParams& P, hipEvent_t* ev)
         launch_rgb_prep(a, c->stream);
     }
     PairDesc* dp = c->d_pair[c->parity];
-    if (P.so3) launch_so3(dp, 1, c->n[2], c->stream);
+    if (P.so3) launch_so3(dp, 1, &P, c->stream);  // begin rides on the last SO3 launch
+    else launch_track_begin(dp, 1, P, c->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
-    launch_track_begin(dp, 1, P, c->stream);
     int e = 0;
     for (size_t k = 0; k < sched.size(); ++k) {
         const int level = sched[k] / 64, iter = sched[k] % 64;
@@ -1241,13 +1241,11 @@ int batch_slot(odom_batch_s* b, PairDesc** out)
 void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEvent_t* ev)
 {
     const int n = (int)b->h.size();
-    int max_l2 = 0;
-    for (auto* c : b->h) max_l2 = std::max(max_l2, c->n[2]);
-    if (P.so3) launch_so3(dp, n, max_l2, b->stream);
+    if (P.so3) launch_so3(dp, n, &P, b->stream);  // begin rides on the last SO3 launch
+    else launch_track_begin(dp, n, P, b->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
-    launch_track_begin(dp, n, P, b->stream);
     int e = 0;
     for (size_t k = 0; k < sched.size(); ++k) {
         const int level = sched[k] / 64, iter = sched[k] % 64;

@@ -558,10 +558,9 @@ __constant__ float c_gsx[9] = {0.52201f, 0.00000f, -0.52201f, 0.79451f, -0.00000
 __constant__ float c_gsy[9] = {0.52201f, 0.79451f, 0.52201f, 0.00000f, 0.00000f, 0.00000f,
                                -0.52201f, -0.79451f, -0.52201f};
 
-__global__ __launch_bounds__(256) void k_rgb_prep(PrepLevels a, Seg3 g)
+// Pixel k of level l.
+__device__ __forceinline__ void rgb_prep_px(const PrepLevels& a, int l, int k)
 {
-    const int l = seg_level(g, blockIdx.x);
-    const int k = (blockIdx.x - g.b0[l]) * 256 + threadIdx.x;
     const int w = a.w[l], h = a.h[l];
     if (k >= w * h) return;
     const int y = k / w, x = k - y * w;
@@ -616,6 +615,12 @@ __global__ __launch_bounds__(256) void k_rgb_prep(PrepLevels a, Seg3 g)
         }
     }
     a.zsrc[l][k] = z;
+}
+
+__global__ __launch_bounds__(256) void k_rgb_prep(PrepLevels a, Seg3 g)
+{
+    const int l = seg_level(g, blockIdx.x);
+    rgb_prep_px(a, l, (blockIdx.x - g.b0[l]) * 256 + threadIdx.x);
 }
 
 void launch_rgb_prep(const PrepLevels& a, hipStream_t s)
@@ -1799,16 +1804,15 @@ __device__ int seed_keep(const PairDesc& D)
     return (t[2] >= 0.99 * t[0]) && (m1 <= 1.01 * m0);
 }
 
-__global__ __launch_bounds__(64) void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
+// State entering the first iteration of a tracking call of pair D.  so3R:
+// the SO3 result when P.so3 (else unused).
+__device__ void track_begin_pair(const PairDesc& D, const TrackParams& P, const double* so3R)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    GLOBAL_AS TrackState* st = gp(pairs[p].st);
+    GLOBAL_AS TrackState* st = gp(D.st);
     // every load first, then the stores
     float pose[12], errs[4];
-    double so3R[9], lastA[36], lastb[6], seed[16];
+    double lastA[36], lastb[6], seed[16];
     for (int a = 0; a < 12; ++a) pose[a] = st->pose_out[a];
-    for (int a = 0; a < 9; ++a) so3R[a] = P.so3 ? st->so3R[a] : 0.0;
     for (int a = 0; a < 36; ++a) lastA[a] = st->lastA[a];
     for (int a = 0; a < 6; ++a) lastb[a] = st->lastb[a];
     for (int a = 0; a < 16; ++a) seed[a] = P.seeded ? st->seedRt[a] : 0.0;
@@ -1816,7 +1820,7 @@ __global__ __launch_bounds__(64) void k_track_begin(const PairDesc* __restrict__
     errs[1] = st->lastICPCount;
     errs[2] = st->lastRGBError;
     errs[3] = st->lastRGBCount;
-    const int keep = (P.seeded == 2) ? seed_keep(pairs[p]) : 1;
+    const int keep = (P.seeded == 2) ? seed_keep(D) : 1;
 
     float Rprev[9], tprev[3], Rprev_inv[9];
     for (int a = 0; a < 3; ++a) {
@@ -1869,6 +1873,15 @@ __global__ __launch_bounds__(64) void k_track_begin(const PairDesc* __restrict__
     G.degenerate = 0;
     G.iters_done = 0;
     G.rgb_err_cmp = FLT_MAX;
+}
+
+__global__ __launch_bounds__(64) void k_track_begin(const PairDesc* __restrict__ pairs, int npairs, TrackParams P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    double so3R[9];
+    for (int a = 0; a < 9; ++a) so3R[a] = P.so3 ? gp(pairs[p].st)->so3R[a] : 0.0;
+    track_begin_pair(pairs[p], P, so3R);
 }
 
 void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s)
@@ -2093,7 +2106,10 @@ __device__ void so3_evaluate(So3Slot& S, const double* tot)
     }
 }
 
-__global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __restrict__ pairs, int it)
+// With `begin`, block 0 of each pair of launch 10 also sets up the
+// Gauss-Newton loop from the SO3 result (the work of k_track_begin).
+__global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __restrict__ pairs, int it, int begin,
+                                                          TrackParams P)
 {
     __shared__ So3Slot S;
     __shared__ double s_tot[kSO3];
@@ -2102,6 +2118,10 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
     const int p = blockIdx.x / kSO3Blocks, lb = blockIdx.x - p * kSO3Blocks;
     const PairDesc& D = pairs[p];
     TrackState* st = D.st;
+    // once an earlier launch has ended the loop, the launches before the last
+    // have nothing to do (the state is in so3_final)
+    const int ended = (it > 0) ? gp(st)->so3_done : 0;
+    if (ended && it < 10) return;
     const LevelDesc& L = D.lv[2];
     const int w = L.w, h = L.h, n = w * h;
     constexpr int stride = kSO3Blocks * kSO3Threads;
@@ -2133,6 +2153,9 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
             S.done = 0;
         }
         __syncthreads();
+    } else if (ended) {
+        copy_in<kSO3Threads>(&S, &st->so3_final);
+        __syncthreads();
     } else {
         // issue the partial loads with the state copy (one memory round trip)
         const GLOBAL_AS float* part = gp(D.part_a[(it - 1) & 1]);
@@ -2157,13 +2180,19 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
         }
     }
     if (lb == 0) {
-        copy_out<kSO3Threads>(&st->so3s[it & 1], &S);
+        GLOBAL_AS TrackState* g = gp(st);
+        if (it < 10) copy_out<kSO3Threads>(&st->so3s[it & 1], &S);
+        if (it == 0 && threadIdx.x == 0) g->so3_done = 0;
+        if (it > 0 && it < 10 && S.done) {
+            copy_out<kSO3Threads>(&st->so3_final, &S);
+            if (threadIdx.x == 0) g->so3_done = 1;
+        }
         if (it == 10 && threadIdx.x == 0) {
-            GLOBAL_AS TrackState* g = gp(st);
             for (int k = 0; k < 9; ++k) g->so3R[k] = S.resultR[k];
             g->so3_iters = S.iters;
             g->lastSO3Error = S.err;
             g->lastSO3Count = S.cnt;
+            if (begin) track_begin_pair(D, P, S.resultR);
         }
     }
     if (S.done || it == 10) return;
@@ -2214,11 +2243,12 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
     block_partial<kSO3, kSO3Threads>(acc, D.part_a[it & 1] + (size_t)lb * kPartStride, s_red);
 }
 
-void launch_so3(const PairDesc* pairs, int npairs, int max_px_l2, hipStream_t s)
+void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, hipStream_t s)
 {
-    (void)max_px_l2;
+    const TrackParams P = begin ? *begin : TrackParams{};
     for (int it = 0; it <= 10; ++it)
-        hipLaunchKernelGGL(k_so3_iter, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, it);
+        hipLaunchKernelGGL(k_so3_iter, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, it,
+                           (it == 10 && begin) ? 1 : 0, P);
 }
 
 // One SO3 step with explicit constants (H, K^-1, K R as 27 floats): sums in
