@@ -190,15 +190,15 @@ int resolve(odom_ctx* c, const odom_frame_view* v, int fmt_a, int fmt_b, void* s
 
 int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
 {
-    Pyr12 py{};
+    PyrJob py{};
     py.count = 1;
-    py.kind[0] = 2;
-    py.src[0] = c->depth[0];
-    py.dst1[0] = c->depth[1];
-    py.dst2[0] = c->depth[2];
+    py.p[0].kind = 2;
+    py.p[0].src0 = c->depth[0];
+    py.p[0].dst[1] = c->depth[1];
+    py.p[0].dst[2] = c->depth[2];
     py.w0 = c->w[0];
     py.h0 = c->h[0];
-    launch_pyr12(py, c->stream);
+    launch_pyramid(py, c->stream);
     VNLevels a{};
     for (int l = 0; l < kLevels; ++l) {
         a.depth[l] = c->depth[l];
@@ -257,25 +257,25 @@ int enqueue_populate(odom_ctx* c, const odom_frame_view* rgb, float** dstDepth, 
     int pitch, fmt;
     int rc = resolve(c, rgb, ODOM_FMT_RGB8, ODOM_FMT_RGBA8, c->stage_rgb, &p, &pitch, &fmt);
     if (rc) return rc;
-    launch_populate_l0(c->vz, dstDepth ? dstDepth[0] : nullptr, 6.0f, (const uint8_t*)p, pitch,
-                       fmt == ODOM_FMT_RGB8 ? 3 : 4, dstImage[0], c->w[0], c->h[0], c->stream);
-    Pyr12 py{};
+    PyrJob py{};
     py.w0 = c->w[0];
     py.h0 = c->h[0];
-    py.count = 0;
     if (dstDepth) {
-        py.kind[py.count] = 0;
-        py.src[py.count] = dstDepth[0];
-        py.dst1[py.count] = dstDepth[1];
-        py.dst2[py.count] = dstDepth[2];
-        py.count++;
+        PyrSrc& q = py.p[py.count++];
+        q.kind = 0;
+        q.gen = 1;
+        q.vz = c->vz;
+        q.cutoff = 6.0f;
+        for (int l = 0; l < kLevels; ++l) q.dst[l] = dstDepth[l];
     }
-    py.kind[py.count] = 1;
-    py.src[py.count] = dstImage[0];
-    py.dst1[py.count] = dstImage[1];
-    py.dst2[py.count] = dstImage[2];
-    py.count++;
-    launch_pyr12(py, c->stream);
+    PyrSrc& q = py.p[py.count++];
+    q.kind = 1;
+    q.gen = 2;
+    q.rgb = (const uint8_t*)p;
+    q.pitch = pitch;
+    q.ch = fmt == ODOM_FMT_RGB8 ? 3 : 4;
+    for (int l = 0; l < kLevels; ++l) q.dst[l] = dstImage[l];
+    launch_pyramid(py, c->stream);
     return check_launch(c);
 }
 

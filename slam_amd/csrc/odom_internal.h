@@ -47,21 +47,26 @@ struct ModelLevels {
     const float* pose_dev;          // if set: [R|t] (3x4 row-major) read on the device
 };
 void launch_model_maps(const ModelLevels& a, hipStream_t s);
-// Levels 1 and 2 of up to two pyramids in one launch.  kind: 0 = float
-// Gaussian (a9), 1 = uint8 Gaussian (a11), 2 = uint16 bilateral-gated (a2).
-struct Pyr12 {
-    int kind[2];
-    const void* src[2];
-    void* dst1[2];
-    void* dst2[2];
-    int w0, h0;
-    int count;
+// Levels 0 (when generated), 1 and 2 of up to two pyramids in one launch.
+// kind: 0 = float Gaussian (a9), 1 = uint8 Gaussian (a11), 2 = uint16
+// bilateral-gated (a2).  gen: 0 = level 0 given (src0); 1 = level-0 depth
+// from the vertex z plane vz (verticesToDepth, cutoff); 2 = level-0
+// intensity from colour (rgb, pitch, ch).  dst[0] is written when gen != 0.
+struct PyrSrc {
+    int kind, gen;
+    const void* src0;
+    const float* vz;
+    float cutoff;
+    const uint8_t* rgb;
+    int pitch, ch;
+    void* dst[3];
 };
-void launch_pyr12(const Pyr12& a, hipStream_t s);
-// level 0 of populateRGBDData: depth from the vertex z plane (skipped when
-// depth0 is null) and intensity
-void launch_populate_l0(const float* vz, float* depth0, float cutoff, const uint8_t* rgb, int pitch, int ch,
-                        uint8_t* img0, int w, int h, hipStream_t s);
+struct PyrJob {
+    PyrSrc p[2];
+    int count;
+    int w0, h0;
+};
+void launch_pyramid(const PyrJob& j, hipStream_t s);
 // derivatives (a12) + static photometric source mask, all levels of all pairs
 struct PrepLevels {
     const uint8_t* img[kLevels];

@@ -446,104 +446,144 @@ __device__ __forceinline__ uint16_t pyr_u16_at(Get get, int sw, int sh, int x, i
     return (uint16_t)(int)(sum / wall);
 }
 
-// Segments of a one-dimensional grid: blocks [b0[s], b0[s+1]) run segment s.
-struct SegN {
-    int b0[9];
-    int n;
+// Levels 0 (when generated), 1 and 2 of up to two pyramids in one launch,
+// from LDS tiles.  A level-1 block stages the level-0 texels of its 16x16
+// outputs' windows (36x36); a level-2 block stages 43x43 level-0 texels,
+// computes the 20x20 level-1 values its 8x8 outputs read (the same
+// computation the level-1 blocks store), then its outputs.  Level 0 is read
+// from memory or generated (verticesToDepth, or the intensity conversion of
+// populateRGBDData, RGBDOdometryef.cpp:212, 225), in which case level-0
+// blocks also store it.  The taps, their clamped coordinates and the
+// summation order are those of the per-level functions above.
+constexpr int kPT = 16;                 // level-1 output tile edge
+constexpr int kT1 = 2 * kPT + 4;        // its level-0 texels
+constexpr int kPT2 = 8;                 // level-2 output tile edge
+constexpr int kT21 = 2 * kPT2 + 4;      // its level-1 values
+constexpr int kT20 = 2 * kT21 + 3;      // its level-0 texels
+
+struct PyrGrid {
+    PyrJob j;
+    int nseg;
+    int b0[7];
+    int seg_q[6], seg_t[6];             // pyramid, and level (0 generation, 1, 2) of a segment
+    int tiles_x[3];                     // tiles per row at levels 1, 2 (index by level)
 };
 
-__device__ __forceinline__ int seg_index(const SegN& g, int b)
+template <class T>
+__device__ __forceinline__ T pyr_l0(const PyrSrc& q, int w0, int x, int y)
 {
-    int s = 0;
-    while (s + 1 < g.n && b >= g.b0[s + 1]) ++s;
-    return s;
-}
-
-static SegN make_segn(const int* n, int count, int per_block)
-{
-    SegN g;
-    g.n = count;
-    g.b0[0] = 0;
-    for (int s = 0; s < count; ++s) g.b0[s + 1] = g.b0[s] + cdiv(n[s], per_block);
-    return g;
-}
-
-__global__ __launch_bounds__(256) void k_pyr_level(Pyr12 a, SegN g)
-{
-    const int pyr = seg_index(g, blockIdx.x);
-    const int i = (blockIdx.x - g.b0[pyr]) * 256 + threadIdx.x;
-    const int sw = a.w0, sh = a.h0, dw = sw / 2, dh = sh / 2;
-    if (i >= dw * dh) return;
-    const int y = i / dw, x = i - y * dw;
-    const int kind = a.kind[pyr];
-    if (kind == 0) {
-        const float* s0 = (const float*)a.src[pyr];
-        auto g0 = [&](int cx, int cy) { return s0[(size_t)cy * sw + cx]; };
-        ((float*)a.dst1[pyr])[i] = gauss_f32_at(g0, sw, sh, x, y);
-    } else if (kind == 1) {
-        const uint8_t* s0 = (const uint8_t*)a.src[pyr];
-        auto g0 = [&](int cx, int cy) { return s0[(size_t)cy * sw + cx]; };
-        ((uint8_t*)a.dst1[pyr])[i] = gauss_u8_at(g0, sw, sh, x, y);
-    } else {
-        const uint16_t* s0 = (const uint16_t*)a.src[pyr];
-        auto g0 = [&](int cx, int cy) { return (int)s0[(size_t)cy * sw + cx]; };
-        ((uint16_t*)a.dst1[pyr])[i] = pyr_u16_at(g0, sw, sh, x, y);
-    }
-}
-
-void launch_pyr12(const Pyr12& a, hipStream_t s)
-{
-    // level 1 from level 0, then level 2 from level 1, each for every pyramid
-    for (int lvl = 1; lvl <= 2; ++lvl) {
-        Pyr12 b = a;
-        b.w0 = a.w0 >> (lvl - 1);
-        b.h0 = a.h0 >> (lvl - 1);
-        for (int p = 0; p < a.count; ++p) {
-            b.src[p] = lvl == 1 ? a.src[p] : a.dst1[p];
-            b.dst1[p] = lvl == 1 ? a.dst1[p] : a.dst2[p];
+    const size_t i = (size_t)y * w0 + x;
+    if constexpr (sizeof(T) == 4) {
+        if (q.gen == 1) {
+            const float z = gp(q.vz)[i];
+            return (z > q.cutoff || z <= 0) ? QNAN : z;
         }
-        int n[2];
-        for (int p = 0; p < a.count; ++p) n[p] = (b.w0 / 2) * (b.h0 / 2);
-        const SegN g = make_segn(n, a.count, 256);
-        hipLaunchKernelGGL(k_pyr_level, dim3(g.b0[g.n]), dim3(256), 0, s, b, g);
+    } else if constexpr (sizeof(T) == 1) {
+        if (q.gen == 2) {
+            const GLOBAL_AS uint8_t* px = gp(q.rgb) + (size_t)y * q.pitch + (size_t)x * q.ch;
+            return (uint8_t)(int)((float)px[0] * 0.114f + (float)px[1] * 0.299f + (float)px[2] * 0.587f);
+        }
+    }
+    return gp((const T*)q.src0)[i];
+}
+
+// One output of level `lvl` from its source level `src` (sw x sh texels)
+// held in an LDS tile with origin (ox, oy) and row length `ld`.
+template <int KIND, class T>
+__device__ __forceinline__ T pyr_at(const T* tile, int ld, int ox, int oy, int sw, int sh, int x, int y)
+{
+    auto get = [&](int cx, int cy) { return tile[(cy - oy) * ld + (cx - ox)]; };
+    if constexpr (KIND == 0) {
+        return gauss_f32_at(get, sw, sh, x, y);
+    } else if constexpr (KIND == 1) {
+        return gauss_u8_at(get, sw, sh, x, y);
+    } else {
+        auto geti = [&](int cx, int cy) { return (int)get(cx, cy); };
+        return pyr_u16_at(geti, sw, sh, x, y);
     }
 }
 
-// Level 0 of populateRGBDData (RGBDOdometryef.cpp:212, 225): verticesToDepth
-// (utils.cu:526-537) and the intensity conversion (utils.cu:550-563) in one launch.
-__global__ __launch_bounds__(256) void k_populate_l0(const float* __restrict__ vz, float* __restrict__ depth0,
-                                                     float cutoff, const uint8_t* __restrict__ rgb, int pitch,
-                                                     int ch, uint8_t* __restrict__ img0, int w, int h, SegN g)
+template <int KIND, class T>
+__device__ void pyr_block(const PyrGrid& g, int q, int t, int lb, unsigned char* smem)
 {
-    const int sgi = seg_index(g, blockIdx.x);
-    const int i = (blockIdx.x - g.b0[sgi]) * 256 + threadIdx.x;
-    if (i >= w * h) return;
-    if (sgi == 0 && depth0) {
-        const float z = vz[i];
-        depth0[i] = (z > cutoff || z <= 0) ? QNAN : z;
-    } else {
-        const int y = i / w, x = i - y * w;
-        const uint8_t* p = rgb + (size_t)y * pitch + (size_t)x * ch;
-        const int value = (int)((float)p[0] * 0.114f + (float)p[1] * 0.299f + (float)p[2] * 0.587f);
-        img0[i] = (uint8_t)value;
+    const PyrSrc& P = g.j.p[q];
+    const int w0 = g.j.w0, h0 = g.j.h0, w1 = w0 / 2, h1 = h0 / 2, w2 = w1 / 2, h2 = h1 / 2;
+    const int tid = threadIdx.x;
+    if (t == 0) {
+        const int i = lb * 256 + tid;
+        if (i < w0 * h0) {
+            const int y = i / w0, x = i - y * w0;
+            gp((T*)P.dst[0])[i] = pyr_l0<T>(P, w0, x, y);
+        }
+        return;
     }
+    const int tx = lb % g.tiles_x[t], ty = lb / g.tiles_x[t];
+    T* t1 = (T*)smem;                            // texels one level below the outputs
+    if (t == 1) {
+        const int X0 = tx * kPT, Y0 = ty * kPT;
+        const int x = X0 + (tid & (kPT - 1)), y = Y0 + tid / kPT;
+        const int ox = 2 * X0 - 2, oy = 2 * Y0 - 2;
+        for (int i = tid; i < kT1 * kT1; i += 256) {
+            const int gx = ox + i % kT1, gy = oy + i / kT1;
+            if (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) t1[i] = pyr_l0<T>(P, w0, gx, gy);
+        }
+        __syncthreads();
+        if (x < w1 && y < h1) gp((T*)P.dst[1])[(size_t)y * w1 + x] = pyr_at<KIND>(t1, kT1, ox, oy, w0, h0, x, y);
+        return;
+    }
+    const int X0 = tx * kPT2, Y0 = ty * kPT2;
+    T* t0 = t1 + kT21 * kT21;                    // level-0 texels
+    const int o1x = 2 * X0 - 2, o1y = 2 * Y0 - 2;
+    const int o0x = 2 * o1x - 2, o0y = 2 * o1y - 2;
+    for (int i = tid; i < kT20 * kT20; i += 256) {
+        const int gx = o0x + i % kT20, gy = o0y + i / kT20;
+        if (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) t0[i] = pyr_l0<T>(P, w0, gx, gy);
+    }
+    __syncthreads();
+    for (int i = tid; i < kT21 * kT21; i += 256) {
+        const int x1 = o1x + i % kT21, y1 = o1y + i / kT21;
+        if (x1 >= 0 && x1 < w1 && y1 >= 0 && y1 < h1) t1[i] = pyr_at<KIND>(t0, kT20, o0x, o0y, w0, h0, x1, y1);
+    }
+    __syncthreads();
+    const int x = X0 + (tid & (kPT2 - 1)), y = Y0 + tid / kPT2;
+    if (tid < kPT2 * kPT2 && x < w2 && y < h2)
+        gp((T*)P.dst[2])[(size_t)y * w2 + x] = pyr_at<KIND>(t1, kT21, o1x, o1y, w1, h1, x, y);
 }
 
-void launch_populate_l0(const float* vz, float* depth0, float cutoff, const uint8_t* rgb, int pitch, int ch,
-                        uint8_t* img0, int w, int h, hipStream_t s)
+__global__ __launch_bounds__(256) void k_pyramid(PyrGrid g)
 {
-    const int n[2] = {w * h, w * h};
-    SegN g;
-    if (depth0) {
-        g = make_segn(n, 2, 256);
-    } else {
-        // intensity only: a single segment indexed as segment 1
-        g.n = 2;
-        g.b0[0] = 0;
-        g.b0[1] = 0;
-        g.b0[2] = cdiv(w * h, 256);
-    }
-    hipLaunchKernelGGL(k_populate_l0, dim3(g.b0[2]), dim3(256), 0, s, vz, depth0, cutoff, rgb, pitch, ch, img0, w, h, g);
+    __shared__ __align__(16) unsigned char smem[(kT1 * kT1 > kT21 * kT21 + kT20 * kT20 ? kT1 * kT1 : kT21 * kT21 + kT20 * kT20) * 4];
+    int sgi = 0;
+    while (sgi + 1 < g.nseg && (int)blockIdx.x >= g.b0[sgi + 1]) ++sgi;
+    const int q = g.seg_q[sgi], t = g.seg_t[sgi], lb = blockIdx.x - g.b0[sgi];
+    const int kind = g.j.p[q].kind;
+    if (kind == 0) pyr_block<0, float>(g, q, t, lb, smem);
+    else if (kind == 1) pyr_block<1, uint8_t>(g, q, t, lb, smem);
+    else pyr_block<2, uint16_t>(g, q, t, lb, smem);
+}
+
+void launch_pyramid(const PyrJob& j, hipStream_t s)
+{
+    PyrGrid g{};
+    g.j = j;
+    const int w1 = j.w0 / 2, h1 = j.h0 / 2, w2 = w1 / 2, h2 = h1 / 2;
+    g.tiles_x[1] = cdiv(w1, kPT);
+    g.tiles_x[2] = cdiv(w2, kPT2);
+    const int n2 = g.tiles_x[2] * cdiv(h2, kPT2), n1 = g.tiles_x[1] * cdiv(h1, kPT);
+    int nb = 0;
+    // level-2 tiles first (the longest blocks), then level 1, then level-0 generation
+    for (int t = 2; t >= 0; --t)
+        for (int q = 0; q < j.count; ++q) {
+            const int n = t == 2 ? n2 : (t == 1 ? n1 : (j.p[q].gen ? cdiv(j.w0 * j.h0, 256) : 0));
+            if (!n) continue;
+            g.b0[g.nseg] = nb;
+            g.seg_q[g.nseg] = q;
+            g.seg_t[g.nseg] = t;
+            g.nseg++;
+            nb += n;
+        }
+    g.b0[g.nseg] = nb;
+    hipLaunchKernelGGL(k_pyramid, dim3(nb), dim3(256), 0, s, g);
 }
 
 // ================================ a12 derivatives + photometric source mask
