@@ -1041,6 +1041,37 @@ __device__ void iter_constants(IterState& S, const LevelDesc& L)
 // exchanges: the factorisation is a chain of dependent FP64 operations that
 // cross-lane shuffles would only lengthen); lane 0 stores the results.
 // rgb_sums: FP64 totals of the photometric pass (unused without it).
+// The system of a solve (lastA, lastb: the entries solve_update_uniform
+// assembles, one per lane) and the ICP statistics of the iteration
+// (residual[] of the ICP step and its errors, RGBDOdometryef.cpp:509-510), by
+// one wave while another solves.
+__device__ void store_system_stats(IterState& S, const TrackParams& P, const double* icp_sums, const double* rgb_sums,
+                                   int lane)
+{
+    if (lane < 42) {
+        const int i = lane < 36 ? lane / 6 : lane - 36, j = lane < 36 ? lane % 6 : 6;
+        const int r = i < j ? i : j, c = i < j ? j : i;
+        const int s = 7 * r - r * (r - 1) / 2 + (c - r);   // packed upper triangle of [A | b]
+        const double ai = P.icp ? icp_sums[s] : 0.0;
+        const double ar = P.rgb ? rgb_sums[s] : 0.0;
+        const double wgt = P.icpWeight;
+        double v;
+        if (P.icp && P.rgb) v = (c == 6) ? ar + wgt * ai : ar + wgt * wgt * ai;
+        else v = P.icp ? ai : ar;
+        if (lane < 36) S.lastA[lane] = v;
+        else S.lastb[lane - 36] = v;
+    }
+    if (P.icp && lane < kSE3) S.icp_sums[lane] = icp_sums[lane];
+    if (lane == 0) {
+        if (P.icp) {
+            S.res0 = (float)icp_sums[27];
+            S.res1 = (float)icp_sums[28];
+        }
+        S.lastICPError = sqrtf(S.res0) / S.res1;
+        S.lastICPCount = S.res1;
+    }
+}
+
 __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const double* icp_sums,
                                      const double* rgb_sums, const float* Rprev, const float* tprev)
 {
@@ -1064,12 +1095,6 @@ __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const d
                 A[j * 6 + i] = v;
             }
         }
-    if (lead) {
-#pragma unroll
-        for (int q = 0; q < 36; ++q) S.lastA[q] = A[q];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) S.lastb[q] = b[q];
-    }
     int zeros = 0;
     if (!ldlt_spd<6>(A, b, x)) zeros = ldlt_solve<6>(A, b, x);
     double rot[9];
@@ -1143,18 +1168,6 @@ __device__ void assoc_rgb(IterState& S, const TrackParams& P, float cmp, int cou
     if (P.rgbOnly) sigmaVal = -1;
     S.sigmaVal = sigmaVal;
     S.pending = 1;
-}
-
-// RGBDOdometryef.cpp:509-510: residual[] of the ICP step and its errors.
-__device__ void icp_stats(IterState& S, const TrackParams& P, const double* icp_sums)
-{
-    if (P.icp) {
-        for (int q = 0; q < kSE3; ++q) S.icp_sums[q] = icp_sums[q];
-        S.res0 = (float)icp_sums[27];
-        S.res1 = (float)icp_sums[28];
-    }
-    S.lastICPError = sqrtf(S.res0) / S.res1;
-    S.lastICPCount = S.res1;
 }
 
 // Load of a state record from global memory into LDS, word per thread.
@@ -1321,17 +1334,19 @@ __device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, 
     partials_issue<NT>(P.icp ? D.part_a[pb] : nullptr, na, P.rgb ? D.part_b[pb] : nullptr, nb, PL);
     __syncthreads();
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
+    // wave 0 solves; wave 1 meanwhile stores the system, the ICP statistics
+    // and the photometric bookkeeping (disjoint fields of the state)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (run) {
         partials_sum<NT>(PL, na, nb, W.tot, W.red, W.fin2);
-        if (threadIdx.x < 64) {
-            if (threadIdx.x == 0) {
-                if (!P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
-                icp_stats(W.S, P, W.tot);
-            }
+        if (wave == 0) {
             solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t);
+        } else if (wave == 1) {
+            store_system_stats(W.S, P, W.tot, W.tot + kSE3, lane);
+            if (lane == 0 && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
         }
     }
-    if (threadIdx.x == 0) {
+    if (wave == 1 && lane == 0) {
         if (W.S.broke) W.S.break_level = prev_level;
         W.S.pending = 0;
         W.S.broke = 0;

@@ -189,6 +189,22 @@ ODOM_HD int ldlt_solve(const double* Ain, const double* b, double* x)
 // operations, no row swaps).  Returns false, leaving x unspecified, when a
 // pivot is not above 1e-9 of the largest diagonal entry: the caller then uses
 // ldlt_solve, whose pivoting and zero-pivot rule decide singular systems.
+// 1/d for the positive pivots of ldlt_spd.  On the device: v_rcp_f64 and two
+// Newton steps (each at least doubles the correct bits, so the result is
+// within an ulp of the quotient) instead of the full division sequence.
+ODOM_HD inline double recip_pos(double d)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / d;
+#endif
+}
+
 template <int N>
 ODOM_HD bool ldlt_spd(const double* A, const double* b, double* x)
 {
@@ -205,7 +221,7 @@ ODOM_HD bool ldlt_spd(const double* A, const double* b, double* x)
         for (int k = 0; k < j; ++k) dj = dj - L[j][k] * L[j][k] * d[k];
         ok = ok && (dj > tol);
         d[j] = dj;
-        id[j] = 1.0 / dj;
+        id[j] = recip_pos(dj);
 #pragma unroll
         for (int i = j + 1; i < N; ++i) {
             double v = A[i * N + j];
@@ -239,8 +255,19 @@ ODOM_HD void rodrigues(const double* src, double* R)
     const double theta = sqrt(rx * rx + ry * ry + rz * rz);
     if (theta >= DBL_EPSILON) {
 #ifdef __HIP_DEVICE_COMPILE__
+        // the increments of a Gauss-Newton step are small: below 0.25 rad the
+        // Taylor series to theta^13 (sin) and theta^14 (cos) are exact to
+        // double precision (next terms below 1e-21); above, the library
         double s, c;
-        sincos(theta, &s, &c);
+        if (theta <= 0.25) {
+            const double t2 = theta * theta;
+            s = theta * (1.0 + t2 * (-1.0 / 6 + t2 * (1.0 / 120 + t2 * (-1.0 / 5040 + t2 * (1.0 / 362880 +
+                t2 * (-1.0 / 39916800 + t2 * (1.0 / 6227020800.0)))))));
+            c = 1.0 + t2 * (-0.5 + t2 * (1.0 / 24 + t2 * (-1.0 / 720 + t2 * (1.0 / 40320 + t2 * (-1.0 / 3628800 +
+                t2 * (1.0 / 479001600.0 + t2 * (-1.0 / 87178291200.0)))))));
+        } else {
+            sincos(theta, &s, &c);
+        }
 #else
         const double c = cos(theta), s = sin(theta);
 #endif
