@@ -823,27 +823,6 @@ __device__ __forceinline__ void partials_total2(const float* __restrict__ pa, in
     __syncthreads();
 }
 
-// Wrapping uint32 totals of one (c0, c1) pair per thread, already loaded;
-// every thread sees tot[2].
-template <int NT>
-__device__ __forceinline__ void int_total_reg(unsigned int c0, unsigned int c1, unsigned int* tot,
-                                              unsigned int (*red)[2])
-{
-    c0 = wave_total_u(c0);
-    c1 = wave_total_u(c1);
-    if ((threadIdx.x & 63) == 63) {
-        red[threadIdx.x >> 6][0] = c0;
-        red[threadIdx.x >> 6][1] = c1;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        unsigned int s = 0;
-        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][threadIdx.x];
-        tot[threadIdx.x] = s;
-    }
-    __syncthreads();
-}
-
 // Wrapping uint32 totals of the integer partials; every thread sees tot[2].
 template <int NT>
 __device__ __forceinline__ void int_total(const unsigned int* __restrict__ part, int nblk, unsigned int* tot,
@@ -1144,14 +1123,21 @@ __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const d
 // RGBDOdometryef.cpp:434-471: photometric bookkeeping after the association
 // pass of an iteration.  cmp: the lastRGBError this iteration compares to.
 // Sets S.broke (rgbOnly early exit) or S.pending (a solve is due).
+// sigmaVal and the error of the photometric association from its count and
+// sum of squared differences (with the precedence of RGBDOdometryef.cpp:457).
+__device__ __forceinline__ void rgb_sigma_error(int count, int sigma, float& sigmaVal, float& rgbError)
+{
+    sigmaVal = (float)sqrt((double)((((float)sigma / (float)count) == 0.0f) ? 1 : count));
+    rgbError = (float)(sqrt((double)sigma) / (double)(count == 0 ? 1 : count));
+}
+
 __device__ void assoc_rgb(IterState& S, const TrackParams& P, float cmp, int count, int sigma)
 {
     float sigmaVal, rgbError;
     if (P.rgb) {
         S.rgb_count = count;
         S.rgb_sigma = sigma;
-        sigmaVal = (float)sqrt((double)((((float)sigma / (float)count) == 0.0f) ? 1 : count));
-        rgbError = (float)(sqrt((double)sigma) / (double)(count == 0 ? 1 : count));
+        rgb_sigma_error(count, sigma, sigmaVal, rgbError);
     } else {
         sigmaVal = 0.0f;
         rgbError = 0.0f;
@@ -1245,13 +1231,17 @@ __device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int
 
 // FP64 totals of the loaded partials in a fixed order: per thread over its
 // records, then per column over the record groups in LDS (each wave a slice
-// of the groups, then the waves' sums).  tot[0..28] pass A, tot[29..57]
-// pass B.
+// of the groups: the slice of wave w holds exactly the groups its own lanes
+// stored, so this step needs no block barrier), then the waves' sums.  Those
+// are formed by wave 0 into tot (for the solve) and by wave 1 into tot1 (for
+// the system and statistics it stores), so one block barrier suffices.
+// tot[0..28] pass A, tot[29..57] pass B.
 template <int NT>
-__device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int nb, double* tot,
+__device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int nb, double* tot, double* tot1,
                                              double (*red)[64], double (*fin)[64])
 {
     constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT, G = NT / 64, SL = RG / G;
+    static_assert(SL * 8 == 64, "a wave's lanes store exactly its slice of record groups");
     const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -1270,25 +1260,25 @@ __device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int 
         red[rg][a * 32 + 4 * c + 2] = s2;
         red[rg][a * 32 + 4 * c + 3] = s3;
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    const int col = threadIdx.x & 63, w = threadIdx.x >> 6;
     {
-        const int col = threadIdx.x & 63, w = threadIdx.x >> 6;
         double t = 0.0;
 #pragma unroll
         for (int g = 0; g < SL; ++g) t += red[w * SL + g][col];
         fin[w][col] = t;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int a = threadIdx.x >> 5, k = threadIdx.x & 31;
+    if (w < 2) {
+        const int a = col >> 5, k = col & 31;
         if (k < kSE3) {
             double t = 0.0;
 #pragma unroll
-            for (int w = 0; w < G; ++w) t += fin[w][threadIdx.x];
-            tot[a * kSE3 + k] = t;
+            for (int q = 0; q < G; ++q) t += fin[q][col];
+            (w == 0 ? tot : tot1)[a * kSE3 + k] = t;
         }
+        __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
 }
 
 template <int NT>
@@ -1298,6 +1288,7 @@ struct Prolog {
     double red[NT / 8][64];
     double fin2[NT / 64][64];
     double tot[2 * kSE3];
+    double tot1[2 * kSE3];
     unsigned int redi[NT / 64][2];
     unsigned int toti[2];
 };
@@ -1338,11 +1329,11 @@ __device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, 
     // and the photometric bookkeeping (disjoint fields of the state)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (run) {
-        partials_sum<NT>(PL, na, nb, W.tot, W.red, W.fin2);
+        partials_sum<NT>(PL, na, nb, W.tot, W.tot1, W.red, W.fin2);
         if (wave == 0) {
             solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t);
         } else if (wave == 1) {
-            store_system_stats(W.S, P, W.tot, W.tot + kSE3, lane);
+            store_system_stats(W.S, P, W.tot1, W.tot1 + kSE3, lane);
             if (lane == 0 && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
         }
     }
@@ -1596,26 +1587,39 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
 
     float sigma;
     if (mode == 0) {
-        int_total_reg<NT>(c0, c1, W.toti, W.redi);
-        if (threadIdx.x == 0) {
-            IterState& S = W.S;
-            assoc_rgb(S, P, err_cmp, (int)W.toti[0], (int)W.toti[1]);
-            if (lb == 0) {
-                IterState* G = &st->it[pb];
-                G->rgb_count = S.rgb_count;
-                G->rgb_sigma = S.rgb_sigma;
-                if (!S.broke) {
-                    G->lastRGBError = S.lastRGBError;
-                    G->lastRGBCount = S.lastRGBCount;
-                    G->sigmaVal = S.sigmaVal;
-                }
-                G->pending = S.pending;
-                G->broke = S.broke;
-            }
+        // the waves' count totals, then every thread sums them and takes the
+        // association decision of assoc_rgb itself (one block barrier)
+        c0 = wave_total_u(c0);
+        c1 = wave_total_u(c1);
+        if ((threadIdx.x & 63) == 63) {
+            W.redi[threadIdx.x >> 6][0] = c0;
+            W.redi[threadIdx.x >> 6][1] = c1;
         }
         __syncthreads();
-        if (W.S.broke) return;
-        sigma = W.S.sigmaVal;
+        unsigned int count = 0, sum2 = 0;
+#pragma unroll
+        for (int wv = 0; wv < NT / 64; ++wv) {
+            count += W.redi[wv][0];
+            sum2 += W.redi[wv][1];
+        }
+        float sv, err;
+        rgb_sigma_error((int)count, (int)sum2, sv, err);
+        const bool broke = P.rgbOnly && err > err_cmp;
+        if (P.rgbOnly) sv = -1;
+        if (lb == 0 && threadIdx.x == 0) {
+            GLOBAL_AS IterState* G = gp(&st->it[pb]);
+            G->rgb_count = (int)count;
+            G->rgb_sigma = (int)sum2;
+            if (!broke) {
+                G->lastRGBError = err;
+                G->lastRGBCount = (float)(int)count;
+                G->sigmaVal = sv;
+            }
+            G->pending = broke ? 0 : 1;
+            G->broke = broke ? 1 : 0;
+        }
+        if (broke) return;
+        sigma = sv;
     } else {
         sigma = sigma_g;
     }
