@@ -14,6 +14,7 @@
 // the file is compiled with -ffp-contract=off, so masks are bit-identical to
 // the CPU restatement of the same algorithm.
 #include <float.h>
+#include <type_traits>
 
 #include "odom_internal.h"
 #include "odom_math.h"
@@ -277,47 +278,12 @@ __device__ __forceinline__ Cam6 model_l1(const ModelLevels& a, int x, int y)
     return mean_cam(c00, c01, c10, c11);
 }
 
-// Level 2 from the 4x4 level-0 texels: the level-1 means of its 2x2
-// quadrants, then their mean.
-__device__ __forceinline__ Cam6 model_l2(const ModelLevels& a, int x, int y)
-{
-    Cam6 t[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) t[r][q] = model_l0(a, 4 * x + q, 4 * y + r);
-    Cam6 m[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            m[i][j] = mean_cam(t[2 * i][2 * j], t[2 * i][2 * j + 1], t[2 * i + 1][2 * j], t[2 * i + 1][2 * j + 1]);
-    return mean_cam(m[0][0], m[0][1], m[1][0], m[1][1]);
-}
 
-__global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, Seg3 g)
+// Transform (when set) and store output i of level l.
+__device__ __forceinline__ void model_store(const ModelLevels& a, int l, size_t i, Cam6 c, const float* R,
+                                            const float* t)
 {
-    // segments in the order level 2, 1, 0: the blocks with the most loads
-    // per thread (a level-2 output reads 16 level-0 texels) start first
-    const int seg = seg_level(g, blockIdx.x);
-    const int l = 2 - seg;
-    const int i = (blockIdx.x - g.b0[seg]) * 256 + threadIdx.x;
-    const int w = a.w[l], h = a.h[l];
-    if (i >= w * h) return;
-    const int y = i / w, x = i - y * w;
-    Cam6 c = (l == 0) ? model_l0(a, x, y) : (l == 1 ? model_l1(a, x, y) : model_l2(a, x, y));
-    if (l == 0 && a.vz) a.vz[i] = a.v4[(size_t)y * a.v_pitch + x].z;
     if (a.transform) {
-        float R[9], t[3];
-        if (a.pose_dev) {
-            for (int r = 0; r < 3; ++r) {
-                for (int k = 0; k < 3; ++k) R[3 * r + k] = a.pose_dev[4 * r + k];
-                t[r] = a.pose_dev[4 * r + 3];
-            }
-        } else {
-            for (int k = 0; k < 9; ++k) R[k] = a.R[k];
-            for (int k = 0; k < 3; ++k) t[k] = a.t[k];
-        }
         if (!isnan(c.v[0])) {
             const float v0 = c.v[0], v1 = c.v[1], v2 = c.v[2];
             for (int r = 0; r < 3; ++r) c.v[r] = R[3 * r] * v0 + R[3 * r + 1] * v1 + R[3 * r + 2] * v2 + t[r];
@@ -331,19 +297,75 @@ __global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, Seg3 g)
             c.n[0] = c.n[1] = c.n[2] = QNAN;
         }
     }
-    const size_t P = (size_t)w * h;
-    float* vm = a.vmap[l];
-    float* nm = a.nmap[l];
+    const size_t P = (size_t)a.w[l] * a.h[l];
+    GLOBAL_AS float* vm = gp(a.vmap[l]);
+    GLOBAL_AS float* nm = gp(a.nmap[l]);
     vm[i] = c.v[0]; vm[i + P] = c.v[1]; vm[i + 2 * P] = c.v[2];
     nm[i] = c.n[0]; nm[i + P] = c.n[1]; nm[i + 2 * P] = c.n[2];
 }
 
+// Blocks [0, nb1): a 16x16 tile of level 1 each, whose untransformed values
+// also give, from LDS, the 8x8 level-2 outputs the tile covers (2x2 means of
+// level-1 values, as level 1 is of level 0); then 256 level-0 pixels per block.
+__global__ __launch_bounds__(256) void k_model_maps(ModelLevels a, int nb1, int tiles_x1)
+{
+    __shared__ float s1[6][16][16];
+    float R[9], t[3];
+    if (a.transform) {
+        if (a.pose_dev) {
+            const GLOBAL_AS float* pd = gp(a.pose_dev);
+            for (int r = 0; r < 3; ++r) {
+                for (int k = 0; k < 3; ++k) R[3 * r + k] = pd[4 * r + k];
+                t[r] = pd[4 * r + 3];
+            }
+        } else {
+            for (int k = 0; k < 9; ++k) R[k] = a.R[k];
+            for (int k = 0; k < 3; ++k) t[k] = a.t[k];
+        }
+    }
+    if ((int)blockIdx.x < nb1) {
+        const int tx = blockIdx.x % tiles_x1, ty = blockIdx.x / tiles_x1;
+        const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+        const int x = 16 * tx + lx, y = 16 * ty + ly;
+        if (x < a.w[1] && y < a.h[1]) {
+            const Cam6 c = model_l1(a, x, y);
+            for (int m = 0; m < 3; ++m) {
+                s1[m][ly][lx] = c.v[m];
+                s1[3 + m][ly][lx] = c.n[m];
+            }
+            model_store(a, 1, (size_t)y * a.w[1] + x, c, R, t);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int qx = threadIdx.x & 7, qy = threadIdx.x >> 3;
+            const int X = 8 * tx + qx, Y = 8 * ty + qy;
+            if (X < a.w[2] && Y < a.h[2]) {
+                Cam6 q[4];   // (2X, 2Y), (2X+1, 2Y), (2X, 2Y+1), (2X+1, 2Y+1) of level 1
+                for (int k = 0; k < 4; ++k) {
+                    const int sx = 2 * qx + (k & 1), sy = 2 * qy + (k >> 1);
+                    for (int m = 0; m < 3; ++m) {
+                        q[k].v[m] = s1[m][sy][sx];
+                        q[k].n[m] = s1[3 + m][sy][sx];
+                    }
+                }
+                model_store(a, 2, (size_t)Y * a.w[2] + X, mean_cam(q[0], q[1], q[2], q[3]), R, t);
+            }
+        }
+        return;
+    }
+    const int i = (blockIdx.x - nb1) * 256 + threadIdx.x;
+    const int w = a.w[0];
+    if (i >= w * a.h[0]) return;
+    const int y = i / w, x = i - y * w;
+    if (a.vz) gp(a.vz)[i] = gp(a.v4)[(size_t)y * a.v_pitch + x].z;
+    model_store(a, 0, (size_t)i, model_l0(a, x, y), R, t);
+}
+
 void launch_model_maps(const ModelLevels& a, hipStream_t s)
 {
-    int n[kLevels];
-    for (int l = 0; l < kLevels; ++l) n[l] = a.w[2 - l] * a.h[2 - l];
-    const Seg3 g = make_seg(n, 256);
-    hipLaunchKernelGGL(k_model_maps, dim3(g.b0[kLevels]), dim3(256), 0, s, a, g);
+    const int tiles_x1 = cdiv(a.w[1], 16), nb1 = tiles_x1 * cdiv(a.h[1], 16);
+    const int nb0 = cdiv(a.w[0] * a.h[0], 256);
+    hipLaunchKernelGGL(k_model_maps, dim3(nb1 + nb0), dim3(256), 0, s, a, nb1, tiles_x1);
 }
 
 // ============================================== a9 + a11 Gaussian pyrDown
@@ -1293,6 +1315,13 @@ struct Prolog {
     unsigned int toti[2];
 };
 
+// Shared memory of the unfused pass A: the state record and the count totals.
+template <int NT>
+struct StateScratch {
+    IterState S;
+    unsigned int redi[NT / 64][2];
+};
+
 // State entering iteration `iter` of `level`: skipped when the rgbOnly test
 // stopped this level, else the warp constants of the current estimate
 // (RGBDOdometryef.cpp:418-432) and the lastRGBError the test compares to.
@@ -1443,7 +1472,8 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
                                                int level, int iter, int k, int prev_level, TrackParams P,
                                                uint8_t* __restrict__ dbg_mask)
 {
-    __shared__ Prolog<NT> W;
+    // the fused prologue's scratch; the unfused pass reads the state only
+    __shared__ std::conditional_t<FUSED, Prolog<NT>, StateScratch<NT>> W;
     __shared__ float s_red[NT / 64][kSE3];
     const int p = blk_pair ? blk_pair[blockIdx.x] : 0;
     const PairDesc& D = pairs[p];
@@ -1461,7 +1491,7 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     PrevPose pp;
     load_prev(st, pp);
 
-    if (FUSED) {
+    if constexpr (FUSED) {
         finish_previous<NT>(D, P, k, prev_level, pp, W);
         if (threadIdx.x == 0) enter_iteration(W.S, L, level, iter);
         __syncthreads();
@@ -1552,7 +1582,7 @@ template <int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pairs, const int* __restrict__ blk_pair,
                                                int level, int iter, int k, TrackParams P, int mode)
 {
-    __shared__ Prolog<NT> W;
+    __shared__ unsigned int s_redi[NT / 64][2];
     __shared__ float s_red[NT / 64][kSE3];
     const int p = blk_pair ? blk_pair[blockIdx.x] : 0;
     const PairDesc& D = pairs[p];
@@ -1592,15 +1622,15 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
         c0 = wave_total_u(c0);
         c1 = wave_total_u(c1);
         if ((threadIdx.x & 63) == 63) {
-            W.redi[threadIdx.x >> 6][0] = c0;
-            W.redi[threadIdx.x >> 6][1] = c1;
+            s_redi[threadIdx.x >> 6][0] = c0;
+            s_redi[threadIdx.x >> 6][1] = c1;
         }
         __syncthreads();
         unsigned int count = 0, sum2 = 0;
 #pragma unroll
         for (int wv = 0; wv < NT / 64; ++wv) {
-            count += W.redi[wv][0];
-            sum2 += W.redi[wv][1];
+            count += s_redi[wv][0];
+            sum2 += s_redi[wv][1];
         }
         float sv, err;
         rgb_sigma_error((int)count, (int)sum2, sv, err);

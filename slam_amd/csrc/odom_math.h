@@ -192,7 +192,7 @@ ODOM_HD int ldlt_solve(const double* Ain, const double* b, double* x)
 // 1/d for the positive pivots of ldlt_spd.  On the device: v_rcp_f64 and two
 // Newton steps (each at least doubles the correct bits, so the result is
 // within an ulp of the quotient) instead of the full division sequence.
-ODOM_HD inline double recip_pos(double d)
+ODOM_HD double recip_pos(double d)
 {
 #ifdef __HIP_DEVICE_COMPILE__
     double r = __builtin_amdgcn_rcp(d);
