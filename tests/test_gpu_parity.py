@@ -37,7 +37,12 @@ def orc_mk(orc, cam, fp64=False):
     return orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, fp64=fp64)
 
 
-@pytest.fixture(scope="module", params=[(synth.CAM_320, 3), (synth.CAM_640, 5)], ids=["320x240", "640x480"])
+# 202x154: odd halvings (levels 101x77, 50x38) and partial tiles at every level
+CAM_ODD = synth.Camera(202, 154, 150.0, 150.0, 100.5, 76.5)
+
+
+@pytest.fixture(scope="module", params=[(synth.CAM_320, 3), (synth.CAM_640, 5), (CAM_ODD, 7)],
+                ids=["320x240", "640x480", "202x154"])
 def pair(request, orc):
     cam, seed = request.param
     T0, T1, f0, f1 = make_pair(cam, seed)
@@ -181,9 +186,9 @@ MODES = {
 }
 
 
+@pytest.mark.parametrize("cam", [synth.CAM_640, CAM_ODD], ids=["640x480", "202x154"])
 @pytest.mark.parametrize("mode", list(MODES))
-def test_track_matches_oracle(orc, mode):
-    cam = synth.CAM_640
+def test_track_matches_oracle(orc, mode, cam):
     T0, T1, f0, f1 = make_pair(cam, 11)
     filt = orc.bilateral(f1.depth, MAXD)
     res = {}
