@@ -285,7 +285,11 @@ def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain
             # relative pose error per frame: estimated step vs true step
             rpe = [synth.pose_error(np.linalg.inv(pose12(e[k - 2])) @ pose12(e[k - 1]),
                                     np.linalg.inv(bposes[k - 1]) @ bposes[k])[0] for k in range(2, nb)]
+            from slam_amd import evaluate as ev
+            st = [k / 30.0 for k in range(1, nb)]
+            aligned = ev.ate(ev.trajectory(st, bposes[1:nb]), ev.trajectory(st, [pose12(e[k - 1]) for k in range(1, nb)]))
             res[mode] = {"ate_mean_mm": round(1e3 * float(np.mean([q[0] for q in errs])), 2),
+                         "ate_rmse_aligned_mm": round(1e3 * aligned["rmse"], 2),
                          "rpe_mean_mm": round(1e3 * float(np.mean(rpe)), 3),
                          "frames_lost_gt_5cm": int(sum(1 for r in rpe if r > 0.05))}
         steps = [synth.pose_error(bposes[k], bposes[k - 1]) for k in range(1, nb)]
@@ -311,6 +315,7 @@ def main() -> int:
     ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
     ap.add_argument("--burst-frames", type=int, default=150)
     ap.add_argument("--no-graph", action="store_true", help="launch tracking calls kernel by kernel")
+    ap.add_argument("--pose-log", default=None, help="write the timed frames' poses in the frame loop's log format")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 4 device copies of 256 MiB (the reference kernel of counter runs)")
     ap.add_argument("--batch-reps", type=int, default=20)
@@ -421,6 +426,23 @@ def main() -> int:
     drift = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, args.steps + 1)]
     drift_t = float(np.mean([d[0] for d in drift]))
     drift_r = float(np.mean([d[1] for d in drift]))
+    # the reference's benchmark measures: ATE after Horn alignment and the
+    # one-frame RPE, at 30 Hz stamps; optionally the pose log it writes
+    from slam_amd import dataio
+    from slam_amd import evaluate as ev
+    stamps = [k / 30.0 for k in range(1, args.steps + 1)]
+    est_poses = [pose12(est[k - 1]) for k in range(1, args.steps + 1)]
+    gt_traj = ev.trajectory(stamps, [poses[k] for k in range(1, args.steps + 1)])
+    est_traj = ev.trajectory(stamps, est_poses)
+    ate_r = ev.ate(gt_traj, est_traj)
+    rpe_r = ev.rpe(gt_traj, est_traj, delta=1, delta_unit="f")
+    trajectory_eval = {"ate_rmse_mm": round(1e3 * ate_r["rmse"], 3), "ate_max_mm": round(1e3 * ate_r["max"], 3),
+                       "rpe_rmse_mm": round(1e3 * rpe_r["translational"]["rmse"], 4),
+                       "rpe_rmse_deg": round(rpe_r["rotational_deg"]["rmse"], 5),
+                       "evaluator": "slam_amd/evaluate.py (benchmark/evaluate_ate.py, evaluate_rpe.py: "
+                                    "Horn-aligned ATE, 1-frame RPE)"}
+    if args.pose_log:
+        dataio.write_pose_log(args.pose_log, list(range(1, args.steps + 1)), est_poses)
 
     # roofline of the step kernels: per level, the last iteration of a
     # completed call relaunched back to back between one HIP event pair on the
@@ -523,6 +545,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "accuracy": accuracy,
         "drift_vs_truth": {"mean_mm": round(1e3 * drift_t, 3), "mean_deg": round(drift_r, 4)},
+        "trajectory": trajectory_eval,
         "last_stats": stats,
         "shapes": shapes,
         "learned": learned,
