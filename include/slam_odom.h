@@ -184,6 +184,15 @@ int odom_profile_steps(odom_handle h, int rgb_only, float icp_weight, int reps, 
  * fastest per kernel and level. */
 int odom_autotune(odom_handle h, int rgb_only, float icp_weight);
 
+/* The loop-closure acceptance test of the reference's frame loop
+ * (elastic_fusion_file.cpp:474-497) on the last tracking call of this
+ * handle, run as model-to-model registration (odom_init_icp_model with the
+ * inactive model's maps, odom_init_icp_maps with the active model's, so3 off):
+ * ok = every diagonal entry of getCovariance() <= cov_thresh and
+ * lastICPCount > icp_count_thresh and lastICPError < icp_err_thresh.  A
+ * singular last system (no covariance) is not accepted.  Synchronous. */
+int odom_loop_check(odom_handle h, double cov_thresh, float icp_count_thresh, float icp_err_thresh, int *ok);
+
 /* ---- batches: many frame pairs (handles, of any size) tracked by the same
  * launches.  No counterpart in the reference, whose RGBDOdometryef tracks one
  * pair per object and call (RGBDOdometryef.cpp:267-595); this is the batched

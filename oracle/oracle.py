@@ -367,6 +367,19 @@ class Odometry:
             raise np.linalg.LinAlgError("lastA is singular")
         return c.reshape(6, 6)
 
+    def loop_accept(self, covThresh, icpCountThresh, icpErrThresh):
+        """Loop-closure acceptance of the last call (elastic_fusion_file.cpp:
+        477-498): every diagonal entry of the covariance <= covThresh,
+        lastICPCount > icpCountThresh and lastICPError < icpErrThresh.  A
+        singular last system has no covariance and is not accepted."""
+        try:
+            cov = self.getCovariance()
+        except np.linalg.LinAlgError:
+            return False
+        s = self.stats()
+        return bool(np.all(np.diag(cov) <= covThresh) and s["lastICPCount"] > icpCountThresh
+                    and s["lastICPError"] < icpErrThresh)
+
     def buf(self, name, level):
         h, w = self.h >> level, self.w >> level
         L = lib()

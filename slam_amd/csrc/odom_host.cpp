@@ -703,6 +703,25 @@ int odom_synchronize(odom_handle c)
     return ODOM_OK;
 }
 
+int odom_loop_check(odom_handle c, double cov_thresh, float icp_count_thresh, float icp_err_thresh, int* ok)
+{
+    if (!c || !ok) return ODOM_ERR_ARG;
+    *ok = 0;
+    double cov[36];
+    const int rc = odom_get_covariance(c, cov);
+    if (rc == ODOM_ERR_STATE) return ODOM_OK;  // singular last system: not accepted
+    if (rc) return rc;
+    bool cov_ok = true;
+    for (int i = 0; i < 6; ++i)
+        if (cov[i * 7] > cov_thresh) {
+            cov_ok = false;
+            break;
+        }
+    const TrackState& s = c->h_st;             // read by odom_get_covariance
+    *ok = (cov_ok && s.lastICPCount > icp_count_thresh && s.lastICPError < icp_err_thresh) ? 1 : 0;
+    return ODOM_OK;
+}
+
 int odom_track_async(odom_handle c, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3)
 {
     if (!c) return ODOM_ERR_ARG;

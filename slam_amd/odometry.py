@@ -63,6 +63,7 @@ EXPORTS = [
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
     "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
+    "odom_loop_check",
 ]
 
 _V = ctypes.c_void_p
@@ -91,6 +92,7 @@ def lib() -> ctypes.CDLL:
     L.odom_get_pose.argtypes = [_V, _V, _V]
     L.odom_synchronize.argtypes = [_V]
     L.odom_get_covariance.argtypes = [_V, _V]
+    L.odom_loop_check.argtypes = [_V, ctypes.c_double, ctypes.c_float, ctypes.c_float, _P(ctypes.c_int)]
     L.odom_get_last_system.argtypes = [_V, _V, _V]
     L.odom_get_stats.argtypes = [_V, _P(Stats)]
     L.odom_icp_step.argtypes = [_V, ctypes.c_int, _V, _V, _V, _V, _V, _V]
@@ -323,6 +325,14 @@ class RGBDOdometry:
         c = np.zeros(36, np.float64)
         _check("odom_get_covariance", lib().odom_get_covariance(self._h, _p(c)))
         return c.reshape(6, 6)
+
+    def loopCheck(self, covThresh: float, icpCountThresh: float, icpErrThresh: float) -> bool:
+        """The frame loop's loop-closure acceptance test on the last call
+        (elastic_fusion_file.cpp:474-497)."""
+        ok = ctypes.c_int(0)
+        _check("odom_loop_check", lib().odom_loop_check(self._h, float(covThresh), float(icpCountThresh),
+                                                        float(icpErrThresh), ctypes.byref(ok)))
+        return bool(ok.value)
 
     def stats(self) -> dict:
         s = Stats()
