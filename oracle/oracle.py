@@ -31,6 +31,15 @@ class Intr(ctypes.Structure):
 DATATERM = np.dtype([("zero_x", "<i2"), ("zero_y", "<i2"), ("one_x", "<i2"), ("one_y", "<i2"),
                      ("diff", "<f4"), ("valid", "<i4")])
 
+class SplatParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("depth_cutoff", ctypes.c_float),
+                ("conf_threshold", ctypes.c_float), ("time", ctypes.c_int), ("max_time", ctypes.c_int),
+                ("time_delta", ctypes.c_int)]
+
+
+_P_SPLAT = ctypes.POINTER(SplatParams)
+
 _lib = None
 
 
@@ -90,6 +99,9 @@ def lib():
         L.orc_pyrdown_u8.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V]
         L.orc_derivative.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V, _V]
         L.orc_point_cloud.argtypes = [_V, ctypes.c_int, ctypes.c_int, Intr, _V]
+        L.orc_splat_predict.argtypes = [_P_SPLAT, _V, _V, ctypes.c_int, _V, _V, _V, _V]
+        L.orc_splat_fill_in.argtypes = [_P_SPLAT, _V, _V, _V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _V, _V, _V]
         _lib = L
     return _lib
 
@@ -392,3 +404,34 @@ class Odometry:
                 a = np.ctypeslib.as_array(p, shape=(n,)).copy()
                 return a.reshape((3, h, w) if planes == 3 else (h, w))
         raise KeyError(name)
+
+
+# ------------------------------------------------------- splat and fill-in
+
+def splat_predict(width, height, cx, cy, fx, fy, pose, model, depthCutoff, confThreshold, time, maxTime, timeDelta):
+    """IndexMap::combinedPredict restated (splat_oracle.c): returns (image
+    RGBA8, vertex float4, normal float4, time uint16), each height x width."""
+    P = SplatParams(width, height, cx, cy, fx, fy, depthCutoff, confThreshold, time, maxTime, timeDelta)
+    T = _c(pose, np.float32).reshape(16)
+    m = _c(model, np.float32).reshape(-1, 12)
+    image = np.zeros((height, width, 4), np.uint8)
+    vertex = np.zeros((height, width, 4), np.float32)
+    normal = np.zeros((height, width, 4), np.float32)
+    tm = np.zeros((height, width), np.uint16)
+    lib().orc_splat_predict(ctypes.byref(P), _p(T), _p(m), m.shape[0], _p(image), _p(vertex), _p(normal), _p(tm))
+    return image, vertex, normal, tm
+
+
+def splat_fill_in(width, height, cx, cy, fx, fy, image, vertex, normal, depth, rgb, passthroughGeometry=False,
+                  passthroughImage=False):
+    """FillIn::vertex / normal / image restated: returns (image RGBA8,
+    vertex float4, normal float4)."""
+    P = SplatParams(width, height, cx, cy, fx, fy, 0.0, 0.0, 0, 0, 0)
+    im, v, n = _c(image, np.uint8), _c(vertex, np.float32), _c(normal, np.float32)
+    d, r = _c(depth, np.uint16), _c(rgb, np.uint8)
+    fi = np.zeros((height, width, 4), np.uint8)
+    fv = np.zeros((height, width, 4), np.float32)
+    fn = np.zeros((height, width, 4), np.float32)
+    lib().orc_splat_fill_in(ctypes.byref(P), _p(im), _p(v), _p(n), _p(d), _p(r), r.shape[2], int(passthroughGeometry),
+                            int(passthroughImage), _p(fi), _p(fv), _p(fn))
+    return fi, fv, fn

@@ -152,9 +152,21 @@ def _p(a: np.ndarray):
 _FMT_BPP = {FMT_DEPTH_U16: 2, FMT_RGB8: 3, FMT_RGBA8: 4, FMT_RGBA32F: 16}
 
 
+class DeviceImage:
+    """A width x height image at a device address owned by a library handle
+    (for instance a splat prediction's output): passed as an ODOM_MEM_DEVICE
+    view.  ``owner`` keeps that handle alive."""
+
+    def __init__(self, ptr: int, width: int, height: int, owner=None):
+        self.ptr, self.width, self.height, self.owner = int(ptr), int(width), int(height), owner
+        self.shape = (self.height, self.width)
+
+
 def frame_view(img, fmt: int) -> tuple[FrameView, object]:
-    """A view of a numpy array (host) or a torch CUDA tensor (device).
-    Returns the view and the object that keeps its memory alive."""
+    """A view of a numpy array (host), a torch CUDA tensor or a DeviceImage
+    (device).  Returns the view and the object that keeps its memory alive."""
+    if isinstance(img, DeviceImage):
+        return FrameView(img.ptr, img.width * _FMT_BPP[fmt], img.width, img.height, fmt, MEM_DEVICE), img
     try:
         import torch
         is_t = isinstance(img, torch.Tensor)
@@ -175,6 +187,8 @@ def frame_view(img, fmt: int) -> tuple[FrameView, object]:
 
 
 def _rgb_fmt(img) -> int:
+    if isinstance(img, DeviceImage):
+        return FMT_RGBA8
     return FMT_RGB8 if img.shape[2] == 3 else FMT_RGBA8
 
 

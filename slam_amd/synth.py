@@ -91,6 +91,45 @@ class Scene:
         return Frame(T.copy(), depth, rgb, vert4, norm4, rgba)
 
 
+# ---------------------------------------------------------------- surfel models
+
+def encode_color(rgb: np.ndarray) -> np.ndarray:
+    """color.glsl encodeColor of 8-bit colours: float(r << 16 | g << 8 | b)."""
+    c = np.asarray(rgb, dtype=np.int64)
+    return ((c[..., 0] << 16) | (c[..., 1] << 8) | c[..., 2]).astype(np.float32)
+
+
+def surfels_from_frame(frame: Frame, cam: Camera, *, step: int = 1, conf: float = 10.0, init_time: int = 0,
+                       last_time: int = 0, conf_jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """A surfel model, (n, 12) float32 in the layout {x, y, z, confidence,
+    colour, 0, init_time, last_time, nx, ny, nz, radius}, from a rendered
+    frame's noise-free maps: every ``step``-th valid pixel (z > 0) in each
+    direction becomes a surfel at its global position with its global normal,
+    the frame's colour and the radius of surfels.glsl getRadius
+    ((z / meanFocal) * sqrt2 / |n_z|, at most twice (z / meanFocal) * sqrt2)."""
+    v = frame.vert4[::step, ::step].reshape(-1, 4).astype(np.float64)
+    n = frame.norm4[::step, ::step].reshape(-1, 4).astype(np.float64)
+    col = encode_color(frame.rgba[::step, ::step].reshape(-1, 4)[:, :3])
+    ok = (v[:, 2] > 0) & np.isfinite(v[:, 2]) & np.isfinite(n[:, 2])
+    v, n, col = v[ok], n[ok], col[ok]
+    R, t = frame.pose[:3, :3], frame.pose[:3, 3]
+    mean_focal = ((1.0 / abs(1.0 / cam.fx)) + (1.0 / abs(1.0 / cam.fy))) / 2.0
+    radius = (v[:, 2] / mean_focal) * 1.41421356237
+    radius_n = np.minimum(2.0 * radius, radius / np.maximum(np.abs(n[:, 2]), 1e-6))
+    out = np.zeros((v.shape[0], 12), np.float32)
+    out[:, 0:3] = v[:, :3] @ R.T + t
+    c = np.full(v.shape[0], conf, np.float64)
+    if conf_jitter:
+        c = c + np.random.default_rng(seed).uniform(-conf_jitter, conf_jitter, v.shape[0])
+    out[:, 3] = c
+    out[:, 4] = col
+    out[:, 6] = init_time
+    out[:, 7] = last_time
+    out[:, 8:11] = n[:, :3] @ R.T
+    out[:, 11] = radius_n
+    return out
+
+
 # ---------------------------------------------------------------- SE(3) helpers
 
 def so3_exp(w: np.ndarray) -> np.ndarray:

@@ -13,6 +13,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "slam_odom.h")
 CNN_HEADER = os.path.join(ROOT, "include", "slam_cnn.h")
+SPLAT_HEADER = os.path.join(ROOT, "include", "slam_splat.h")
 
 
 def declared(path=HEADER, prefix="odom_"):
@@ -42,8 +43,13 @@ def test_cnn_header_and_binding_agree():
     assert declared(CNN_HEADER, "cnn_") == set(warmstart.EXPORTS)
 
 
+def test_splat_header_and_binding_agree():
+    from slam_amd import splat
+    assert declared(SPLAT_HEADER, "splat_") == set(splat.EXPORTS)
+
+
 def test_library_exports_every_declared_symbol(L):
-    for name in declared() | declared(CNN_HEADER, "cnn_"):
+    for name in declared() | declared(CNN_HEADER, "cnn_") | declared(SPLAT_HEADER, "splat_"):
         assert hasattr(L, name), name
     # and they are real dynamic symbols of the shared object
     from slam_amd import _native
@@ -74,6 +80,21 @@ def test_no_device_is_an_error_not_a_fallback(L):
     assert not h.value
     with pytest.raises(RuntimeError):
         odometry.RGBDOdometry(64, 48, 31.5, 23.5, 50.0, 50.0)
+
+
+def test_splat_without_device_is_an_error(L):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from slam_amd import splat
+    splat.lib()
+    h = ctypes.c_void_p()
+    assert L.splat_create(64, 48, 31.5, 23.5, 50.0, 50.0, 0, ctypes.byref(h)) == -4   # ODOM_ERR_NODEV
+    assert not h.value
+    assert L.splat_create(0, 48, 31.5, 23.5, 50.0, 50.0, 0, ctypes.byref(h)) == -1    # ODOM_ERR_ARG
+    assert L.splat_destroy(None) == -1
+    with pytest.raises(RuntimeError):
+        splat.IndexMap(64, 48, 31.5, 23.5, 50.0, 50.0)
 
 
 def test_null_handles_are_rejected(L):
