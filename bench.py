@@ -169,6 +169,81 @@ def batched_pairs(args) -> dict:
     return out
 
 
+def splat_prediction(args) -> dict:
+    """The model prediction that feeds frame-to-model tracking: a 640x480
+    combinedPredict of a 2M-surfel model (48 B per surfel: seven views of one
+    room, in insertion order) and its fill-in, with the reference configs'
+    confidence 10, timeDelta 200 and depth cutoff 20 m.  Per-launch times come
+    from HIP events on the handle's stream; the raster pass's roofline counts
+    the model bytes it must read.  One prediction and fill-in are checked bit
+    for bit against the oracle, whose time for that prediction on one host
+    core is the CPU baseline."""
+    import torch
+    from slam_amd import splat, synth
+
+    cam = synth.CAM_640
+    scene = synth.Scene(900)
+    poses = synth.spline_trajectory(8, seed=900, max_step_m=0.12, max_step_deg=8.0)
+    t0 = time.time()
+    parts = []
+    for k, T in enumerate(poses[:7]):
+        f = scene.render(T, cam, noise_seed=900 + k, sensor=False)
+        parts.append(synth.surfels_from_frame(f, cam, conf=10.0 + k, init_time=k, last_time=100 + k))
+    model = np.ascontiguousarray(np.concatenate(parts)[:args.splat_surfels])
+    n = model.shape[0]
+    view = scene.render(poses[3], cam, noise_seed=990)
+    log(f"splat: {n} surfels built in {time.time() - t0:.1f}s")
+    pose = poses[3].astype(np.float32)
+    P = dict(depthCutoff=20.0, confThreshold=10.0, time=110, maxTime=110, timeDelta=200)
+    dev_model = torch.from_numpy(model).cuda()
+    dev_depth = torch.from_numpy(view.depth).cuda()
+    dev_rgb = torch.from_numpy(view.rgb).cuda()
+    im = splat.IndexMap(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    im.profile(pose, dev_model, P["depthCutoff"], P["confThreshold"], P["time"], P["maxTime"], P["timeDelta"],
+               dev_depth, dev_rgb, reps=5)
+    prof = im.profile(pose, dev_model, P["depthCutoff"], P["confThreshold"], P["time"], P["maxTime"],
+                      P["timeDelta"], dev_depth, dev_rgb, reps=args.splat_reps)
+    total_us = prof["raster_us"] + prof["resolve_us"] + prof["fill_us"]
+    model_bytes = 48 * n
+    out_bytes = cam.width * cam.height * (8 + 8 + 38)          # keys read + cleared, outputs
+    fill_bytes = cam.width * cam.height * (36 + 2 + 3 + 36)    # prediction + sensor in, fill out
+    res = {
+        "workload": f"640x480 combinedPredict + fill-in of a {n}-surfel model (48 B each), confidence 10, "
+                    "timeDelta 200, cutoff 20 m",
+        "surfels": n,
+        "predictions_per_s": round(1e6 / total_us, 1),
+        "us": {k: round(prof[k], 2) for k in ("raster_us", "resolve_us", "fill_us")},
+        "fragments": prof["fragments"],
+        "roofline": {"bound": "hbm", "kernel": "k_splat_raster", "achieved": round(model_bytes / prof["raster_us"] / 1e3, 1),
+                     "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(model_bytes / prof["raster_us"] / 1e3 / 8000.0, 4),
+                     "bytes_per_launch": model_bytes, "avg_launch_us": round(prof["raster_us"], 2)},
+        "resolve_gbs": round(out_bytes / prof["resolve_us"] / 1e3, 1),
+        "fill_gbs": round(fill_bytes / prof["fill_us"] / 1e3, 1) if prof["fill_us"] else None,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        im.combinedPredict(pose, dev_model, P["depthCutoff"], P["confThreshold"], P["time"], P["maxTime"],
+                           P["timeDelta"])
+        im.fillIn(dev_depth, dev_rgb)
+        got = [im.read(w) for w in (splat.IMAGE, splat.VERTEX, splat.NORMAL, splat.TIME)]
+        fill = [im.read(w) for w in (splat.FILL_IMAGE, splat.FILL_VERTEX, splat.FILL_NORMAL)]
+        t1 = time.time()
+        exp = orc.splat_predict(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, pose, model,
+                                P["depthCutoff"], P["confThreshold"], P["time"], P["maxTime"], P["timeDelta"])
+        cpu_s = time.time() - t1
+        expf = orc.splat_fill_in(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, *exp[:3], view.depth,
+                                 view.rgb)
+        same = all(np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+                   for a, b in zip(got + fill, list(exp) + list(expf)))
+        res["bit_exact_vs_oracle"] = bool(same)
+        res["cpu_baseline"] = {"value": round(1.0 / cpu_s, 3), "unit": "predictions/s", "cores": 1, "kind": "port",
+                               "sample": "one prediction of the same model and pose, oracle/splat_oracle.c "
+                                         "(in-order GL_LESS draw, scalar)"}
+    log(f"splat {res}")
+    return res
+
+
 def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain_est, ref64) -> dict | None:
     """The learned form on the same sequence: per frame the same init calls,
     then the network input from the handle's level-2 buffers, the network
@@ -319,6 +394,10 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 4 device copies of 256 MiB (the reference kernel of counter runs)")
     ap.add_argument("--batch-reps", type=int, default=20)
+    ap.add_argument("--no-splat", action="store_true", help="skip the splat prediction measurement")
+    ap.add_argument("--splat-only", action="store_true", help="measure only the splat prediction (profiling)")
+    ap.add_argument("--splat-surfels", type=int, default=2_000_000)
+    ap.add_argument("--splat-reps", type=int, default=50)
     ap.add_argument("--batch-oracle-pairs", type=int, default=6,
                     help="pairs of the batch checked against the FP64 oracle on the host")
     args = ap.parse_args()
@@ -328,6 +407,9 @@ def main() -> int:
         print(json.dumps({"error": "this build measures one GPU: the odometry path runs one frame pair per "
                                    "GPU and shards by sequence; run with --gpus 1"}))
         return 2
+    if args.splat_only:
+        print(json.dumps({"splat": splat_prediction(args)}), flush=True)
+        return 0
 
     import torch
     from slam_amd import synth
@@ -523,6 +605,7 @@ def main() -> int:
     if not args.no_learned:
         learned = learned_vs_plain(args, cam, poses, views, L, h, odo, reset, fps, est, ref64)
     batched = None if args.no_batch else batched_pairs(args)
+    splat_res = None if args.no_splat else splat_prediction(args)
 
     line = {
         "metric": "frames/s 640x480 ICP+RGB at <=1mm vs FP64 oracle; learned-init vs plain",
@@ -550,6 +633,7 @@ def main() -> int:
         "shapes": shapes,
         "learned": learned,
         "batched": batched,
+        "splat": splat_res,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -13,11 +13,13 @@
 //
 // Here that is two launches and no rasterizer:
 //   k_splat_raster   one thread per surfel: the vertex stage, then every
-//                    pixel centre of the sprite square; a fragment that passes
-//                    the disk test offers ((24-bit depth) << 32 | surfel index)
-//                    to the pixel with a 64-bit atomic minimum, which is
-//                    GL_LESS in draw order: the nearest depth wins and, among
-//                    equal depths, the lowest index (the first drawn);
+//                    pixel centre of the sprite square (exact bounds); a
+//                    fragment that passes the disk test offers
+//                    ((24-bit depth) << 32 | surfel index) to the pixel with a
+//                    64-bit atomic minimum, which is GL_LESS in draw order: the
+//                    nearest depth wins and, among equal depths, the lowest
+//                    index (the first drawn).  The ray through each pixel
+//                    centre is read from a per-handle table (k_rays);
 //   k_splat_resolve  one thread per pixel: reads the winning key, clears it
 //                    for the next prediction, and recomputes that surfel's
 //                    fragment at this pixel to write the four outputs.
@@ -130,20 +132,26 @@ __device__ __forceinline__ bool sprite(const Surf& s, const SplatCam& K, float& 
     return true;
 }
 
-// combo_splat.frag:37-48, 59: the ray through the pixel centre (fcx, fcy)
-// meets the surfel plane at z; false when the point lies outside the disk or
-// the depth does not pass against the cleared buffer.
-__device__ __forceinline__ bool fragment(const Surf& s, const SplatCam& K, float maxDepth, float fcx, float fcy,
-                                         float& z, unsigned& q)
+// combo_splat.frag:37: the normalized ray through the pixel centre (fcx,
+// fcy).  It depends on the pixel only, so a handle computes it once per
+// pixel (k_rays) and every fragment reads it.
+__device__ __forceinline__ float4 pixel_ray(const SplatCam& K, float fcx, float fcy)
 {
     const float lx = (fcx - K.cx) / K.fx;
     const float ly = (fcy - K.cy) / K.fy;
     const float rl = 1.0f / sqrtf((lx * lx + ly * ly) + 1.0f);
-    const float l0 = lx * rl, l1 = ly * rl, l2 = 1.0f * rl;
-    const float num = (s.hx * s.nx + s.hy * s.ny) + s.hz * s.nz;
-    const float den = (l0 * s.nx + l1 * s.ny) + l2 * s.nz;
+    return make_float4(lx * rl, ly * rl, 1.0f * rl, 0.0f);
+}
+
+// combo_splat.frag:38-48, 59: the ray l meets the surfel plane at z (num is
+// dot(position, normal)); false when the point lies outside the disk or the
+// depth does not pass against the cleared buffer.
+__device__ __forceinline__ bool fragment(const Surf& s, float num, float maxDepth, const float4 l, float& z,
+                                         unsigned& q)
+{
+    const float den = (l.x * s.nx + l.y * s.ny) + l.z * s.nz;
     const float t = num / den;
-    const float c0 = t * l0, c1 = t * l1, c2 = t * l2;
+    const float c0 = t * l.x, c1 = t * l.y, c2 = t * l.z;
     const float d0 = c0 - s.hx, d1 = c1 - s.hy, d2 = c2 - s.hz;
     const float dd = (d0 * d0 + d1 * d1) + d2 * d2;
     if (!(dd <= s.rad * s.rad)) return false;
@@ -155,52 +163,76 @@ __device__ __forceinline__ bool fragment(const Surf& s, const SplatCam& K, float
     return true;
 }
 
+__device__ __forceinline__ float plane_num(const Surf& s) { return (s.hx * s.nx + s.hy * s.ny) + s.hz * s.nz; }
+
+// The pixel centres c = i + 0.5 of the sprite: lo <= c < hi.  Window
+// coordinates stay below 2^13 and the size at most 256, so lo - 0.5 is exact
+// and the ceilings give the bounds; the checks only guard that reasoning.
+__device__ __forceinline__ int first_centre(float lo)
+{
+    int i = (int)ceilf(lo - 0.5f);
+    if ((float)(i - 1) + 0.5f >= lo) i -= 1;
+    else if ((float)i + 0.5f < lo) i += 1;
+    return i;
+}
+
+__device__ __forceinline__ int last_centre(float hi)
+{
+    int i = (int)ceilf(hi - 0.5f) - 1;
+    if ((float)(i + 1) + 0.5f < hi) i += 1;
+    else if ((float)i + 0.5f >= hi) i -= 1;
+    return i;
+}
+
+__global__ __launch_bounds__(256) void k_rays(SplatCam K, float4* rays)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= K.w * K.h) return;
+    rays[p] = pixel_ray(K, (float)(p % K.w) + 0.5f, (float)(p / K.w) + 0.5f);
+}
+
 __global__ __launch_bounds__(256) void k_fill_keys(unsigned long long* keys, int n)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) keys[i] = kKeyClear;
 }
 
+// One thread per surfel: the vertex stage, then the pixel centres of its
+// sprite (exact bounds, so every iteration is a candidate fragment).
 __global__ __launch_bounds__(256) void k_splat_raster(const float4* __restrict__ model, int n, SplatUniforms U,
-                                                      unsigned long long* keys, unsigned long long* frag_count)
+                                                      const float4* __restrict__ rays, unsigned long long* keys,
+                                                      unsigned long long* frag_count)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     const size_t b = (size_t)k * 3;
     Surf s;
-    if (!surfel_stage(model[b], model[b + 1], model[b + 2], U, s)) return;
     float xw, yw, size;
-    if (!sprite(s, U.cam, xw, yw, size)) return;
-    // pixel centres i + 0.5 with lo <= i + 0.5 < hi (and likewise in y);
-    // the loop bounds are a superset, the test decides
+    if (!surfel_stage(model[b], model[b + 1], model[b + 2], U, s) || !sprite(s, U.cam, xw, yw, size)) return;
     const float half = size * 0.5f;
-    const float lox = xw - half, hix = xw + half, loy = yw - half, hiy = yw + half;
-    const int i0 = max(0, (int)floorf(lox - 0.5f)), i1 = min(U.cam.w - 1, (int)ceilf(hix));
-    const int j0 = max(0, (int)floorf(loy - 0.5f)), j1 = min(U.cam.h - 1, (int)ceilf(hiy));
+    const int i0 = max(0, first_centre(xw - half));
+    const int i1 = min(U.cam.w - 1, last_centre(xw + half));
+    const int j0 = max(0, first_centre(yw - half));
+    const int j1 = min(U.cam.h - 1, last_centre(yw + half));
+    const float num = plane_num(s);
+    const int w = U.cam.w;
     unsigned drawn = 0;
     for (int j = j0; j <= j1; ++j) {
-        const float fcy = (float)j + 0.5f;
-        if (!(loy <= fcy && fcy < hiy)) continue;
+        const size_t row = (size_t)j * w;
         for (int i = i0; i <= i1; ++i) {
-            const float fcx = (float)i + 0.5f;
-            if (!(lox <= fcx && fcx < hix)) continue;
             float z;
             unsigned q;
-            if (!fragment(s, U.cam, U.maxDepth, fcx, fcy, z, q)) continue;
+            if (!fragment(s, num, U.maxDepth, rays[row + i], z, q)) continue;
             ++drawn;
-            const unsigned long long key = ((unsigned long long)q << 32) | (unsigned)k;
-            unsigned long long* p = keys + (size_t)j * U.cam.w + i;
-            // a stale read is never below the current minimum, so skipping on
-            // it never drops a winning fragment
-            if (key < *p) atomicMin(p, key);
+            atomicMin(keys + row + i, ((unsigned long long)q << 32) | (unsigned)k);
         }
     }
     if (frag_count && drawn) atomicAdd(frag_count, (unsigned long long)drawn);
 }
 
 __global__ __launch_bounds__(256) void k_splat_resolve(const float4* __restrict__ model, SplatUniforms U,
-                                                       unsigned long long* keys, uchar4* image, float4* vertex,
-                                                       float4* normal, uint16_t* time)
+                                                       const float4* __restrict__ rays, unsigned long long* keys,
+                                                       uchar4* image, float4* vertex, float4* normal, uint16_t* time)
 {
     const int w = U.cam.w;
     const int p = blockIdx.x * 256 + threadIdx.x;
@@ -220,7 +252,7 @@ __global__ __launch_bounds__(256) void k_splat_resolve(const float4* __restrict_
         // the winner passed both stages in the raster pass; the same
         // expressions give the same values here
         if (surfel_stage(model[b], model[b + 1], model[b + 2], U, s) &&
-            fragment(s, U.cam, U.maxDepth, fcx, fcy, z, q)) {
+            fragment(s, plane_num(s), U.maxDepth, rays[p], z, q)) {
             // color.glsl decodeColor, stored to an 8-bit unorm target
             const int ci = (int)fminf(fmaxf(s.col, 0.0f), 16777215.0f);
             im = make_uchar4((ci >> 16) & 0xFF, (ci >> 8) & 0xFF, ci & 0xFF, 255);
@@ -313,6 +345,7 @@ struct splat_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     unsigned long long* keys = nullptr;
+    float4* rays = nullptr;          // per-pixel normalized ray through the pixel centre
     unsigned long long* frag_count = nullptr;
     uchar4 *image = nullptr, *fimage = nullptr;
     float4 *vertex = nullptr, *normal = nullptr, *fvertex = nullptr, *fnormal = nullptr;
@@ -337,7 +370,7 @@ namespace {
 
 void free_ctx(splat_ctx* c)
 {
-    void* bufs[] = {c->keys, c->frag_count, c->image, c->fimage, c->vertex, c->normal, c->fvertex,
+    void* bufs[] = {c->keys, c->rays, c->frag_count, c->image, c->fimage, c->vertex, c->normal, c->fvertex,
                     c->fnormal, c->time, c->staging, c->depth_staging, c->rgb_staging};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -389,10 +422,10 @@ int enqueue_predict(splat_ctx* c, const SplatUniforms& U, const float* model, in
 {
     if (n > 0)
         hipLaunchKernelGGL(k_splat_raster, dim3(cdiv(n, 256)), dim3(256), 0, c->stream, (const float4*)model, n, U,
-                           c->keys, count ? c->frag_count : nullptr);
+                           c->rays, c->keys, count ? c->frag_count : nullptr);
     const int npix = c->cam.w * c->cam.h;
     hipLaunchKernelGGL(k_splat_resolve, dim3(cdiv(npix, 256)), dim3(256), 0, c->stream, (const float4*)model, U,
-                       c->keys, c->image, c->vertex, c->normal, c->time);
+                       c->rays, c->keys, c->image, c->vertex, c->normal, c->time);
     SCHK(hipGetLastError());
     return ODOM_OK;
 }
@@ -450,6 +483,7 @@ int splat_create(int width, int height, float cx, float cy, float fx, float fy, 
     bool ok = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) == hipSuccess;
     c->stream = c->own;
     ok = ok && hipMalloc((void**)&c->keys, npix * 8) == hipSuccess;
+    ok = ok && hipMalloc((void**)&c->rays, npix * 16) == hipSuccess;
     ok = ok && hipMalloc((void**)&c->frag_count, 8) == hipSuccess;
     ok = ok && hipMalloc((void**)&c->image, npix * 4) == hipSuccess;
     ok = ok && hipMalloc((void**)&c->fimage, npix * 4) == hipSuccess;
@@ -467,6 +501,7 @@ int splat_create(int width, int height, float cx, float cy, float fx, float fy, 
     if (ok) {
         hipLaunchKernelGGL(k_fill_keys, dim3(cdiv((long long)npix, 256)), dim3(256), 0, c->stream, c->keys,
                            (int)npix);
+        hipLaunchKernelGGL(k_rays, dim3(cdiv((long long)npix, 256)), dim3(256), 0, c->stream, c->cam, c->rays);
         ok = hipGetLastError() == hipSuccess;
         // outputs start as a prediction of an empty model
         ok = ok && hipMemsetAsync(c->image, 0, npix * 4, c->stream) == hipSuccess;
@@ -584,10 +619,10 @@ int splat_profile(splat_handle c, const float pose[16], const float* model_dev, 
         if (last) SCHK(hipMemsetAsync(c->frag_count, 0, 8, c->stream));
         SCHK(hipEventRecord(ev[0], c->stream));
         hipLaunchKernelGGL(k_splat_raster, dim3(cdiv(n, 256)), dim3(256), 0, c->stream, (const float4*)model_dev, n,
-                           U, c->keys, last ? c->frag_count : nullptr);
+                           U, c->rays, c->keys, last ? c->frag_count : nullptr);
         SCHK(hipEventRecord(ev[1], c->stream));
         hipLaunchKernelGGL(k_splat_resolve, dim3(cdiv(npix, 256)), dim3(256), 0, c->stream,
-                           (const float4*)model_dev, U, c->keys, c->image, c->vertex, c->normal, c->time);
+                           (const float4*)model_dev, U, c->rays, c->keys, c->image, c->vertex, c->normal, c->time);
         SCHK(hipEventRecord(ev[2], c->stream));
         if (fill_depth)
             hipLaunchKernelGGL(k_fill_in, dim3(cdiv(npix, 256)), dim3(256), 0, c->stream, A);

@@ -26,12 +26,20 @@ MAX_BLK = 256
 
 
 def short(name: str) -> str:
-    n = name.split("(")[0].replace("void ", "").replace("slam::", "").strip()
+    n = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("slam::", "").strip()
     return n
 
 
+def _stats_file(d: str) -> str:
+    """The kernel statistics of an output directory (``-o`` prefix any)."""
+    names = sorted(f for f in os.listdir(d) if f.endswith("_kernel_stats.csv"))
+    if not names:
+        raise FileNotFoundError(f"no *_kernel_stats.csv in {d}")
+    return os.path.join(d, "run_kernel_stats.csv" if "run_kernel_stats.csv" in names else names[0])
+
+
 def cmd_stats(d: str, top: int, cmd: str | None) -> str:
-    rows = list(csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))))
+    rows = list(csv.DictReader(open(_stats_file(d))))
     out = []
     if cmd:
         out.append(f"Command: `{cmd}`\n")
