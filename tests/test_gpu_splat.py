@@ -163,3 +163,18 @@ def test_prediction_feeds_the_odometry_on_the_device():
     assert np.array_equal(out[0], out[1])
     dt, dr = synth.pose_error(out[0], T1)
     assert dt < 1e-2 and dr < 0.5, (dt, dr)
+    # the prediction enqueued on the odometry's own stream: ordered by the
+    # stream alone, no host wait between the prediction and the init calls
+    odo = RGBDOdometry(*cam_args(cam)[:2], cam.cx, cam.cy, cam.fx, cam.fy)
+    im2 = splat.IndexMap(*cam_args(cam), max_host_surfels=model.shape[0])
+    im2.set_stream(odo.stream())
+    odo.initFirstRGB(f0.rgb)
+    im2.combinedPredict(T0.astype(np.float32), model, CUTOFF, 6.0, 20, 20, 15)
+    im2.fillIn(ref.bilateral(f0.depth, MAXD), f0.rgb)
+    odo.initICPModel(im2.device(splat.FILL_VERTEX), im2.device(splat.FILL_NORMAL), CUTOFF, T0.astype(np.float32))
+    odo.initRGBModel(im2.device(splat.FILL_IMAGE))
+    odo.initICP(filt1, CUTOFF)
+    odo.initRGB(f1.rgb)
+    t, R = odo.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 10.0, True, False, True)
+    assert np.array_equal(pose_of(t, R), out[0])
+    im2.set_stream(None)
