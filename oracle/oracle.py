@@ -100,6 +100,23 @@ def lib():
         L.orc_derivative.argtypes = [_V, ctypes.c_int, ctypes.c_int, _V, _V]
         L.orc_point_cloud.argtypes = [_V, ctypes.c_int, ctypes.c_int, Intr, _V]
         L.orc_splat_predict.argtypes = [_P_SPLAT, _V, _V, ctypes.c_int, _V, _V, _V, _V]
+        _I, _Fl = ctypes.c_int, ctypes.c_float
+        L.orc_model_create.restype = _V
+        L.orc_model_create.argtypes = [_I, _I, _Fl, _Fl, _Fl, _Fl, _I]
+        L.orc_model_destroy.argtypes = [_V]
+        L.orc_model_count.argtypes = [_V]
+        L.orc_model_fresh_count.argtypes = [_V]
+        L.orc_model_data.restype = _V
+        L.orc_model_data.argtypes = [_V]
+        L.orc_model_fresh.restype = _V
+        L.orc_model_fresh.argtypes = [_V]
+        L.orc_model_set.argtypes = [_V, _V, _I]
+        L.orc_model_index_map.argtypes = [_V, _V, _V, _V, _V]
+        L.orc_depth_metric.argtypes = [_V, _I, _Fl, _V]
+        L.orc_model_initialise.argtypes = [_V, _V, _V, _V, _Fl, _I, _Fl]
+        L.orc_model_predict_indices.argtypes = [_V, _V, _I, _Fl, _I]
+        L.orc_model_fuse.argtypes = [_V, _V, _I, _V, _V, _V, _Fl, _Fl, _Fl]
+        L.orc_model_clean.argtypes = [_V, _V, _I, _Fl, _I]
         L.orc_splat_fill_in.argtypes = [_P_SPLAT, _V, _V, _V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         _V, _V, _V]
         _lib = L
@@ -435,3 +452,72 @@ def splat_fill_in(width, height, cx, cy, fx, fy, image, vertex, normal, depth, r
     lib().orc_splat_fill_in(ctypes.byref(P), _p(im), _p(v), _p(n), _p(d), _p(r), r.shape[2], int(passthroughGeometry),
                             int(passthroughImage), _p(fi), _p(fv), _p(fn))
     return fi, fv, fn
+
+
+# ------------------------------------------------------------ surfel model
+
+def depth_metric(depth, maxD):
+    d = _c(depth, np.uint16)
+    out = np.empty(d.shape, np.float32)
+    lib().orc_depth_metric(_p(d), d.size, maxD, _p(out))
+    return out
+
+
+class SurfelModel:
+    """Model::initialise / fuse / clean and IndexMap::predictIndices restated
+    (model_oracle.c)."""
+
+    def __init__(self, width, height, cx, cy, fx, fy, max_surfels):
+        self.w, self.h = width, height
+        self._h = lib().orc_model_create(width, height, cx, cy, fx, fy, max_surfels)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_model_destroy(self._h)
+            self._h = None
+
+    def count(self):
+        return lib().orc_model_count(self._h)
+
+    def model(self):
+        n = self.count()
+        p = lib().orc_model_data(self._h)
+        return np.ctypeslib.as_array(ctypes.cast(p, _F), shape=(n * 12,)).reshape(n, 12).copy() if n else \
+            np.zeros((0, 12), np.float32)
+
+    def fresh(self):
+        n = lib().orc_model_fresh_count(self._h)
+        p = lib().orc_model_fresh(self._h)
+        return np.ctypeslib.as_array(ctypes.cast(p, _F), shape=(n * 12,)).reshape(n, 12).copy() if n else \
+            np.zeros((0, 12), np.float32)
+
+    def set(self, model):
+        m = _c(model, np.float32).reshape(-1, 12)
+        assert lib().orc_model_set(self._h, _p(m), m.shape[0]) == 0
+
+    set_model = set
+
+    def index_map(self):
+        idx = np.zeros((self.h, self.w), np.uint32)
+        vc = np.zeros((self.h, self.w, 4), np.float32)
+        ct = np.zeros((self.h, self.w, 4), np.float32)
+        nr = np.zeros((self.h, self.w, 4), np.float32)
+        lib().orc_model_index_map(self._h, _p(idx), _p(vc), _p(ct), _p(nr))
+        return idx, vc, ct, nr
+
+    def initialise(self, rgb, raw, filtered, maxD, time, maxDepth):
+        r, a, f = _c(rgb, np.uint8), _c(raw, np.uint16), _c(filtered, np.uint16)
+        return lib().orc_model_initialise(self._h, _p(r), _p(a), _p(f), maxD, time, maxDepth)
+
+    def predictIndices(self, pose, time, maxDepth, timeDelta):
+        T = _c(pose, np.float32).reshape(16)
+        lib().orc_model_predict_indices(self._h, _p(T), time, maxDepth, timeDelta)
+
+    def fuse(self, pose, time, rgb, raw, filtered, maxD, maxDepth, weighting):
+        T = _c(pose, np.float32).reshape(16)
+        r, a, f = _c(rgb, np.uint8), _c(raw, np.uint16), _c(filtered, np.uint16)
+        lib().orc_model_fuse(self._h, _p(T), time, _p(r), _p(a), _p(f), maxD, maxDepth, weighting)
+
+    def clean(self, pose, time, confThreshold, timeDelta):
+        T = _c(pose, np.float32).reshape(16)
+        return lib().orc_model_clean(self._h, _p(T), time, confThreshold, timeDelta)

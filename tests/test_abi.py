@@ -14,6 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "slam_odom.h")
 CNN_HEADER = os.path.join(ROOT, "include", "slam_cnn.h")
 SPLAT_HEADER = os.path.join(ROOT, "include", "slam_splat.h")
+SURFEL_HEADER = os.path.join(ROOT, "include", "slam_surfel.h")
 
 
 def declared(path=HEADER, prefix="odom_"):
@@ -48,8 +49,14 @@ def test_splat_header_and_binding_agree():
     assert declared(SPLAT_HEADER, "splat_") == set(splat.EXPORTS)
 
 
+def test_surfel_header_and_binding_agree():
+    from slam_amd import surfels
+    assert declared(SURFEL_HEADER, "surfel_") == set(surfels.EXPORTS)
+
+
 def test_library_exports_every_declared_symbol(L):
-    for name in declared() | declared(CNN_HEADER, "cnn_") | declared(SPLAT_HEADER, "splat_"):
+    for name in (declared() | declared(CNN_HEADER, "cnn_") | declared(SPLAT_HEADER, "splat_")
+                 | declared(SURFEL_HEADER, "surfel_")):
         assert hasattr(L, name), name
     # and they are real dynamic symbols of the shared object
     from slam_amd import _native
@@ -95,6 +102,11 @@ def test_splat_without_device_is_an_error(L):
     assert L.splat_destroy(None) == -1
     with pytest.raises(RuntimeError):
         splat.IndexMap(64, 48, 31.5, 23.5, 50.0, 50.0)
+    from slam_amd import surfels
+    surfels.lib()
+    assert L.surfel_create(64, 48, 31.5, 23.5, 50.0, 50.0, 1000, ctypes.byref(h)) == -4
+    assert L.surfel_create(64, 48, 31.5, 23.5, 50.0, 50.0, 1 << 23, ctypes.byref(h)) == -1
+    assert L.surfel_destroy(None) == -1
 
 
 def test_null_handles_are_rejected(L):

@@ -167,7 +167,7 @@ def test_prediction_feeds_the_odometry_on_the_device():
     # stream alone, no host wait between the prediction and the init calls
     odo = RGBDOdometry(*cam_args(cam)[:2], cam.cx, cam.cy, cam.fx, cam.fy)
     im2 = splat.IndexMap(*cam_args(cam), max_host_surfels=model.shape[0])
-    im2.set_stream(odo.stream())
+    im2.set_stream(odo.stream)
     odo.initFirstRGB(f0.rgb)
     im2.combinedPredict(T0.astype(np.float32), model, CUTOFF, 6.0, 20, 20, 15)
     im2.fillIn(ref.bilateral(f0.depth, MAXD), f0.rgb)
