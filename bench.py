@@ -244,6 +244,73 @@ def splat_prediction(args) -> dict:
     return res
 
 
+def surfel_fusion(args) -> dict:
+    """The model update that follows each tracked frame: at 640x480, a model
+    grown by fusing a sequence of one room (initialise, then per frame index
+    map, fuse, index map, clean with confidence 10 and timeDelta 200), then
+    one more frame's update timed with HIP events per stage (the same frame
+    fused into the same starting model, which is restored before each
+    repetition).  The update of that frame is checked bit for bit against the
+    oracle, whose time for it on one host core is the CPU baseline."""
+    import torch
+    from slam_amd import surfels, synth
+    from oracle import oracle as orc
+
+    cam = synth.CAM_640
+    scene = synth.Scene(910)
+    n = args.fusion_frames + 1
+    poses = synth.spline_trajectory(n, seed=910, max_step_m=0.02, max_step_deg=1.0)
+    T0i = np.linalg.inv(poses[0])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    frames = []
+    for k, T in enumerate(poses):
+        f = scene.render(T, cam, noise_seed=9100 + k, model=False)
+        frames.append(((T0i @ T).astype(np.float32), f, orc.bilateral(f.depth, 5.0)))
+    args_cam = (cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    P = dict(depthCutoff=5.0, maxDepth=20.0, conf=10.0, delta=200)
+    m = surfels.SurfelModel(*args_cam, 4_000_000)
+    t0 = time.time()
+    for k, (T, f, filt) in enumerate(frames[:-1]):
+        if k == 0:
+            m.initialise(dev(f.rgb), dev(f.depth), dev(filt), P["depthCutoff"], 1, P["maxDepth"])
+            continue
+        m.predictIndices(T, k + 1, P["maxDepth"], P["delta"])
+        m.fuse(T, k + 1, dev(f.rgb), dev(f.depth), dev(filt), P["depthCutoff"], P["maxDepth"], 1.0)
+        m.predictIndices(T, k + 1, P["maxDepth"], P["delta"])
+        m.clean(T, k + 1, P["conf"], P["delta"])
+    start = m.model()
+    log(f"fusion: model of {start.shape[0]} surfels from {n - 1} frames in {time.time() - t0:.1f}s")
+    T, f, filt = frames[-1]
+    tick = n
+    prof = m.profile(T, tick, dev(f.rgb), dev(f.depth), dev(filt), P["depthCutoff"], P["maxDepth"], 1.0, P["conf"],
+                     P["delta"], reps=args.fusion_reps)
+    total = sum(prof.values())
+    res = {
+        "workload": f"640x480 frame fused into a {start.shape[0]}-surfel model: predictIndices, fuse, "
+                    "predictIndices, clean (confidence 10, timeDelta 200)",
+        "surfels": int(start.shape[0]),
+        "frames_per_s": round(1e6 / total, 1),
+        "us": {k: round(v, 2) for k, v in prof.items()},
+    }
+    if not args.no_cpu:
+        got = m.model()
+        ref = orc.SurfelModel(*args_cam, 4_000_000)
+        ref.set(start)
+        t1 = time.time()
+        ref.predictIndices(T, tick, P["maxDepth"], P["delta"])
+        ref.fuse(T, tick, f.rgb, f.depth, filt, P["depthCutoff"], P["maxDepth"], 1.0)
+        ref.predictIndices(T, tick, P["maxDepth"], P["delta"])
+        ref.clean(T, tick, P["conf"], P["delta"])
+        cpu_s = time.time() - t1
+        exp = ref.model()
+        res["bit_exact_vs_oracle"] = bool(got.shape == exp.shape and np.array_equal(got.view(np.uint8),
+                                                                                   exp.view(np.uint8)))
+        res["cpu_baseline"] = {"value": round(1.0 / cpu_s, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": "the same frame's update of the same model, oracle/model_oracle.c (scalar)"}
+    log(f"fusion {res}")
+    return res
+
+
 def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain_est, ref64) -> dict | None:
     """The learned form on the same sequence: per frame the same init calls,
     then the network input from the handle's level-2 buffers, the network
@@ -395,6 +462,10 @@ def main() -> int:
                     help="also run 4 device copies of 256 MiB (the reference kernel of counter runs)")
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--no-splat", action="store_true", help="skip the splat prediction measurement")
+    ap.add_argument("--no-fusion", action="store_true", help="skip the surfel fusion measurement")
+    ap.add_argument("--fusion-only", action="store_true", help="measure only the surfel fusion (profiling)")
+    ap.add_argument("--fusion-frames", type=int, default=10)
+    ap.add_argument("--fusion-reps", type=int, default=20)
     ap.add_argument("--splat-only", action="store_true", help="measure only the splat prediction (profiling)")
     ap.add_argument("--splat-surfels", type=int, default=2_000_000)
     ap.add_argument("--splat-reps", type=int, default=50)
@@ -409,6 +480,9 @@ def main() -> int:
         return 2
     if args.splat_only:
         print(json.dumps({"splat": splat_prediction(args)}), flush=True)
+        return 0
+    if args.fusion_only:
+        print(json.dumps({"fusion": surfel_fusion(args)}), flush=True)
         return 0
 
     import torch
@@ -606,6 +680,7 @@ def main() -> int:
         learned = learned_vs_plain(args, cam, poses, views, L, h, odo, reset, fps, est, ref64)
     batched = None if args.no_batch else batched_pairs(args)
     splat_res = None if args.no_splat else splat_prediction(args)
+    fusion_res = None if args.no_fusion else surfel_fusion(args)
 
     line = {
         "metric": "frames/s 640x480 ICP+RGB at <=1mm vs FP64 oracle; learned-init vs plain",
@@ -634,6 +709,7 @@ def main() -> int:
         "learned": learned,
         "batched": batched,
         "splat": splat_res,
+        "fusion": fusion_res,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -91,6 +91,35 @@ __device__ __forceinline__ float fetch(const float* img, const Img& I, float u, 
     return img[(size_t)texel(v, I.h) * I.w + texel(u, I.w)];
 }
 
+// The texels the shaders' window loops, for (float s = lo; s < hi; s += step),
+// sample along one axis: a base texel and the repetitions of it and the three
+// after it (samples are half a texel apart, so the texels they hit rise by at
+// most one per step).  Visiting these texels once each, in ascending order,
+// with their repetitions as weights, gives what the loop gives: a repeated
+// sample sees the same texel.  false (walk the loop itself) if the loop
+// would run more than 64 times or reach a fifth texel.
+struct Runs {
+    int t0;
+    int m[4];
+};
+
+__device__ __forceinline__ bool window_runs(float lo, float hi, float step, int size, Runs& R)
+{
+    R.t0 = texel(lo, size);
+    R.m[0] = R.m[1] = R.m[2] = R.m[3] = 0;
+    int k = 0;
+    for (float s = lo; s < hi; s += step) {
+        if (++k > 64) return false;
+        const int r = texel(s, size) - R.t0;
+        if (r < 0 || r > 3) return false;
+        R.m[0] += r == 0;
+        R.m[1] += r == 1;
+        R.m[2] += r == 2;
+        R.m[3] += r == 3;
+    }
+    return true;
+}
+
 // geometry.glsl:21-25
 __device__ __forceinline__ void vertex_at(const float* depth, const Img& I, float u, float v, float x, float y,
                                           const Cam4& c, float* o)
@@ -497,26 +526,38 @@ __global__ __launch_bounds__(256) void k_data(DataArgs A)
         const float ray_len = sqrtf(dot3(ray, ray));
         const float u0 = u - ((scale * sx) * wm), u1 = u + ((scale * sx) * wm);
         const float v0 = v - ((scale * sy) * wm), v1 = v + ((scale * sy) * wm);
-        for (float ii = u0; ii < u1; ii += sx)
-            for (float jj = v0; jj < v1; jj += sy) {
-                const size_t t = (size_t)texel(jj, I.h) * I.w + texel(ii, I.w);
-                const uint32_t cur = A.index[t];
-                if (cur == 0) continue;
-                const float4 vc = A.vert_conf[t];
-                if (!(fabsf((vc.z * lambda) - (pl[2] * lambda)) < 0.05f)) continue;
-                const float cr[3] = {ray[1] * vc.z - ray[2] * vc.y, ray[2] * vc.x - ray[0] * vc.z,
-                                     ray[0] * vc.y - ray[1] * vc.x};
-                const float dist = sqrtf(dot3(cr, cr)) / ray_len;
-                const float4 nr4 = A.norm_rad[t];
-                const float nr[3] = {nr4.x, nr4.y, nr4.z};
-                // acos(c) < 0.5 as c > cos(0.5)
-                const float cosang = dot3(nr, nl) / (sqrtf(dot3(nr, nr)) * sqrtf(dot3(nl, nl)));
-                if (dist < best_dist && (fabsf(nr[2]) < 0.75f || cosang > 0.87758255f)) {
-                    counter++;
-                    best_dist = dist;
-                    best = cur;
-                }
+        // one candidate texel: the nearest ray wins, the first visited on ties
+        auto visit = [&](size_t t) {
+            const uint32_t cur = A.index[t];
+            if (cur == 0) return;
+            const float4 vc = A.vert_conf[t];
+            if (!(fabsf((vc.z * lambda) - (pl[2] * lambda)) < 0.05f)) return;
+            const float cr[3] = {ray[1] * vc.z - ray[2] * vc.y, ray[2] * vc.x - ray[0] * vc.z,
+                                 ray[0] * vc.y - ray[1] * vc.x};
+            const float dist = sqrtf(dot3(cr, cr)) / ray_len;
+            const float4 nr4 = A.norm_rad[t];
+            const float nr[3] = {nr4.x, nr4.y, nr4.z};
+            // acos(c) < 0.5 as c > cos(0.5)
+            const float cosang = dot3(nr, nl) / (sqrtf(dot3(nr, nr)) * sqrtf(dot3(nl, nl)));
+            if (dist < best_dist && (fabsf(nr[2]) < 0.75f || cosang > 0.87758255f)) {
+                counter++;
+                best_dist = dist;
+                best = cur;
             }
+        };
+        Runs rx, ry;
+        if (window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry)) {
+            // a repeated sample revisits a texel with the same distance, which
+            // never wins again: each texel once, in the loop's order
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (rx.m[a] && ry.m[b]) visit((size_t)(ry.t0 + b) * I.w + (rx.t0 + a));
+        } else {
+            for (float ii = u0; ii < u1; ii += sx)
+                for (float jj = v0; jj < v1; jj += sy) visit((size_t)texel(jj, I.h) * I.w + texel(ii, I.w));
+        }
         update = counter > 0 ? 1 : 2;
         o[7] = counter > 0 ? -1.0f : -2.0f;
     }
@@ -626,20 +667,30 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs A)
     if (tm - s[7] < A.timeDelta && lp[2] > 0.0f && x > 0.0f && y > 0.0f && x < I.cols && y < I.rows) {
         const float u0 = x / I.cols - ((scale * sx) * wm), u1 = x / I.cols + ((scale * sx) * wm);
         const float v0 = y / I.rows - ((scale * sy) * wm), v1 = y / I.rows + ((scale * sy) * wm);
-        for (float ii = u0; ii < u1; ii += sx)
-            for (float jj = v0; jj < v1; jj += sy) {
-                const size_t q = (size_t)texel(jj, I.h) * I.w + texel(ii, I.w);
-                if (A.index[q] == 0) continue;
-                const float4 vc = A.vert_conf[q];
-                const float4 ct = A.color_time[q];
-                const float dx = vc.x - lp[0], dy = vc.y - lp[1];
-                if (ct.z < s[6] && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] < 0.01f &&
-                    sqrtf(dx * dx + dy * dy) < s[11] * 1.4f)
-                    count++;
-                if (ct.w == tm && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] > 0.01f &&
-                    fabsf(ln[2]) > 0.85f)
-                    zcount++;
-            }
+        // one texel, counted as often as the loop samples it
+        auto visit = [&](size_t q, int times) {
+            if (A.index[q] == 0) return;
+            const float4 vc = A.vert_conf[q];
+            const float4 ct = A.color_time[q];
+            const float dx = vc.x - lp[0], dy = vc.y - lp[1];
+            if (ct.z < s[6] && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] < 0.01f &&
+                sqrtf(dx * dx + dy * dy) < s[11] * 1.4f)
+                count += times;
+            if (ct.w == tm && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] > 0.01f &&
+                fabsf(ln[2]) > 0.85f)
+                zcount += times;
+        };
+        Runs rx, ry;
+        if (window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry)) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (rx.m[a] && ry.m[b]) visit((size_t)(ry.t0 + b) * I.w + (rx.t0 + a), rx.m[a] * ry.m[b]);
+        } else {
+            for (float ii = u0; ii < u1; ii += sx)
+                for (float jj = v0; jj < v1; jj += sy) visit((size_t)texel(jj, I.h) * I.w + texel(ii, I.w), 1);
+        }
     }
     int test = 1;
     if (count > 8 || zcount > 4) test = 0;
