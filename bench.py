@@ -311,6 +311,89 @@ def surfel_fusion(args) -> dict:
     return res
 
 
+def frame_loop_c5(args) -> dict:
+    """The reference's frame loop without loop closure on the device
+    (slam_amd/frame_loop.py): per frame the bilateral filter, frame-to-model
+    ICP+RGB+SO3 tracking against the filled-in splat prediction of the surfel
+    model, the fusion and cleaning, and the next prediction, with the
+    reference configs' values (depth cutoff 3 m, confidence 10, timeDelta
+    200).  A 640x480 sequence of one room with bursts up to 10 cm / 8 deg per
+    frame, frames resident on the device; reports the whole loop's frames/s
+    (host reads of the pose and the surfel count included, as the reference
+    loop makes them), the model size, and ATE / RPE against the generated
+    ground truth."""
+    import torch
+    from slam_amd import evaluate, synth
+    from slam_amd.frame_loop import FrameLoop
+
+    cam = synth.CAM_640
+    n = args.c5_frames
+    poses = synth.spline_trajectory(n, seed=920, max_step_m=0.10, max_step_deg=8.0, bursts=True)
+    T0i = np.linalg.inv(poses[0])
+    gt = [T0i @ T for T in poses]
+    scene = synth.Scene(920)
+    t0 = time.time()
+    frames = []
+    for k, T in enumerate(poses):
+        f = scene.render(T, cam, noise_seed=92000 + k, model=False)
+        frames.append((torch.from_numpy(f.rgb).cuda(), torch.from_numpy(f.depth).cuda()))
+        if (k + 1) % 500 == 0:
+            log(f"c5: rendered {k + 1}/{n} frames")
+    torch.cuda.synchronize()
+    log(f"c5: {n} frames resident in {time.time() - t0:.1f}s")
+    steps = [np.linalg.inv(gt[k]) @ gt[k + 1] for k in range(n - 1)]
+    from slam_amd.frame_loop import LoopParams
+
+    def run(learned: bool) -> dict:
+        loop = FrameLoop(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, LoopParams(learned=learned))
+        warm = min(5, n)
+        for k in range(warm):
+            loop.process(*frames[k])
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for k in range(warm, n):
+            loop.process(*frames[k])
+        torch.cuda.synchronize()
+        el = time.time() - t1
+        stamps = [float(k) for k in range(n)]
+        gtt = evaluate.trajectory(stamps, gt)
+        est = evaluate.trajectory(stamps, [T.astype(np.float64) for T in loop.poses])
+        ate = evaluate.ate(gtt, est)
+        rpe = evaluate.rpe(gtt, est, delta=1, delta_unit="f")
+        M = loop.model.model()
+        st = loop.odo.stats()
+        r = {
+            "frames_per_s": round((n - warm) / el, 1),
+            "ms_per_frame": round(1e3 * el / (n - warm), 3),
+            "surfels_final": int(M.shape[0]),
+            "surfels_stable": int((M[:, 3] > 10.0).sum()),
+            "ate_rmse_mm": round(1e3 * ate["rmse"], 2),
+            "ate_max_mm": round(1e3 * ate["max"], 2),
+            "rpe_rmse_mm": round(1e3 * rpe["translational"]["rmse"], 3),
+            "rpe_rmse_deg": round(rpe["rotational_deg"]["rmse"], 4),
+            "frames_lost_gt_5cm": int(sum(np.linalg.norm((np.linalg.inv(g) @ e.astype(np.float64))[:3, 3]) > 0.05
+                                          for g, e in zip(gt, loop.poses))),
+        }
+        lost = [k for k, (g, e) in enumerate(zip(gt, loop.poses))
+                if np.linalg.norm((np.linalg.inv(g) @ e.astype(np.float64))[:3, 3]) > 0.05]
+        r["first_lost_frame"] = lost[0] if lost else None
+        if learned:
+            r["prediction_kept_frac"] = round(st["seedKept"] / max(1, st["seedCalls"]), 3)
+        return r
+
+    res = {
+        "workload": f"{n}-frame 640x480 frame loop (track, fuse, clean, predict), bursts up to "
+                    f"{1e3 * max(np.linalg.norm(s[:3, 3]) for s in steps):.0f} mm / "
+                    f"{max(np.degrees(np.arccos(np.clip((np.trace(s[:3, :3]) - 1) / 2, -1, 1))) for s in steps):.1f}"
+                    " deg per frame, no loop closure",
+        "frames": n,
+        "plain": run(False),
+        "learned": run(True) if os.path.exists(os.path.join(ROOT, "slam_amd", "weights", "warmstart.npz")) else None,
+    }
+    log(f"c5 {res}")
+    return res
+
+
 def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain_est, ref64) -> dict | None:
     """The learned form on the same sequence: per frame the same init calls,
     then the network input from the handle's level-2 buffers, the network
@@ -463,6 +546,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--no-splat", action="store_true", help="skip the splat prediction measurement")
     ap.add_argument("--no-fusion", action="store_true", help="skip the surfel fusion measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the full frame-loop sequence")
+    ap.add_argument("--c5-only", action="store_true", help="run only the full frame-loop sequence")
+    ap.add_argument("--c5-frames", type=int, default=2000)
     ap.add_argument("--fusion-only", action="store_true", help="measure only the surfel fusion (profiling)")
     ap.add_argument("--fusion-frames", type=int, default=10)
     ap.add_argument("--fusion-reps", type=int, default=20)
@@ -483,6 +569,9 @@ def main() -> int:
         return 0
     if args.fusion_only:
         print(json.dumps({"fusion": surfel_fusion(args)}), flush=True)
+        return 0
+    if args.c5_only:
+        print(json.dumps({"frame_loop": frame_loop_c5(args)}), flush=True)
         return 0
 
     import torch
@@ -681,6 +770,7 @@ def main() -> int:
     batched = None if args.no_batch else batched_pairs(args)
     splat_res = None if args.no_splat else splat_prediction(args)
     fusion_res = None if args.no_fusion else surfel_fusion(args)
+    c5_res = None if args.no_c5 else frame_loop_c5(args)
 
     line = {
         "metric": "frames/s 640x480 ICP+RGB at <=1mm vs FP64 oracle; learned-init vs plain",
@@ -710,6 +800,7 @@ def main() -> int:
         "batched": batched,
         "splat": splat_res,
         "fusion": fusion_res,
+        "frame_loop": c5_res,
     }
     print(json.dumps(line), flush=True)
     return 0

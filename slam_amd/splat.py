@@ -54,7 +54,12 @@ def lib() -> ctypes.CDLL:
 
 
 def _model_arg(model):
-    """(pointer, count, memory kind, keep-alive) of an (n, 12) float32 model."""
+    """(pointer, count, memory kind, keep-alive) of an (n, 12) float32 model:
+    a numpy array (host), a torch CUDA tensor, or a (device pointer, count)
+    pair such as ``SurfelModel.device_model()``."""
+    if isinstance(model, tuple):
+        ptr, n = model
+        return int(ptr), int(n), MEM_DEVICE, None
     try:
         import torch
         if isinstance(model, torch.Tensor) and model.is_cuda:
