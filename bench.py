@@ -38,6 +38,11 @@ ICP_WEIGHT = 10.0
 # algorithmic bytes per pixel and iteration (unique data each pass must touch)
 BYTES_ICP = 48   # v_c, n_c, v_p, n_p: 4 x 3 FP32
 BYTES_RGB = 14   # I_next 1, dIdx 2, dIdy 2, z_next 4, z_last 4, I_last 1
+# k_step_a computes the ICP products and the photometric association, so it
+# reads the ICP bytes plus the association's z_next, I_next, z_last, I_last;
+# k_step_b then reads the gradients (the rest of the RGB bytes)
+BYTES_A = BYTES_ICP + 10
+BYTES_B = BYTES_RGB - 10
 
 
 def log(msg: str) -> None:
@@ -144,10 +149,10 @@ def batched_pairs(args) -> dict:
     us_a = [float(np.mean([p["us_a"][l] for p in prof])) for l in range(3)]
     us_b = [float(np.mean([p["us_b"][l] for p in prof])) for l in range(3)]
     launches = sum(iters)
-    bytes_a = sum(iters[l] * BYTES_ICP * npx[l] for l in range(3)) / launches
+    bytes_a = sum(iters[l] * BYTES_A * npx[l] for l in range(3)) / launches
     avg_a = sum(iters[l] * us_a[l] for l in range(3)) / launches
     ach = bytes_a / (avg_a * 1e-6) / 1e9
-    l0_gbs = BYTES_ICP * npx[0] / (us_a[0] * 1e-6) / 1e9
+    l0_gbs = BYTES_A * npx[0] / (us_a[0] * 1e-6) / 1e9
     joint = sum(iters[l] * (BYTES_ICP + BYTES_RGB) * npx[l] for l in range(3)) / \
         (sum(iters[l] * (us_a[l] + us_b[l]) for l in range(3)) * 1e-6) / 1e9
     out = {"workload": "64 pairs: 16x320x240 + 32x640x480 + 16x848x480, <=3cm/2deg, ICP+RGB+SO3 10/5/4, "
@@ -703,7 +708,7 @@ def main() -> int:
     track_us = float(np.mean([p["us_total"] for p in profs]))
     tot_t = {k: sum(iters[l] * us[k][l] for l in range(3)) for k in (0, 1)}
     dom = 0 if tot_t[0] >= tot_t[1] else 1
-    bpp = BYTES_ICP if dom == 0 else BYTES_RGB
+    bpp = BYTES_A if dom == 0 else BYTES_B
     launches = sum(iters)
     bytes_per_launch = sum(iters[l] * bpp * npx[l] for l in range(3)) / launches
     avg_us = tot_t[dom] / launches
