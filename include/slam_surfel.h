@@ -78,7 +78,9 @@ int surfel_read_fresh(surfel_handle h, float *host_dst, int max_points, int *cou
 /* Measurement: microseconds per call of predict_indices, fuse and clean
  * (HIP events on the handle's stream around one frame's sequence
  * predict_indices -> fuse -> predict_indices -> clean, reps times from the
- * same starting model, which is restored before each rep). */
+ * same starting model, which is restored before each rep; afterwards the
+ * model is the one update's result).  Allocates a copy of the starting model
+ * for the call's duration. */
 int surfel_profile(surfel_handle h, const float pose[16], int time, const odom_frame_view *rgb,
                    const odom_frame_view *raw_depth, const odom_frame_view *filtered_depth, float depth_cutoff,
                    float max_depth, float weighting, float conf_threshold, int time_delta, int reps, float us[4]);

@@ -178,3 +178,21 @@ def test_prediction_feeds_the_odometry_on_the_device():
     t, R = odo.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 10.0, True, False, True)
     assert np.array_equal(pose_of(t, R), out[0])
     im2.set_stream(None)
+
+
+def test_bad_arguments_raise():
+    from slam_amd.odometry import OdomError
+    cam = synth.CAM_320
+    im = splat.IndexMap(*cam_args(cam), max_host_surfels=10)
+    pose = np.eye(4, dtype=np.float32)
+    with pytest.raises(OdomError):                       # more surfels than the host staging holds
+        im.combinedPredict(pose, np.zeros((11, 12), np.float32), CUTOFF, 0.0, 0, 0, 100)
+    with pytest.raises(ValueError):
+        im.combinedPredict(pose, np.zeros((5, 11), np.float32), CUTOFF, 0.0, 0, 0, 100)
+    with pytest.raises(OdomError):                       # frame of another size
+        im.fillIn(np.zeros((10, 10), np.uint16), np.zeros((10, 10, 3), np.uint8))
+    with pytest.raises(OdomError):
+        splat.IndexMap(0, 10, 1.0, 1.0, 1.0, 1.0)
+    # an empty model is a valid prediction (all outputs cleared)
+    im.combinedPredict(pose, np.zeros((0, 12), np.float32), CUTOFF, 0.0, 0, 0, 100)
+    assert not im.read(splat.VERTEX).any()

@@ -130,3 +130,23 @@ def test_fused_model_feeds_the_splat_prediction(orc):
     assert same_bits(im.read(splat.VERTEX), exp[1])
     assert (exp[1][..., 2] > 0).mean() > 0.5
     assert ptr != 0 and n == model_host.shape[0]
+
+
+def test_bad_arguments_raise():
+    from slam_amd.odometry import OdomError
+    cam = synth.CAM_320
+    args = (cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    m = surfels.SurfelModel(*args, 1000)
+    with pytest.raises(OdomError):                       # more surfels than the model holds
+        m.set_model(np.zeros((1001, 12), np.float32))
+    with pytest.raises(OdomError):                       # frame of another size
+        m.initialise(np.zeros((10, 10, 3), np.uint8), np.zeros((10, 10), np.uint16),
+                     np.zeros((10, 10), np.uint16), DEPTH_CUTOFF, 1, MAX_DEPTH)
+    with pytest.raises(OdomError):                       # max_surfels + pixels beyond 2^23
+        surfels.SurfelModel(*args, 1 << 23)
+    # an empty model: the index map is empty and cleaning keeps nothing
+    m.set_model(np.zeros((0, 12), np.float32))
+    T = np.eye(4, dtype=np.float32)
+    m.predictIndices(T, 1, MAX_DEPTH, DELTA)
+    assert not m.index_map()[0].any()
+    assert m.clean(T, 1, CONF, DELTA) == 0
