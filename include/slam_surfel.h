@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-/* index map outputs for surfel_read_index_map */
+/* The model, its index map and the fusion scratch on one device. */
 typedef struct surfel_ctx *surfel_handle;
 
 /* The model buffers (two of max_surfels surfels, Model.cpp:26-80), the index
@@ -38,7 +38,7 @@ int surfel_destroy(surfel_handle h);
 int surfel_set_stream(surfel_handle h, void *stream);
 
 /* Model::initialise(FeedbackBuffer raw, filtered) of the first frame
- * (elastic_fusion loop, first tick): a surfel per pixel with depth in
+ * (elastic_fusion_file.cpp:353-360): a surfel per pixel with depth in
  * (0, max_depth], in the camera frame.  Synchronous (the count is read
  * back, as glGetQueryObjectuiv does). */
 int surfel_initialise(surfel_handle h, const odom_frame_view *rgb, const odom_frame_view *raw_depth,
