@@ -546,9 +546,43 @@ __global__ __launch_bounds__(256) void k_data(DataArgs A)
             }
         };
         Runs rx, ry;
-        if (window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry)) {
-            // a repeated sample revisits a texel with the same distance, which
-            // never wins again: each texel once, in the loop's order
+        const bool runs = window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry);
+        // a repeated sample revisits a texel with the same distance, which
+        // never wins again: each texel once, in the loop's order
+        if (runs && !rx.m[3] && !ry.m[3]) {
+            // the usual 3 x 3 texels: every load issued before the first use
+            // (unused slots read a clamped, valid texel and are skipped)
+            uint32_t id[9];
+            float4 vcs[9], nrs[9];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const size_t q = (size_t)min(ry.t0 + b, I.h - 1) * I.w + min(rx.t0 + a, I.w - 1);
+                    id[3 * a + b] = A.index[q];
+                    vcs[3 * a + b] = A.vert_conf[q];
+                    nrs[3 * a + b] = A.norm_rad[q];
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const uint32_t cur = id[3 * a + b];
+                    if (!rx.m[a] || !ry.m[b] || cur == 0) continue;
+                    const float4 vc = vcs[3 * a + b];
+                    if (!(fabsf((vc.z * lambda) - (pl[2] * lambda)) < 0.05f)) continue;
+                    const float cr[3] = {ray[1] * vc.z - ray[2] * vc.y, ray[2] * vc.x - ray[0] * vc.z,
+                                         ray[0] * vc.y - ray[1] * vc.x};
+                    const float dist = sqrtf(dot3(cr, cr)) / ray_len;
+                    const float nr[3] = {nrs[3 * a + b].x, nrs[3 * a + b].y, nrs[3 * a + b].z};
+                    const float cosang = dot3(nr, nl) / (sqrtf(dot3(nr, nr)) * sqrtf(dot3(nl, nl)));
+                    if (dist < best_dist && (fabsf(nr[2]) < 0.75f || cosang > 0.87758255f)) {
+                        counter++;
+                        best_dist = dist;
+                        best = cur;
+                    }
+                }
+        } else if (runs) {
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -681,7 +715,37 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs A)
                 zcount += times;
         };
         Runs rx, ry;
-        if (window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry)) {
+        const bool runs = window_runs(u0, u1, sx, I.w, rx) && window_runs(v0, v1, sy, I.h, ry);
+        if (runs && !rx.m[3] && !ry.m[3]) {
+            // the usual 3 x 3 texels: every load issued before the first use
+            // (unused slots read a clamped, valid texel and weigh 0)
+            uint32_t id[9];
+            float4 vcs[9], cts[9];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const size_t q = (size_t)min(ry.t0 + b, I.h - 1) * I.w + min(rx.t0 + a, I.w - 1);
+                    id[3 * a + b] = A.index[q];
+                    vcs[3 * a + b] = A.vert_conf[q];
+                    cts[3 * a + b] = A.color_time[q];
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const int times = rx.m[a] * ry.m[b];
+                    if (!times || id[3 * a + b] == 0) continue;
+                    const float4 vc = vcs[3 * a + b], ct = cts[3 * a + b];
+                    const float dx = vc.x - lp[0], dy = vc.y - lp[1];
+                    if (ct.z < s[6] && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] < 0.01f &&
+                        sqrtf(dx * dx + dy * dy) < s[11] * 1.4f)
+                        count += times;
+                    if (ct.w == tm && vc.w > A.confThreshold && vc.z > lp[2] && vc.z - lp[2] > 0.01f &&
+                        fabsf(ln[2]) > 0.85f)
+                        zcount += times;
+                }
+        } else if (runs) {
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
