@@ -3,12 +3,14 @@ filter, frame-to-model tracking against the filled-in splat prediction, the
 model update, and the next prediction (elastic_fusion_file.cpp:335-420,
 551-602, with closeLoops off).
 
-Every stage runs through the C ABI on one HIP stream (the odometry's); the
-host reads back only what the reference's loop reads: the tracked pose (for
-the fusion weighting and the next prediction) and the model's surfel count.
-The prediction that the reference makes right after tracking
-(elastic_fusion_file.cpp:399) feeds only the loop-closure search, so it is
-not made here; the one at the end of the frame (:601) is.
+``FrameLoop`` drives the native loop of ``include/slam_loop.h``
+(``efloop_process``, C++ over the odometry, splat and surfel entry points);
+with the learned warm start it runs the same steps from Python, adding the
+network call.  Every stage runs on one HIP stream (the odometry's); the host
+reads back only what the reference's loop reads: the tracked pose and the
+model's surfel count.  The prediction that the reference makes right after
+tracking (elastic_fusion_file.cpp:399) feeds only the loop-closure search, so
+it is not made; the one at the end of the frame (:601) is.
 """
 from __future__ import annotations
 
@@ -18,7 +20,74 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import splat, surfels
-from .odometry import FMT_DEPTH_U16, MEM_DEVICE, RGBDOdometry, _check, frame_view, lib as odom_lib
+from .odometry import (FMT_DEPTH_U16, FMT_RGB8, MEM_DEVICE, RGBDOdometry, Stats, _check, _p, frame_view,
+                       lib as odom_lib)
+
+# Every symbol include/slam_loop.h declares.
+EXPORTS = ["efloop_default_params", "efloop_create", "efloop_destroy", "efloop_process", "efloop_get_handles",
+           "efloop_get_state"]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("depth_cutoff", ctypes.c_float), ("max_depth", ctypes.c_float), ("confidence", ctypes.c_float),
+                ("time_delta", ctypes.c_int), ("icp_weight", ctypes.c_float), ("so3", ctypes.c_int),
+                ("max_surfels", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+def lib() -> ctypes.CDLL:
+    L = surfels.lib()
+    splat.lib()
+    if getattr(L, "_loop_typed", False):
+        return L
+    V, P = ctypes.c_void_p, ctypes.POINTER
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    L.efloop_default_params.argtypes = [P(_Params)]
+    L.efloop_create.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4 + [P(_Params), P(V)]
+    L.efloop_destroy.argtypes = [V]
+    L.efloop_process.argtypes = [V, P(odom_frame_view_type()), P(odom_frame_view_type()), V]
+    L.efloop_get_handles.argtypes = [V, P(V), P(V), P(V)]
+    L.efloop_get_state.argtypes = [V, P(ctypes.c_int), P(ctypes.c_float), P(ctypes.c_float)]
+    L._loop_typed = True
+    return L
+
+
+def odom_frame_view_type():
+    from .odometry import FrameView
+    return FrameView
+
+
+class _OdoView:
+    """The loop's odometry handle (owned by the loop): statistics."""
+
+    def __init__(self, h):
+        self._h = h
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check("odom_get_stats", odom_lib().odom_get_stats(self._h, ctypes.byref(s)))
+        return {"lastICPError": s.last_icp_error, "lastICPCount": s.last_icp_count,
+                "lastRGBError": s.last_rgb_error, "lastRGBCount": s.last_rgb_count,
+                "seedCalls": s.seed_calls, "seedKept": s.seed_kept}
+
+
+class _ModelView:
+    """The loop's surfel model (owned by the loop): count and a host copy."""
+
+    def __init__(self, h):
+        self._h = h
+
+    def lastCount(self) -> int:
+        n = ctypes.c_int(0)
+        _check("surfel_count", surfels.lib().surfel_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def model(self) -> np.ndarray:
+        n = self.lastCount()
+        out = np.zeros((max(n, 1), 12), np.float32)
+        got = ctypes.c_int(0)
+        _check("surfel_read_model", surfels.lib().surfel_read_model(self._h, _p(out), n, ctypes.byref(got)))
+        return out[:n]
 
 
 @dataclass
@@ -31,6 +100,7 @@ class LoopParams:
     icp_weight: float = 10.0
     so3: bool = True
     max_surfels: int = 4_000_000
+    use_graph: bool = True           # replay the tracking call from a hipGraph (native loop)
     learned: bool = False            # seed each call from the warm-start network (no SO3 pass then)
     learned_iters: tuple = (10, 5, 4)
 
@@ -41,7 +111,56 @@ def _rodrigues_norm(R: np.ndarray) -> float:
 
 
 class FrameLoop:
-    """Processes frames (uint16 depth in mm, RGB8) held in device memory."""
+    """Processes frames (uint16 depth in mm, RGB8), host or device."""
+
+    def __new__(cls, width, height, cx, cy, fx, fy, params: LoopParams | None = None):
+        if params is not None and params.learned:
+            return super().__new__(_PythonFrameLoop)
+        return super().__new__(cls)
+
+    def __init__(self, width: int, height: int, cx: float, cy: float, fx: float, fy: float,
+                 params: LoopParams | None = None):
+        self.p = params or LoopParams()
+        P = _Params()
+        _check("efloop_default_params", lib().efloop_default_params(ctypes.byref(P)))
+        P.depth_cutoff, P.max_depth, P.confidence = self.p.depth_cutoff, self.p.max_depth, self.p.confidence
+        P.time_delta, P.icp_weight, P.so3 = self.p.time_delta, self.p.icp_weight, int(self.p.so3)
+        P.max_surfels = self.p.max_surfels
+        P.use_graph = int(self.p.use_graph)
+        h = ctypes.c_void_p()
+        _check("efloop_create", lib().efloop_create(width, height, cx, cy, fx, fy, ctypes.byref(P), ctypes.byref(h)))
+        self._h = h
+        o, m = ctypes.c_void_p(), ctypes.c_void_p()
+        _check("efloop_get_handles", lib().efloop_get_handles(h, ctypes.byref(o), None, ctypes.byref(m)))
+        self.odo = _OdoView(o)
+        self.model = _ModelView(m)
+        self.poses: list[np.ndarray] = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().efloop_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+    def process(self, rgb, depth) -> np.ndarray:
+        """One frame; returns the tracked camera-to-model pose (4x4)."""
+        r, kr = frame_view(rgb, FMT_RGB8)
+        d, kd = frame_view(depth, FMT_DEPTH_U16)
+        pose = np.zeros(16, np.float32)
+        _check("efloop_process", lib().efloop_process(self._h, ctypes.byref(r), ctypes.byref(d), _p(pose)))
+        T = pose.reshape(4, 4)
+        self.poses.append(T.copy())
+        return T
+
+
+class _PythonFrameLoop(FrameLoop):
+    """The same loop driven from Python, with the learned warm start seeding
+    each tracking call (no SO3 pass then)."""
 
     def __init__(self, width: int, height: int, cx: float, cy: float, fx: float, fy: float,
                  params: LoopParams | None = None):
@@ -58,6 +177,7 @@ class FrameLoop:
         self.pose = np.eye(4, dtype=np.float32)
         self.odo.set_pose(self.pose[:3, 3], self.pose[:3, :3])
         self.net = None
+        self._h = None
         if self.p.learned:
             from . import warmstart as ws
             self.ws = ws

@@ -15,6 +15,7 @@ HEADER = os.path.join(ROOT, "include", "slam_odom.h")
 CNN_HEADER = os.path.join(ROOT, "include", "slam_cnn.h")
 SPLAT_HEADER = os.path.join(ROOT, "include", "slam_splat.h")
 SURFEL_HEADER = os.path.join(ROOT, "include", "slam_surfel.h")
+LOOP_HEADER = os.path.join(ROOT, "include", "slam_loop.h")
 
 
 def declared(path=HEADER, prefix="odom_"):
@@ -54,9 +55,14 @@ def test_surfel_header_and_binding_agree():
     assert declared(SURFEL_HEADER, "surfel_") == set(surfels.EXPORTS)
 
 
+def test_loop_header_and_binding_agree():
+    from slam_amd import frame_loop
+    assert declared(LOOP_HEADER, "efloop_") == set(frame_loop.EXPORTS)
+
+
 def test_library_exports_every_declared_symbol(L):
     for name in (declared() | declared(CNN_HEADER, "cnn_") | declared(SPLAT_HEADER, "splat_")
-                 | declared(SURFEL_HEADER, "surfel_")):
+                 | declared(SURFEL_HEADER, "surfel_") | declared(LOOP_HEADER, "efloop_")):
         assert hasattr(L, name), name
     # and they are real dynamic symbols of the shared object
     from slam_amd import _native
@@ -107,6 +113,13 @@ def test_splat_without_device_is_an_error(L):
     assert L.surfel_create(64, 48, 31.5, 23.5, 50.0, 50.0, 1000, ctypes.byref(h)) == -4
     assert L.surfel_create(64, 48, 31.5, 23.5, 50.0, 50.0, 1 << 23, ctypes.byref(h)) == -1
     assert L.surfel_destroy(None) == -1
+    from slam_amd import frame_loop
+    frame_loop.lib()
+    P = frame_loop._Params()
+    assert L.efloop_default_params(ctypes.byref(P)) == 0
+    assert (P.depth_cutoff, P.max_depth, P.confidence, P.time_delta) == (3.0, 20.0, 10.0, 200)   # ef_file.cfg
+    assert L.efloop_create(64, 48, 31.5, 23.5, 50.0, 50.0, ctypes.byref(P), ctypes.byref(h)) == -4
+    assert L.efloop_destroy(None) == -1
 
 
 def test_null_handles_are_rejected(L):
