@@ -14,8 +14,11 @@
 //   - transform feedback, which appends in draw order: flags, an exclusive
 //     scan in draw order, and a scatter.
 // Draw order is the reference's: pixels column by column (its uv buffers),
-// surfels by index.  Per-pixel kernels run over row-major pixels (coalesced
-// reads) and key their results by the draw index d = x * height + y.
+// surfels by index.  The per-pixel maps of this file (metric depth, index map,
+// per-pixel fusion data) are stored in that order too, at d = x * height + y:
+// per-pixel kernels run over d, and a wave of surfels, which the model holds
+// mostly in draw order, reads its neighbourhoods from a few contiguous runs
+// instead of one cache line per lane.
 //
 // Arithmetic is FP32 with -ffp-contract=off in the shaders' operation order.
 #include <hip/hip_runtime.h>
@@ -23,6 +26,8 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
+
+#include <vector>
 
 #include "odom_math.h"
 #include "slam_surfel.h"
@@ -86,9 +91,12 @@ __device__ __forceinline__ int texel(float u, int n)
     return !(t >= 0.0f) ? 0 : (t > (float)(n - 1) ? n - 1 : (int)t);
 }
 
+// Address of pixel (x, y) in a map stored in draw order.
+__device__ __forceinline__ size_t cm(const Img& I, int x, int y) { return (size_t)x * I.h + y; }
+
 __device__ __forceinline__ float fetch(const float* img, const Img& I, float u, float v)
 {
-    return img[(size_t)texel(v, I.h) * I.w + texel(u, I.w)];
+    return img[cm(I, texel(u, I.w), texel(v, I.h))];
 }
 
 // The texels the shaders' window loops, for (float s = lo; s < hi; s += step),
@@ -286,14 +294,14 @@ __device__ __forceinline__ int scan_pos(const int* prefix, const int* block_sums
 __global__ __launch_bounds__(256) void k_metric2(const uint8_t* raw, int rpitch, const uint8_t* filt, int fpitch,
                                                  float maxD, Img I, float* dr, float* df)
 {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= I.w * I.h) return;
-    const int x = p % I.w, y = p / I.w;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= I.w * I.h) return;
+    const int x = d / I.h, y = d - x * I.h;
     const unsigned lim = (unsigned)(maxD * 1000.0f);
     const unsigned a = *reinterpret_cast<const uint16_t*>(raw + (size_t)y * rpitch + 2 * (size_t)x);
     const unsigned b = *reinterpret_cast<const uint16_t*>(filt + (size_t)y * fpitch + 2 * (size_t)x);
-    dr[p] = (a > lim || a < 300u) ? 0.0f : (float)a / 1000.0f;
-    df[p] = (b > lim || b < 300u) ? 0.0f : (float)b / 1000.0f;
+    dr[d] = (a > lim || a < 300u) ? 0.0f : (float)a / 1000.0f;
+    df[d] = (b > lim || b < 300u) ? 0.0f : (float)b / 1000.0f;
 }
 
 // ---------------------------------------------------------- initialise
@@ -314,10 +322,9 @@ __global__ __launch_bounds__(256) void k_feedback(const float* dr, const float* 
                                                   int time, float maxDepth, float* raw_items, float* filt_norm,
                                                   int* flag_raw, int* flag_filt)
 {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= I.w * I.h) return;
-    const int i = p % I.w, j = p / I.w;
-    const int d = i * I.h + j;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= I.w * I.h) return;
+    const int i = d / I.h, j = d - i * I.h;
     const float u = pixel_tc(i, I.w), v = pixel_tc(j, I.h);
     const float x = u * I.cols, y = v * I.rows;
     const uint8_t* col = rgb_at(rgb, I, u, v);
@@ -333,10 +340,10 @@ __global__ __launch_bounds__(256) void k_feedback(const float* dr, const float* 
             // init_unstable.vert: colour y = 0, init time z = 1
             const float o[12] = {vp[0], vp[1], vp[2], confidence_of(x, y, 1.0f, c), encode_rgb8(col), 0.0f, 1.0f,
                                  (float)time, 0.0f, 0.0f, 0.0f, 0.0f};
-            store12(raw_items + 12 * (size_t)p, o);
+            store12(raw_items + 12 * (size_t)d, o);
         } else {
             flag_filt[d] = ok ? 1 : 0;
-            reinterpret_cast<float4*>(filt_norm)[p] = make_float4(n[0], n[1], n[2], rad);
+            reinterpret_cast<float4*>(filt_norm)[d] = make_float4(n[0], n[1], n[2], rad);
         }
     }
 }
@@ -348,14 +355,12 @@ __global__ __launch_bounds__(256) void k_init_scatter(const float* raw_items, co
 {
     const int d = blockIdx.x * 256 + threadIdx.x;
     if (d >= I.w * I.h) return;
-    const int i = d / I.h, j = d - i * I.h;
-    const int p = j * I.w + i;
     const int nr = min(*total_raw, max_surfels);
     if (flag_raw[d]) {
         const int k = scan_pos(pre_raw, bs_raw, d);
         if (k < nr) {
             float o[12];
-            load12(raw_items + 12 * (size_t)p, o);
+            load12(raw_items + 12 * (size_t)d, o);
             // normal and radius: the k-th filtered vertex's, zero if there is none
             o[8] = o[9] = o[10] = o[11] = 0.0f;
             store12(model + 12 * (size_t)k, o);
@@ -372,11 +377,9 @@ __global__ __launch_bounds__(256) void k_init_normals(const float* filt_norm, co
 {
     const int d = blockIdx.x * 256 + threadIdx.x;
     if (d >= I.w * I.h || !flag_filt[d]) return;
-    const int i = d / I.h, j = d - i * I.h;
-    const int p = j * I.w + i;
     const int nr = min(*total_raw, max_surfels);
     const int k = scan_pos(pre_filt, bs_filt, d);
-    if (k < nr) reinterpret_cast<float4*>(model + 12 * (size_t)k)[2] = reinterpret_cast<const float4*>(filt_norm)[p];
+    if (k < nr) reinterpret_cast<float4*>(model + 12 * (size_t)k)[2] = reinterpret_cast<const float4*>(filt_norm)[d];
 }
 
 // ------------------------------------------------------------ index map
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(256) void k_index_raster(IndexArgs A, unsigned long
     if (px < 0 || py < 0) return;
     const unsigned q = index_depth(A, hp[2]);
     if (q >= kDepthClear) return;
-    atomicMin(keys + (size_t)py * A.I.w + px, ((unsigned long long)q << 32) | (unsigned)k);
+    atomicMin(keys + cm(A.I, px, py), ((unsigned long long)q << 32) | (unsigned)k);
 }
 
 __global__ __launch_bounds__(256) void k_index_resolve(IndexArgs A, unsigned long long* keys, uint32_t* index,
@@ -479,7 +482,7 @@ struct DataArgs {
     const uint32_t* index;
     const float4* vert_conf;
     const float4* norm_rad;
-    float* pix;             // 12 floats per pixel (row-major)
+    float* pix;             // 12 floats per pixel (draw order)
     int* flag_fresh;        // by draw index
     unsigned* claim;        // per surfel: first draw index that matched it
     int count;              // surfels in the model the claims refer to
@@ -489,10 +492,9 @@ struct DataArgs {
 __global__ __launch_bounds__(256) void k_data(DataArgs A)
 {
     const Img& I = A.I;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= I.w * I.h) return;
-    const int i = p % I.w, j = p / I.w;
-    const int d = i * I.h + j;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= I.w * I.h) return;
+    const int i = d / I.h, j = d - i * I.h;
     const float u = pixel_tc(i, I.w), v = pixel_tc(j, I.h);
     const float x = u * I.cols, y = v * I.rows;
     const Cam4& c = A.c;
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(256) void k_data(DataArgs A)
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const size_t q = (size_t)min(ry.t0 + b, I.h - 1) * I.w + min(rx.t0 + a, I.w - 1);
+                    const size_t q = cm(I, min(rx.t0 + a, I.w - 1), min(ry.t0 + b, I.h - 1));
                     id[3 * a + b] = A.index[q];
                     vcs[3 * a + b] = A.vert_conf[q];
                     nrs[3 * a + b] = A.norm_rad[q];
@@ -587,15 +589,15 @@ __global__ __launch_bounds__(256) void k_data(DataArgs A)
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    if (rx.m[a] && ry.m[b]) visit((size_t)(ry.t0 + b) * I.w + (rx.t0 + a));
+                    if (rx.m[a] && ry.m[b]) visit(cm(I, rx.t0 + a, ry.t0 + b));
         } else {
             for (float ii = u0; ii < u1; ii += sx)
-                for (float jj = v0; jj < v1; jj += sy) visit((size_t)texel(jj, I.h) * I.w + texel(ii, I.w));
+                for (float jj = v0; jj < v1; jj += sy) visit(cm(I, texel(ii, I.w), texel(jj, I.h)));
         }
         update = counter > 0 ? 1 : 2;
         o[7] = counter > 0 ? -1.0f : -2.0f;
     }
-    store12(A.pix + 12 * (size_t)p, o);
+    store12(A.pix + 12 * (size_t)d, o);
     A.flag_fresh[d] = update > 0 ? 1 : 0;
     // an index map predicted from a larger model can name surfels that are gone
     if (update == 1 && best < (unsigned)A.count) atomicMin(A.claim + best, (unsigned)d);
@@ -610,10 +612,9 @@ __global__ __launch_bounds__(256) void k_update(float* model, int count, const f
     const unsigned d = claim[k];
     if (d == kNoClaim) return;
     claim[k] = kNoClaim;
-    const int i = (int)(d / (unsigned)I.h), j = (int)(d - (unsigned)i * (unsigned)I.h);
     float s[12], g[12];
     load12(model + 12 * (size_t)k, s);
-    load12(pix + 12 * ((size_t)j * I.w + i), g);
+    load12(pix + 12 * (size_t)d, g);
     const float ck = s[3], a = g[3];
     if (g[11] < (1.0f + 0.5f) * s[11]) {
         float np[3], oc[3], nc[3], avg[3], nn[4];
@@ -650,9 +651,8 @@ __global__ __launch_bounds__(256) void k_fresh_scatter(const float* pix, const i
 {
     const int d = blockIdx.x * 256 + threadIdx.x;
     if (d >= I.w * I.h || !flags[d]) return;
-    const int i = d / I.h, j = d - i * I.h;
     float o[12];
-    load12(pix + 12 * ((size_t)j * I.w + i), o);
+    load12(pix + 12 * (size_t)d, o);
     store12(fresh + 12 * (size_t)scan_pos(prefix, block_sums, d), o);
 }
 
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs A)
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const size_t q = (size_t)min(ry.t0 + b, I.h - 1) * I.w + min(rx.t0 + a, I.w - 1);
+                    const size_t q = cm(I, min(rx.t0 + a, I.w - 1), min(ry.t0 + b, I.h - 1));
                     id[3 * a + b] = A.index[q];
                     vcs[3 * a + b] = A.vert_conf[q];
                     cts[3 * a + b] = A.color_time[q];
@@ -750,10 +750,10 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs A)
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    if (rx.m[a] && ry.m[b]) visit((size_t)(ry.t0 + b) * I.w + (rx.t0 + a), rx.m[a] * ry.m[b]);
+                    if (rx.m[a] && ry.m[b]) visit(cm(I, rx.t0 + a, ry.t0 + b), rx.m[a] * ry.m[b]);
         } else {
             for (float ii = u0; ii < u1; ii += sx)
-                for (float jj = v0; jj < v1; jj += sy) visit((size_t)texel(jj, I.h) * I.w + texel(ii, I.w), 1);
+                for (float jj = v0; jj < v1; jj += sy) visit(cm(I, texel(ii, I.w), texel(jj, I.h)), 1);
         }
     }
     int test = 1;
@@ -1189,12 +1189,24 @@ int surfel_set_model(surfel_handle c, const float* model, int n, int mem)
 int surfel_read_index_map(surfel_handle c, uint32_t* index, float* vert_conf, float* color_time, float* norm_rad)
 {
     if (!c || !index || !vert_conf || !color_time || !norm_rad) return ODOM_ERR_ARG;
-    const size_t np = (size_t)c->I.w * c->I.h;
-    SCHK(hipMemcpyAsync(index, c->index, np * 4, hipMemcpyDeviceToHost, c->stream));
-    SCHK(hipMemcpyAsync(vert_conf, c->vert_conf, np * 16, hipMemcpyDeviceToHost, c->stream));
-    SCHK(hipMemcpyAsync(color_time, c->color_time, np * 16, hipMemcpyDeviceToHost, c->stream));
-    SCHK(hipMemcpyAsync(norm_rad, c->norm_rad, np * 16, hipMemcpyDeviceToHost, c->stream));
+    const int w = c->I.w, h = c->I.h;
+    const size_t np = (size_t)w * h;
+    // the device maps are in draw order (column by column); the host gets rows
+    std::vector<uint32_t> ti(np);
+    std::vector<float> tv(np * 4), tc(np * 4), tn(np * 4);
+    SCHK(hipMemcpyAsync(ti.data(), c->index, np * 4, hipMemcpyDeviceToHost, c->stream));
+    SCHK(hipMemcpyAsync(tv.data(), c->vert_conf, np * 16, hipMemcpyDeviceToHost, c->stream));
+    SCHK(hipMemcpyAsync(tc.data(), c->color_time, np * 16, hipMemcpyDeviceToHost, c->stream));
+    SCHK(hipMemcpyAsync(tn.data(), c->norm_rad, np * 16, hipMemcpyDeviceToHost, c->stream));
     SCHK(hipStreamSynchronize(c->stream));
+    for (int x = 0; x < w; ++x)
+        for (int y = 0; y < h; ++y) {
+            const size_t d = (size_t)x * h + y, r = (size_t)y * w + x;
+            index[r] = ti[d];
+            memcpy(vert_conf + 4 * r, &tv[4 * d], 16);
+            memcpy(color_time + 4 * r, &tc[4 * d], 16);
+            memcpy(norm_rad + 4 * r, &tn[4 * d], 16);
+        }
     return ODOM_OK;
 }
 
