@@ -198,33 +198,71 @@ __global__ __launch_bounds__(256) void k_fill_keys(unsigned long long* keys, int
 }
 
 // One thread per surfel: the vertex stage, then the pixel centres of its
-// sprite (exact bounds, so every iteration is a candidate fragment).
+// sprite (exact bounds, so every iteration is a candidate fragment).  A
+// sprite of more than kBigSprite candidates would hold its whole wave for its
+// length, so the wave works through such sprites together, 64 candidates at
+// a time, after the lanes' own small ones.
+constexpr int kBigSprite = 100;
+
 __global__ __launch_bounds__(256) void k_splat_raster(const float4* __restrict__ model, int n, SplatUniforms U,
                                                       const float4* __restrict__ rays, unsigned long long* keys,
                                                       unsigned long long* frag_count)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const size_t b = (size_t)k * 3;
-    Surf s;
-    float xw, yw, size;
-    if (!surfel_stage(model[b], model[b + 1], model[b + 2], U, s) || !sprite(s, U.cam, xw, yw, size)) return;
-    const float half = size * 0.5f;
-    const int i0 = max(0, first_centre(xw - half));
-    const int i1 = min(U.cam.w - 1, last_centre(xw + half));
-    const int j0 = max(0, first_centre(yw - half));
-    const int j1 = min(U.cam.h - 1, last_centre(yw + half));
+    Surf s{};
+    int i0 = 0, j0 = 0, ncols = 1, cnt = 0;
+    if (k < n) {
+        const size_t b = (size_t)k * 3;
+        float xw, yw, size;
+        if (surfel_stage(model[b], model[b + 1], model[b + 2], U, s) && sprite(s, U.cam, xw, yw, size)) {
+            const float half = size * 0.5f;
+            i0 = max(0, first_centre(xw - half));
+            const int i1 = min(U.cam.w - 1, last_centre(xw + half));
+            j0 = max(0, first_centre(yw - half));
+            const int j1 = min(U.cam.h - 1, last_centre(yw + half));
+            if (i1 >= i0 && j1 >= j0) {
+                ncols = i1 - i0 + 1;
+                cnt = ncols * (j1 - j0 + 1);
+            }
+        }
+    }
     const float num = plane_num(s);
     const int w = U.cam.w;
     unsigned drawn = 0;
-    for (int j = j0; j <= j1; ++j) {
-        const size_t row = (size_t)j * w;
-        for (int i = i0; i <= i1; ++i) {
-            float z;
-            unsigned q;
-            if (!fragment(s, num, U.maxDepth, rays[row + i], z, q)) continue;
-            ++drawn;
-            atomicMin(keys + row + i, ((unsigned long long)q << 32) | (unsigned)k);
+    auto candidate = [&](const Surf& f, float fnum, int i, int j, unsigned idx) {
+        const size_t p = (size_t)j * w + i;
+        float z;
+        unsigned q;
+        if (!fragment(f, fnum, U.maxDepth, rays[p], z, q)) return;
+        ++drawn;
+        atomicMin(keys + p, ((unsigned long long)q << 32) | idx);
+    };
+    if (cnt > 0 && cnt <= kBigSprite) {
+        const int j1 = j0 + cnt / ncols - 1, i1 = i0 + ncols - 1;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) candidate(s, num, i, j, (unsigned)k);
+    }
+    // the wave's big sprites, one after another, each spread over the lanes
+    const int lane = threadIdx.x & 63;
+    unsigned long long big = __ballot(cnt > kBigSprite);
+    while (big) {
+        const int src = __ffsll((long long)big) - 1;
+        big &= big - 1;
+        Surf f;
+        f.hx = __shfl(s.hx, src, 64);
+        f.hy = __shfl(s.hy, src, 64);
+        f.hz = __shfl(s.hz, src, 64);
+        f.nx = __shfl(s.nx, src, 64);
+        f.ny = __shfl(s.ny, src, 64);
+        f.nz = __shfl(s.nz, src, 64);
+        f.rad = __shfl(s.rad, src, 64);
+        const float fnum = __shfl(num, src, 64);
+        const int bi0 = __shfl(i0, src, 64), bj0 = __shfl(j0, src, 64);
+        const int bcols = __shfl(ncols, src, 64), bcnt = __shfl(cnt, src, 64);
+        const unsigned bk = (unsigned)__shfl(k, src, 64);
+        for (int t = lane; t < bcnt; t += 64) {
+            const int r = t / bcols;
+            candidate(f, fnum, bi0 + (t - r * bcols), bj0 + r, bk);
         }
     }
     if (frag_count && drawn) atomicAdd(frag_count, (unsigned long long)drawn);

@@ -196,3 +196,31 @@ def test_bad_arguments_raise():
     # an empty model is a valid prediction (all outputs cleared)
     im.combinedPredict(pose, np.zeros((0, 12), np.float32), CUTOFF, 0.0, 0, 0, 100)
     assert not im.read(splat.VERTEX).any()
+
+
+def test_large_sprites_shared_by_the_wave(orc):
+    """Sprites above the per-lane limit (here up to ~130 px wide) are worked
+    through by the whole wave: same outputs as the oracle, mixed with small
+    ones in the same waves and overlapping each other."""
+    cam = synth.CAM_320
+    rng = np.random.default_rng(5)
+    model = []
+    for k in range(300):
+        z = rng.uniform(0.8, 4.0)
+        big = k % 7 == 0
+        r = rng.uniform(0.1, 0.3) if big else rng.uniform(0.002, 0.02)
+        n = rng.normal(size=3)
+        n[2] = -abs(n[2]) - 0.5
+        n /= np.linalg.norm(n)
+        model.append([rng.uniform(-0.6, 0.6) * z, rng.uniform(-0.45, 0.45) * z, z, 20.0,
+                      float(rng.integers(0, 1 << 24)), 0.0, float(k), 5.0, n[0], n[1], n[2], r])
+    model = np.array(model, np.float32)
+    im = splat.IndexMap(*cam_args(cam), max_host_surfels=model.shape[0])
+    pose = np.eye(4, dtype=np.float32)
+    im.combinedPredict(pose, model, CUTOFF, 0.0, 5, 5, 100)
+    exp = orc.splat_predict(*cam_args(cam), pose, model, CUTOFF, 0.0, 5, 5, 100)
+    for name, w, e in zip(("image", "vertex", "normal", "time"), (splat.IMAGE, splat.VERTEX, splat.NORMAL, splat.TIME),
+                          exp):
+        g = im.read(w)
+        assert same_bits(g, e), (name, int(np.sum(g != e)))
+    assert (exp[1][..., 2] > 0).mean() > 0.3
