@@ -316,7 +316,7 @@ def surfel_fusion(args) -> dict:
     return res
 
 
-def frame_loop_c5(args) -> dict:
+def frame_loop_sequence(args) -> dict:
     """The reference's frame loop without loop closure on the device
     (slam_amd/frame_loop.py): per frame the bilateral filter, frame-to-model
     ICP+RGB+SO3 tracking against the filled-in splat prediction of the surfel
@@ -332,7 +332,7 @@ def frame_loop_c5(args) -> dict:
     from slam_amd.frame_loop import FrameLoop
 
     cam = synth.CAM_640
-    n = args.c5_frames
+    n = args.loop_frames
     poses = synth.spline_trajectory(n, seed=920, max_step_m=0.10, max_step_deg=8.0, bursts=True)
     T0i = np.linalg.inv(poses[0])
     gt = [T0i @ T for T in poses]
@@ -343,9 +343,9 @@ def frame_loop_c5(args) -> dict:
         f = scene.render(T, cam, noise_seed=92000 + k, model=False)
         frames.append((torch.from_numpy(f.rgb).cuda(), torch.from_numpy(f.depth).cuda()))
         if (k + 1) % 500 == 0:
-            log(f"c5: rendered {k + 1}/{n} frames")
+            log(f"frame loop: rendered {k + 1}/{n} frames")
     torch.cuda.synchronize()
-    log(f"c5: {n} frames resident in {time.time() - t0:.1f}s")
+    log(f"frame loop: {n} frames resident in {time.time() - t0:.1f}s")
     steps = [np.linalg.inv(gt[k]) @ gt[k + 1] for k in range(n - 1)]
     from slam_amd.frame_loop import LoopParams
 
@@ -418,7 +418,7 @@ def frame_loop_c5(args) -> dict:
         "plain": run(False),
         "learned": run(True) if os.path.exists(os.path.join(ROOT, "slam_amd", "weights", "warmstart.npz")) else None,
     }
-    log(f"c5 {res}")
+    log(f"frame loop {res}")
     return res
 
 
@@ -574,9 +574,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=20)
     ap.add_argument("--no-splat", action="store_true", help="skip the splat prediction measurement")
     ap.add_argument("--no-fusion", action="store_true", help="skip the surfel fusion measurement")
-    ap.add_argument("--no-c5", action="store_true", help="skip the full frame-loop sequence")
-    ap.add_argument("--c5-only", action="store_true", help="run only the full frame-loop sequence")
-    ap.add_argument("--c5-frames", type=int, default=2000)
+    ap.add_argument("--no-loop", action="store_true", help="skip the full frame-loop sequences")
+    ap.add_argument("--loop-only", action="store_true", help="run only the full frame-loop sequences")
+    ap.add_argument("--loop-frames", type=int, default=2000)
     ap.add_argument("--fusion-only", action="store_true", help="measure only the surfel fusion (profiling)")
     ap.add_argument("--fusion-frames", type=int, default=10)
     ap.add_argument("--fusion-reps", type=int, default=20)
@@ -598,8 +598,8 @@ def main() -> int:
     if args.fusion_only:
         print(json.dumps({"fusion": surfel_fusion(args)}), flush=True)
         return 0
-    if args.c5_only:
-        print(json.dumps({"frame_loop": frame_loop_c5(args)}), flush=True)
+    if args.loop_only:
+        print(json.dumps({"frame_loop": frame_loop_sequence(args)}), flush=True)
         return 0
 
     import torch
@@ -798,7 +798,7 @@ def main() -> int:
     batched = None if args.no_batch else batched_pairs(args)
     splat_res = None if args.no_splat else splat_prediction(args)
     fusion_res = None if args.no_fusion else surfel_fusion(args)
-    c5_res = None if args.no_c5 else frame_loop_c5(args)
+    loop_res = None if args.no_loop else frame_loop_sequence(args)
 
     line = {
         "metric": "frames/s 640x480 ICP+RGB at <=1mm vs FP64 oracle; learned-init vs plain",
@@ -828,7 +828,7 @@ def main() -> int:
         "batched": batched,
         "splat": splat_res,
         "fusion": fusion_res,
-        "frame_loop": c5_res,
+        "frame_loop": loop_res,
     }
     print(json.dumps(line), flush=True)
     return 0
