@@ -142,10 +142,11 @@ int efloop_process(efloop_handle c, const odom_frame_view* rgb, const odom_frame
         float last[16];
         memcpy(last, c->pose, sizeof last);
         const bool tracking_ok = c->last_icp_error < 1e-4f;  // the previous call's, as the loop reads it
-        void *fv, *fn, *fi;
-        splat_get_buffer(c->pred, SPLAT_FILL_VERTEX, &fv);
-        splat_get_buffer(c->pred, SPLAT_FILL_NORMAL, &fn);
-        splat_get_buffer(c->pred, SPLAT_FILL_IMAGE, &fi);
+        void *fv = nullptr, *fn = nullptr, *fi = nullptr;
+        rc = splat_get_buffer(c->pred, SPLAT_FILL_VERTEX, &fv);
+        if (!rc) rc = splat_get_buffer(c->pred, SPLAT_FILL_NORMAL, &fn);
+        if (!rc) rc = splat_get_buffer(c->pred, SPLAT_FILL_IMAGE, &fi);
+        if (rc) return rc;
         const odom_frame_view vv = device_view(fv, c->w, c->h, 16, ODOM_FMT_RGBA32F);
         const odom_frame_view nv = device_view(fn, c->w, c->h, 16, ODOM_FMT_RGBA32F);
         const odom_frame_view iv = device_view(fi, c->w, c->h, 4, ODOM_FMT_RGBA8);
