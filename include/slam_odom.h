@@ -11,7 +11,10 @@
  *
  * Threading: a handle owns one HIP stream and is not re-entrant; distinct
  * handles may be used from distinct host threads.  Inputs are borrowed only
- * for the duration of the call.
+ * for the duration of the call.  Calls that read results back (the pose, the
+ * statistics, the surfel count of slam_surfel.h) poll the stream until they
+ * arrive, keeping the calling thread busy meanwhile, rather than sleeping
+ * until the runtime's completion interrupt.
  */
 #ifndef SLAM_ODOM_H
 #define SLAM_ODOM_H

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 
+#include "odom_internal.h"
 #include "slam_loop.h"
 
 struct efloop_ctx {
@@ -160,8 +161,8 @@ int efloop_process(efloop_handle c, const odom_frame_view* rgb, const odom_frame
             for (int k = 0; k < 3; ++k) R[3 * r + k] = c->pose[4 * r + k];
         }
         if (!rc) rc = odom_track(c->odo, t, R, 0, p.icp_weight, 1, 0, p.so3);
-        odom_stats st;
-        if (!rc) rc = odom_get_stats(c->odo, &st);
+        odom_stats st;  // read back with the pose by odom_track
+        if (!rc) rc = slam::odom_stats_last_read(c->odo, &st);
         if (rc) return rc;
         c->last_icp_error = st.last_icp_error;
         for (int r = 0; r < 3; ++r) {

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "host_wait.h"
 #include "odom_internal.h"
 #include "odom_math.h"
 #include "slam_odom.h"
@@ -78,6 +79,7 @@ struct odom_ctx {
     PairDesc h_pair[2]{};
     StepShape shape_a[kLevels], shape_b[kLevels];
     TrackState h_st{};
+    TrackState* h_pin = nullptr;     // pinned landing buffer of the state readback
     bool have_state = false;
     // call-order bookkeeping: which inputs have been provided at least once.
     // vtmp: the model vertex buffer that initRGB/initRGBModel read
@@ -408,8 +410,9 @@ int enqueue_track(odom_ctx* c, const TrackParams& P, hipEvent_t* ev = nullptr)
 
 int read_state(odom_ctx* c)
 {
-    HIPCHK(c, hipMemcpyAsync(&c->h_st, c->st, sizeof(TrackState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_pin, c->st, sizeof(TrackState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, poll_stream(c->stream));
+    memcpy(&c->h_st, c->h_pin, sizeof(TrackState));
     c->have_state = true;
     return ODOM_OK;
 }
@@ -515,7 +518,8 @@ int odom_create(const odom_params* p, odom_handle* out)
     size_t total = 0;
     for (auto& r : reqs) total += align_up(r.bytes, 256);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void**)&c->arena, total) != hipSuccess) {
+        hipMalloc((void**)&c->arena, total) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_pin, sizeof(TrackState), 0) != hipSuccess) {
         odom_destroy(c);
         return ODOM_ERR_HIP;
     }
@@ -554,6 +558,7 @@ int odom_destroy(odom_handle c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     if (c->arena) (void)hipFree(c->arena);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ODOM_OK;
@@ -788,8 +793,16 @@ int odom_track_seeded(odom_handle c, const float* pred, float t_scale, float r_s
 int odom_get_stats(odom_handle c, odom_stats* s)
 {
     if (!c || !s) return ODOM_ERR_ARG;
-    int rc = read_state(c);
+    const int rc = read_state(c);
     if (rc) return rc;
+    return slam::odom_stats_last_read(c, s);
+}
+
+}  // extern "C"
+
+int slam::odom_stats_last_read(odom_handle c, odom_stats* s)
+{
+    if (!c || !s || !c->have_state) return ODOM_ERR_ARG;
     const TrackState& t = c->h_st;
     s->last_icp_error = t.lastICPError;
     s->last_icp_count = t.lastICPCount;
@@ -806,6 +819,8 @@ int odom_get_stats(odom_handle c, odom_stats* s)
     s->seed_kept = t.seed_kept;
     return ODOM_OK;
 }
+
+extern "C" {
 
 int odom_get_last_system(odom_handle c, double A[36], double b[6])
 {

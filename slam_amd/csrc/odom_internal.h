@@ -6,8 +6,14 @@
 #include <stdint.h>
 
 #include "odom_device.h"
+#include "slam_odom.h"
 
 namespace slam {
+
+// The statistics of the state last read back to the host (by odom_track,
+// odom_get_pose or odom_get_stats), without a device round trip; for a caller
+// that has just made one of those calls.  ODOM_ERR_ARG before any readback.
+int odom_stats_last_read(odom_handle c, odom_stats* s);
 
 // Launch shape of a step kernel at one level (replaces GPUConfig.h's
 // threads/blocks table; chosen by the autotuner).

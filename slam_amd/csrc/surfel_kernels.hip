@@ -29,6 +29,7 @@
 
 #include <vector>
 
+#include "host_wait.h"
 #include "odom_math.h"
 #include "slam_surfel.h"
 
@@ -816,6 +817,7 @@ struct surfel_ctx {
     int *prefix = nullptr, *prefix2 = nullptr;
     int *bsums = nullptr, *bsums2 = nullptr;
     int* totals = nullptr;            // [0] scan total, [1] second scan total
+    int* h_count = nullptr;           // pinned landing buffer of the count readback
 };
 
 #define SCHK(x)                                                                                      \
@@ -837,6 +839,7 @@ void free_ctx(surfel_ctx* c)
                  c->feed_norm, c->flags, c->flags2, c->prefix, c->prefix2, c->bsums, c->bsums2, c->totals};
     for (void* p : b)
         if (p) (void)hipFree(p);
+    if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -905,8 +908,9 @@ void scan(surfel_ctx* c, const int* flags, int n, int* prefix, int* bsums, int* 
 
 int read_count(surfel_ctx* c, const int* dev, int* out)
 {
-    SCHK(hipMemcpyAsync(out, dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    SCHK(hipStreamSynchronize(c->stream));
+    SCHK(hipMemcpyAsync(c->h_count, dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SCHK(slam::poll_stream(c->stream));
+    *out = *c->h_count;
     return ODOM_OK;
 }
 
@@ -1048,6 +1052,7 @@ int surfel_create(int width, int height, float cx, float cy, float fx, float fy,
     A((void**)&c->bsums, 4096 * 4);
     A((void**)&c->bsums2, 4096 * 4);
     A((void**)&c->totals, 16);
+    ok = ok && hipHostMalloc((void**)&c->h_count, sizeof(int), 0) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(npix, 256)), dim3(256), 0, c->stream, c->keys, (int)npix, kKeyClear);
         hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(max_surfels, 256)), dim3(256), 0, c->stream, c->claim, max_surfels,
@@ -1104,12 +1109,12 @@ int surfel_initialise(surfel_handle c, const odom_frame_view* rgb, const odom_fr
                        c->model[0]);
     hipLaunchKernelGGL(k_init_normals, dim3(cdiv(npix, 256)), dim3(256), 0, c->stream, c->feed_norm, c->flags2,
                        c->prefix2, c->bsums2, c->totals, c->I, c->max_surfels, c->model[0]);
+    SCHK(hipMemsetAsync(c->totals + 1, 0, 4, c->stream));  // no new points pending
     SCHK(hipGetLastError());
     int n = 0;
     rc = read_count(c, c->totals, &n);
     if (rc) return rc;
     c->count = n < c->max_surfels ? n : c->max_surfels;
-    SCHK(hipMemsetAsync(c->totals + 1, 0, 4, c->stream));  // no new points pending
     return ODOM_OK;
 }
 
@@ -1138,13 +1143,13 @@ int surfel_clean(surfel_handle c, const float pose[16], int time, float conf_thr
 {
     if (!c || !pose) return ODOM_ERR_ARG;
     enqueue_clean(c, pose, time, conf_threshold, time_delta);
+    SCHK(hipMemsetAsync(c->totals + 1, 0, 4, c->stream));  // the new points are appended
     SCHK(hipGetLastError());
     int n = 0;
     const int rc = read_count(c, c->totals, &n);
     if (rc) return rc;
     c->cur ^= 1;
     c->count = n < c->max_surfels ? n : c->max_surfels;
-    SCHK(hipMemsetAsync(c->totals + 1, 0, 4, c->stream));
     return ODOM_OK;
 }
 
