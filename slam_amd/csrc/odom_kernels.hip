@@ -14,6 +14,8 @@
 // the file is compiled with -ffp-contract=off, so masks are bit-identical to
 // the CPU restatement of the same algorithm.
 #include <float.h>
+
+#include <atomic>
 #include <type_traits>
 
 #include "odom_internal.h"
@@ -63,19 +65,21 @@ constexpr int kBLY = kBTY + 12;         // LDS tile height
 
 // The 13x13 spatial factors, set from the host copy of bexpf (the same
 // operation sequence) so the unrolled interior path reads them as scalars.
+// Set once per device; handles created concurrently on one device may both
+// upload the (identical) table.
 __constant__ float c_bilat_es[169];
-static bool g_bilat_es_set[64];
+static std::atomic<bool> g_bilat_es_set[64];
 
 void init_device_tables()
 {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_bilat_es_set[dev]) return;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_bilat_es_set[dev].load()) return;
     float es[169];
     for (int i = 0; i < 169; ++i) {
         const int dy = i / 13 - 6, dx = i % 13 - 6;
         es[i] = bexpf(-((float)(dx * dx + dy * dy) * 0.024691358f));
     }
-    if (hipMemcpyToSymbol(HIP_SYMBOL(c_bilat_es), es, sizeof es) == hipSuccess) g_bilat_es_set[dev] = true;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_bilat_es), es, sizeof es) == hipSuccess) g_bilat_es_set[dev].store(true);
 }
 
 __global__ __launch_bounds__(256) void k_bilateral(const uint16_t* __restrict__ src, int pitch,
