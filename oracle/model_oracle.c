@@ -305,7 +305,10 @@ int orc_model_initialise(orc_model* m, const uint8_t* rgb, const uint16_t* raw, 
     float tmp[12];
     for (int i = 0; i < w; ++i)
         for (int j = 0; j < h; ++j) {
-            if (feedback_pixel(dr, rgb, w, h, i, j, cam, time, maxDepth, tmp)) memcpy(m->scratch + F(nr++), tmp, 48);
+            if (feedback_pixel(dr, rgb, w, h, i, j, cam, time, maxDepth, tmp)) {
+                if (nr < m->max_surfels) memcpy(m->scratch + F(nr), tmp, 48); /* a full buffer drops the rest */
+                ++nr;
+            }
             if (feedback_pixel(df, rgb, w, h, i, j, cam, time, maxDepth, tmp)) memcpy(fr + F(nf++), tmp, 48);
         }
     if (nr > m->max_surfels) nr = m->max_surfels;

@@ -79,6 +79,39 @@ def test_fusion_sequence_bit_exact(orc, cam, seed, frames):
         assert same_bits(gpu.model(), ref.model()), ("cleaned model", k)
 
 
+@pytest.mark.parametrize("frac", [0.35, 0.979], ids=["initialise-overflows", "clean-overflows"])
+def test_full_model_keeps_the_first_surfels_in_order(orc, frac):
+    """A model buffer smaller than the sequence needs: initialise and clean
+    keep the first max_surfels surfels in draw order and drop the rest, as
+    the oracle does (transform feedback into a full buffer)."""
+    cam = synth.CAM_320
+    seq = sequence(cam, 64, 3)
+    args = (cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    cap = int(frac * cam.width * cam.height)
+    gpu = surfels.SurfelModel(*args, cap)
+    ref = orc.SurfelModel(*args, cap)
+    T, f = seq[0]
+    filt = orc.bilateral(f.depth, MAXD)
+    ng = gpu.initialise(f.rgb, f.depth, filt, DEPTH_CUTOFF, 1, MAX_DEPTH)
+    nr = ref.initialise(f.rgb, f.depth, filt, DEPTH_CUTOFF, 1, MAX_DEPTH)
+    assert ng == nr and ng <= cap and same_bits(gpu.model(), ref.model())
+    full = ng == cap
+    for k in range(1, len(seq)):
+        T, f = seq[k]
+        filt = orc.bilateral(f.depth, MAXD)
+        for m in (gpu, ref):
+            m.predictIndices(T, k + 1, MAX_DEPTH, DELTA)
+            m.fuse(T, k + 1, f.rgb, f.depth, filt, DEPTH_CUTOFF, MAX_DEPTH, 1.0)
+            m.predictIndices(T, k + 1, MAX_DEPTH, DELTA)
+        assert same_bits(gpu.model(), ref.model()), ("fused model", k)
+        cg = gpu.clean(T, k + 1, CONF, DELTA)
+        cr = ref.clean(T, k + 1, CONF, DELTA)
+        assert cg == cr and cg <= cap, (k, cg, cr)
+        assert same_bits(gpu.model(), ref.model()), ("cleaned model", k)
+        full = full or cg == cap
+    assert full                      # the capacity was reached
+
+
 def test_cleaning_drops_stale_surfels_on_the_device(orc):
     """Thirty frames past the last update, unconfident surfels go and
     confident ones stay, as the oracle decides."""

@@ -72,6 +72,17 @@ def test_initialise_a_wall():
     assert model().initialise(rgb, d, wall(z), MAXD, 7, MAX_DEPTH) == W * H - 2
 
 
+def test_full_buffer_keeps_the_first_surfels():
+    """Transform feedback into a full buffer drops what does not fit: a model
+    of capacity 100 holds the first 100 surfels of the unbounded one."""
+    rgb = rgb_img(4)
+    d = wall(2.0)              # every pixel makes a surfel; the random colours tell them apart
+    full, small = model(4096), model(100)
+    n = full.initialise(rgb, d, d, MAXD, 3, MAX_DEPTH)
+    assert n == W * H and small.initialise(rgb, d, d, MAXD, 3, MAX_DEPTH) == 100
+    assert np.array_equal(small.model(), full.model()[:100])
+
+
 def one(x, y, z, k_conf=5.0, last=3.0, init=1.0, n=(0.0, 0.0, 1.0), r=0.01, col=0.0):
     return [x, y, z, k_conf, col, 0.0, init, last, n[0], n[1], n[2], r]
 
