@@ -4,6 +4,11 @@
         markdown table of the kernel statistics (DIR/run_kernel_stats.csv)
     python -m slam_amd.perfsummary frame  DIR
         kernels of one mid-run tracking call (DIR/run_kernel_trace.csv)
+    python -m slam_amd.perfsummary loop   DIR [--frames A B]
+        frames A..B of the first frame-loop run in the trace (a frame runs
+        from one bilateral launch to the next and holds a tracking call and
+        a splat prediction): kernel time per frame, the frame's span, and
+        the idle gaps above 4 us with the launches on either side
     python -m slam_amd.perfsummary traffic FETCH_DIR WRITE_DIR [--out JSON]
         HBM bytes per launch of the ICP/association pass (k_step_a) per
         pyramid level, from FETCH_SIZE and WRITE_SIZE counter runs (KiB per
@@ -80,6 +85,49 @@ def cmd_frame(d: str, mode: str = "plain") -> str:
     return "\n".join(out)
 
 
+def cmd_loop(d: str, first: int, last: int) -> str:
+    tr = list(csv.DictReader(open(os.path.join(d, "run_kernel_trace.csv"))))
+    tr.sort(key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(tr) if "k_bilateral" in r["Kernel_Name"]]
+    frames = []
+    for a, b in zip(starts[:-1], starts[1:]):
+        names = " ".join(r["Kernel_Name"] for r in tr[a:b])
+        if "k_track_end" in names and "k_splat_raster" in names:
+            frames.append((a, b))
+        elif frames:
+            break                       # the first run of loop frames has ended
+    frames = frames[first:last]
+    if not frames:
+        raise ValueError("no frame-loop frames in the trace")
+    agg = defaultdict(lambda: [0, 0.0])
+    gaps = defaultdict(lambda: [0, 0.0])
+    span = busy = 0.0
+    for a, b in frames:
+        span += (int(tr[b]["Start_Timestamp"]) - int(tr[a]["Start_Timestamp"])) / 1e3
+        for i in range(a, b):
+            r = tr[i]
+            us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            busy += us
+            g = agg[short(r["Kernel_Name"])]
+            g[0] += 1
+            g[1] += us
+            idle = (int(tr[i + 1]["Start_Timestamp"]) - int(r["End_Timestamp"])) / 1e3
+            if idle > 4.0:
+                e = gaps[f"{short(r['Kernel_Name'])} -> {short(tr[i + 1]['Kernel_Name'])}"]
+                e[0] += 1
+                e[1] += idle
+    n = len(frames)
+    out = [f"Frames {first}..{first + n - 1} of the first loop run: span {span / n:.1f} us per frame, "
+           f"kernels {busy / n:.1f} us per frame.\n",
+           "| kernel | launches per frame | us per frame |", "|---|---:|---:|"]
+    for k, (c, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        out.append(f"| `{k}` | {c / n:.2f} | {us / n:.1f} |")
+    out += ["\nIdle gaps above 4 us:\n", "| after -> before | per frame | us per frame |", "|---|---:|---:|"]
+    for k, (c, us) in sorted(gaps.items(), key=lambda x: -x[1][1]):
+        out.append(f"| `{k}` | {c / n:.2f} | {us / n:.1f} |")
+    return "\n".join(out)
+
+
 def _level(name: str, grid: int, wg: int) -> int | None:
     m = re.search(r"k_step_a<(\d+), (\d+)", name)
     if not m:
@@ -139,6 +187,9 @@ def main() -> int:
     f.add_argument("dir")
     f.add_argument("--mode", choices=("plain", "learned", "any"), default="plain",
                    help="plain: calls with the SO3 pass; learned: seeded calls")
+    lp = sub.add_parser("loop")
+    lp.add_argument("dir")
+    lp.add_argument("--frames", type=int, nargs=2, default=(10, 200))
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -148,6 +199,8 @@ def main() -> int:
         print(cmd_stats(a.dir, a.top, a.command))
     elif a.cmd == "frame":
         print(cmd_frame(a.dir, a.mode))
+    elif a.cmd == "loop":
+        print(cmd_loop(a.dir, *a.frames))
     else:
         res = cmd_traffic(a.fetch_dir, a.write_dir)
         text = json.dumps(res, indent=1)
