@@ -37,12 +37,14 @@ typedef struct efloop_params {
     int so3;              /* the SO3 pre-alignment of frame-to-model tracking */
     int max_surfels;      /* model capacity */
     int use_graph;        /* replay the tracking call from a hipGraph */
+    int autotune;         /* after the first tracking call, time the step kernels' launch
+                             shapes on its inputs and keep the fastest (odom_autotune) */
 } efloop_params;
 
 typedef struct efloop_ctx *efloop_handle;
 
 /* The reference configs' values (ef_file.cfg): 3 m, 20 m, 10, 200, icpWeight
- * 10, SO3 on; 4M surfels; graphs on. */
+ * 10, SO3 on; 4M surfels; graphs and autotuning on. */
 int efloop_default_params(efloop_params *p);
 int efloop_create(int width, int height, float cx, float cy, float fx, float fy, const efloop_params *p,
                   efloop_handle *out);

@@ -77,6 +77,7 @@ int efloop_default_params(efloop_params* p)
     p->so3 = 1;
     p->max_surfels = 4000000;
     p->use_graph = 1;
+    p->autotune = 1;
     return ODOM_OK;
 }
 
@@ -163,6 +164,9 @@ int efloop_process(efloop_handle c, const odom_frame_view* rgb, const odom_frame
         if (!rc) rc = odom_track(c->odo, t, R, 0, p.icp_weight, 1, 0, p.so3);
         odom_stats st;  // read back with the pose by odom_track
         if (!rc) rc = slam::odom_stats_last_read(c->odo, &st);
+        // launch shapes from the first tracking call's inputs; the timing
+        // passes write block partials only, which the next call rewrites
+        if (!rc && tick == 2 && p.autotune) rc = odom_autotune(c->odo, 0, p.icp_weight);
         if (rc) return rc;
         c->last_icp_error = st.last_icp_error;
         for (int r = 0; r < 3; ++r) {

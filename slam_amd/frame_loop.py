@@ -31,7 +31,7 @@ EXPORTS = ["efloop_default_params", "efloop_create", "efloop_destroy", "efloop_p
 class _Params(ctypes.Structure):
     _fields_ = [("depth_cutoff", ctypes.c_float), ("max_depth", ctypes.c_float), ("confidence", ctypes.c_float),
                 ("time_delta", ctypes.c_int), ("icp_weight", ctypes.c_float), ("so3", ctypes.c_int),
-                ("max_surfels", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("max_surfels", ctypes.c_int), ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 def lib() -> ctypes.CDLL:
@@ -101,6 +101,7 @@ class LoopParams:
     so3: bool = True
     max_surfels: int = 4_000_000
     use_graph: bool = True           # replay the tracking call from a hipGraph (native loop)
+    autotune: bool = True            # step-kernel launch shapes timed on the first tracked frame (native loop)
     learned: bool = False            # seed each call from the warm-start network (no SO3 pass then)
     learned_iters: tuple = (10, 5, 4)
 
@@ -127,6 +128,7 @@ class FrameLoop:
         P.time_delta, P.icp_weight, P.so3 = self.p.time_delta, self.p.icp_weight, int(self.p.so3)
         P.max_surfels = self.p.max_surfels
         P.use_graph = int(self.p.use_graph)
+        P.autotune = int(self.p.autotune)
         h = ctypes.c_void_p()
         _check("efloop_create", lib().efloop_create(width, height, cx, cy, fx, fy, ctypes.byref(P), ctypes.byref(h)))
         self._h = h

@@ -99,7 +99,10 @@ def test_frame_loop_tracks_a_sequence():
     gt = evaluate.trajectory(stamps, [T for T, _ in frames])
     est = evaluate.trajectory(stamps, [T.astype(np.float64) for T, _ in gpu])
     ate = evaluate.ate(gt, est)
-    assert ate["rmse"] < 0.01, ate["rmse"]
+    # about 1 cm of drift over the 40 frames; the loop autotunes its launch
+    # shapes from timing, and the reduction order they set moves the result
+    # by about a millimetre from run to run (a lost track is tens of cm)
+    assert ate["rmse"] < 0.015, ate["rmse"]
     # the model holds the room within the 3 m depth window, and repeated
     # observation has begun to make it stable (confidence above 10: about 25
     # associations at 0.35-0.5 each, a quarter of the pixels associating per
