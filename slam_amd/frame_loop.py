@@ -101,7 +101,7 @@ class LoopParams:
     so3: bool = True
     max_surfels: int = 4_000_000
     use_graph: bool = True           # replay the tracking call from a hipGraph (native loop)
-    autotune: bool = True            # step-kernel launch shapes timed on the first tracked frame (native loop)
+    autotune: bool = True            # step-kernel launch shapes timed on the first tracked frame
     learned: bool = False            # seed each call from the warm-start network (no SO3 pass then)
     learned_iters: tuple = (10, 5, 4)
 
@@ -236,6 +236,8 @@ class _PythonFrameLoop(FrameLoop):
             self.pose[:3, :3] = np.asarray(R, np.float32).reshape(3, 3)
             self.pose[:3, 3] = np.asarray(t, np.float32).reshape(3)
             self.last_icp_error = self.odo.stats()["lastICPError"]
+            if self.tick == 2 and p.autotune:
+                self.odo.autotune(False, p.icp_weight)  # launch shapes from the first tracked frame
             # weight by velocity (elastic_fusion_file.cpp:377-398)
             diff = np.linalg.inv(self.pose.astype(np.float64)) @ last.astype(np.float64)
             weighting = max(float(np.linalg.norm(diff[:3, 3])), _rodrigues_norm(diff[:3, :3]))

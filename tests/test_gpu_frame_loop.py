@@ -110,3 +110,20 @@ def test_frame_loop_tracks_a_sequence():
     M = run_gpu.last.model.model()
     assert M.shape[0] > 0.1 * cam.width * cam.height
     assert (M[:, 3] > 10.0).sum() > 100
+
+
+def test_learned_loop_tracks_a_short_sequence():
+    """The learned form (network warm start, seeded ICP+RGB, autotuned after
+    the first tracked frame) on 640x480 frames, the network's input size."""
+    import os
+    if not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "slam_amd", "weights", "warmstart.npz")):
+        pytest.skip("no warm-start weights in the tree")
+    cam = synth.CAM_640
+    frames = frames_of(cam, 74, 12, max_step_m=0.01, max_step_deg=0.5)
+    gpu = run_gpu(cam, frames, LoopParams(learned=True, max_surfels=2_000_000))
+    stamps = [float(k) for k in range(len(frames))]
+    gt = evaluate.trajectory(stamps, [T for T, _ in frames])
+    est = evaluate.trajectory(stamps, [T.astype(np.float64) for T, _ in gpu])
+    assert evaluate.ate(gt, est)["rmse"] < 0.015
+    # the model holds the part of the room inside the 3 m depth window
+    assert run_gpu.last.model.lastCount() > 0.1 * cam.width * cam.height
