@@ -38,7 +38,10 @@ typedef struct efloop_params {
     int max_surfels;      /* model capacity */
     int use_graph;        /* replay the tracking call from a hipGraph */
     int autotune;         /* after the first tracking call, time the step kernels' launch
-                             shapes on its inputs and keep the fastest (odom_autotune) */
+                             shapes on its inputs and keep the fastest (odom_autotune).  The
+                             shapes set the order of the block sums, and the choice follows
+                             timing, so poses can differ between runs in their last bits;
+                             0 keeps the fixed default shapes (reproducible runs). */
 } efloop_params;
 
 typedef struct efloop_ctx *efloop_handle;
