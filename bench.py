@@ -174,6 +174,16 @@ def batched_pairs(args) -> dict:
     return out
 
 
+def _splat_traffic():
+    """HBM bytes per raster launch from the committed counter runs (FETCH_SIZE
+    + WRITE_SIZE, copy-scaled) of the same 2M-surfel workload, or None."""
+    path = os.path.join(ROOT, "profiles", "k_splat_raster_traffic_640x480_2M.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return int(json.load(fh)["traffic_bytes"])
+
+
 def splat_prediction(args) -> dict:
     """The model prediction that feeds frame-to-model tracking: a 640x480
     combinedPredict of a 2M-surfel model (48 B per surfel: seven views of one
@@ -222,7 +232,8 @@ def splat_prediction(args) -> dict:
         "roofline": {"bound": "hbm", "kernel": "k_splat_raster", "achieved": round(model_bytes / prof["raster_us"] / 1e3, 1),
                      "peak": 8000.0, "unit": "GB/s",
                      "frac": round(model_bytes / prof["raster_us"] / 1e3 / 8000.0, 4),
-                     "bytes_per_launch": model_bytes, "avg_launch_us": round(prof["raster_us"], 2)},
+                     "bytes_per_launch": model_bytes, "avg_launch_us": round(prof["raster_us"], 2),
+                     "traffic": _splat_traffic()},
         "resolve_gbs": round(out_bytes / prof["resolve_us"] / 1e3, 1),
         "fill_gbs": round(fill_bytes / prof["fill_us"] / 1e3, 1) if prof["fill_us"] else None,
     }
@@ -592,6 +603,17 @@ def main() -> int:
         print(json.dumps({"error": "this build measures one GPU: the odometry path runs one frame pair per "
                                    "GPU and shards by sequence; run with --gpus 1"}))
         return 2
+    if args.calibrate_copy:
+        # four 256 MiB device copies: the known bytes that scale the HBM
+        # counters of a profiled run
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 4 x 256 MiB device copies")
+        del src, dst
     if args.splat_only:
         print(json.dumps({"splat": splat_prediction(args)}), flush=True)
         return 0
@@ -609,14 +631,6 @@ def main() -> int:
 
     name, arch = device_info()
     log(f"device {name} {arch}")
-    if args.calibrate_copy:
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy calibration: 4 x 256 MiB device copies")
-        del src, dst
     cam = synth.CAM_640
     nframes = max(args.steps, args.warmup) + 1
     scene = synth.Scene(args.seed)

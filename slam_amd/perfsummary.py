@@ -14,6 +14,10 @@
         pyramid level, from FETCH_SIZE and WRITE_SIZE counter runs (KiB per
         dispatch), each scaled by what the counter reads for the 256 MiB
         device copies of ``bench.py --calibrate-copy``.
+    python -m slam_amd.perfsummary kernel-traffic FETCH_DIR WRITE_DIR --kernel NAME --algorithmic BYTES
+        the same for every dispatch of the kernels whose name contains NAME:
+        mean FETCH_SIZE and WRITE_SIZE, copy-scaled bytes, and their ratio to
+        BYTES (the algorithmic bytes per dispatch)
 """
 from __future__ import annotations
 
@@ -176,6 +180,30 @@ def cmd_traffic(fetch_dir: str, write_dir: str) -> dict:
             "levels": levels, "traffic_bytes_per_launch_10_5_4": int(per_launch)}
 
 
+def _kernel_counter(path: str, name: str):
+    rows = list(csv.DictReader(open(path)))
+    copy = [float(r["Counter_Value"]) for r in rows
+            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    vals = [float(r["Counter_Value"]) for r in rows if name in r["Kernel_Name"]]
+    if not copy or not vals:
+        raise ValueError(f"no copy calibration or no {name} dispatches in {path}")
+    return sum(copy) / len(copy), sum(vals) / len(vals), len(vals)
+
+
+def cmd_kernel_traffic(fetch_dir: str, write_dir: str, name: str, algorithmic: float) -> dict:
+    fc, fv, fn = _kernel_counter(os.path.join(fetch_dir, "run_counter_collection.csv"), name)
+    wc, wv, wn = _kernel_counter(os.path.join(write_dir, "run_counter_collection.csv"), name)
+    fb = fv * COPY_BYTES / fc
+    wb = wv * COPY_BYTES / wc
+    return {"kernel": name, "counter_unit": "KiB per dispatch", "dispatches": [fn, wn],
+            "calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": round(fc, 1), "write_kib": round(wc, 1),
+                            "fetch_scale": round(COPY_BYTES / (1024.0 * fc), 4),
+                            "write_scale": round(COPY_BYTES / (1024.0 * wc), 4)},
+            "fetch_kib": round(fv, 1), "write_kib": round(wv, 1), "fetch_bytes": int(fb), "write_bytes": int(wb),
+            "traffic_bytes": int(fb + wb), "algorithmic_bytes": int(algorithmic),
+            "ratio": round((fb + wb) / algorithmic, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -190,6 +218,11 @@ def main() -> int:
     lp = sub.add_parser("loop")
     lp.add_argument("dir")
     lp.add_argument("--frames", type=int, nargs=2, default=(10, 200))
+    kt = sub.add_parser("kernel-traffic")
+    kt.add_argument("fetch_dir")
+    kt.add_argument("write_dir")
+    kt.add_argument("--kernel", required=True)
+    kt.add_argument("--algorithmic", type=float, required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -201,6 +234,8 @@ def main() -> int:
         print(cmd_frame(a.dir, a.mode))
     elif a.cmd == "loop":
         print(cmd_loop(a.dir, *a.frames))
+    elif a.cmd == "kernel-traffic":
+        print(json.dumps(cmd_kernel_traffic(a.fetch_dir, a.write_dir, a.kernel, a.algorithmic), indent=1))
     else:
         res = cmd_traffic(a.fetch_dir, a.write_dir)
         text = json.dumps(res, indent=1)
