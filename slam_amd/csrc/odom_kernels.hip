@@ -4,10 +4,11 @@
 // intensity, derivatives) and the Gauss-Newton steps: the fused ICP +
 // photometric-association pass (A), the photometric pass (B) and the SO(3)
 // pre-alignment.  Each step folds its per-pixel products through wave64
-// shuffles and LDS into one block partial; the last workgroup to finish (an
-// atomic ticket) reduces the partials of its pair in FP64 in a fixed order
-// and runs the 6x6 solve and SE(3) update, so a tracking iteration never
-// returns to the host.
+// shuffles and LDS into one block partial per workgroup.  The next launch on
+// the stream reduces those partials in FP64 in a fixed order in its prologue
+// (every block of the single-pair pass A, or one k_solve workgroup per pair
+// of a batch) and runs the 6x6 solve and SE(3) update there, so a tracking
+// iteration needs no device atomics and never returns to the host.
 //
 // Every expression that decides a correspondence (projection, rounding,
 // thresholds, NaN tests) is written in the reference's operation order and
