@@ -181,11 +181,17 @@ def pose_error(Ta: np.ndarray, Tb: np.ndarray) -> tuple[float, float]:
 
 
 def spline_trajectory(n: int, seed: int, max_step_m: float = 0.01, max_step_deg: float = 0.5,
-                      bursts: bool = False) -> list[np.ndarray]:
+                      bursts: bool = False, max_excursion_m: float = 0.8) -> list[np.ndarray]:
     """A smooth seeded trajectory of ``n`` camera-to-world poses around the rest
     pose: sums of sinusoids per axis, scaled so the largest per-frame step is
-    ``max_step_m`` / ``max_step_deg``.  ``bursts`` modulates the speed with
-    short fast segments (fast-motion stress)."""
+    ``max_step_m`` / ``max_step_deg`` (reached to within 1 %).  ``bursts``
+    modulates the speed with short fast segments (fast-motion stress).
+
+    The camera stays within ``max_excursion_m`` of its start on every axis,
+    so it stays well inside the room.  A long sequence would need a wider
+    excursion at the requested step; instead its sinusoids run at a higher
+    frequency (the path folds back sooner), which keeps the steps at the bound.
+    """
     rng = np.random.default_rng(seed)
     k = np.arange(n, dtype=np.float64)
     speed = np.ones(n)
@@ -195,31 +201,44 @@ def spline_trajectory(n: int, seed: int, max_step_m: float = 0.01, max_step_deg:
             speed += 6.0 * np.exp(-0.5 * ((k - c) / 6.0) ** 2)
     s = np.cumsum(speed)
     s = s / s[-1] * n
-    pos = np.zeros((n, 3))
-    ang = np.zeros((n, 3))
+    # the sinusoids' parameters, drawn once: (axis, harmonic) ->
+    # (frequency factor, position amplitude, phase, angle amplitude, phase)
+    terms = []
     for ax in range(3):
         for m in range(1, 4):
-            om = 2 * math.pi * m / max(n, 2) * rng.uniform(0.8, 1.5)
-            pos[:, ax] += rng.uniform(0.2, 0.5) / m * np.sin(om * s + rng.uniform(0, 2 * math.pi))
-            ang[:, ax] += rng.uniform(0.1, 0.3) / m * np.sin(om * s + rng.uniform(0, 2 * math.pi))
-    pos -= pos[0]
-    ang -= ang[0]
-    ang[:, 2] *= 0.3  # less roll
-    dp = np.linalg.norm(np.diff(pos, axis=0), axis=1).max() if n > 1 else 1.0
-    pos *= max_step_m / dp
+            terms.append((ax, m, rng.uniform(0.8, 1.5), rng.uniform(0.2, 0.5) / m, rng.uniform(0, 2 * math.pi),
+                          rng.uniform(0.1, 0.3) / m, rng.uniform(0, 2 * math.pi)))
+
+    def curves(fscale: float):
+        pos = np.zeros((n, 3))
+        ang = np.zeros((n, 3))
+        for ax, m, fq, pa, pp, aa, ap in terms:
+            om = 2 * math.pi * m / max(n, 2) * fq * fscale
+            pos[:, ax] += pa * np.sin(om * s + pp)
+            ang[:, ax] += aa * np.sin(om * s + ap)
+        pos -= pos[0]
+        ang -= ang[0]
+        ang[:, 2] *= 0.3  # less roll
+        return pos, ang
+
+    fscale = 1.0
+    for _ in range(30):
+        pos, ang = curves(fscale)
+        dp = np.linalg.norm(np.diff(pos, axis=0), axis=1).max() if n > 1 else 1.0
+        pos *= max_step_m / dp
+        ext = np.abs(pos).max()
+        if ext <= max_excursion_m:
+            break
+        fscale *= ext / max_excursion_m * 1.02
     Rs = [so3_exp(a) for a in ang]
     if n > 1:
         # scale to the bound; the step angle is not linear in the scale, so
-        # shrink again while it overshoots
-        for it in range(5):
+        # rescale until it lands within 1 % below the bound
+        for _ in range(30):
             dr = max(np.degrees(np.linalg.norm(so3_log(Rs[i].T @ Rs[i + 1]))) for i in range(n - 1))
-            if it > 0 and dr <= max_step_deg:
+            if 0.99 * max_step_deg <= dr <= max_step_deg:
                 break
-            ang *= max_step_deg / dr * (1.0 - 1e-6)
+            ang *= max_step_deg / dr * (1.0 - 2e-3)
             Rs = [so3_exp(a) for a in ang]
-    # bound the excursion so the camera stays well inside the room
-    ext = np.abs(pos).max()
-    if ext > 0.8:
-        pos *= 0.8 / ext
     B = base_pose()
     return [B @ make_pose(Rs[i], pos[i]) for i in range(n)]

@@ -52,3 +52,19 @@ def test_spline_step_bounds():
     steps = [synth.pose_error(poses[i + 1], poses[i]) for i in range(99)]
     assert max(s[0] for s in steps) <= 0.01 + 1e-9
     assert max(s[1] for s in steps) <= 0.5 + 1e-6
+
+
+def test_long_burst_sequence_reaches_its_step_bound_inside_the_room():
+    """The fast-motion stress sequence (2000 frames, bursts up to 10 cm /
+    8 deg per frame) realises its bound while the camera stays within the
+    0.8 m excursion: the path folds back rather than its steps shrinking."""
+    poses = synth.spline_trajectory(2000, 920, max_step_m=0.10, max_step_deg=8.0, bursts=True)
+    steps = [synth.pose_error(poses[i + 1], poses[i]) for i in range(len(poses) - 1)]
+    dt = max(s[0] for s in steps)
+    dr = max(s[1] for s in steps)
+    assert 0.9 * 0.10 <= dt <= 0.10 + 1e-9
+    assert 0.9 * 8.0 <= dr <= 8.0 + 1e-6
+    ext = max(np.abs(T[:3, 3] - poses[0][:3, 3]).max() for T in poses)
+    assert ext <= 0.8 + 1e-9
+    # most frames move at the base speed; the bursts carry the largest steps
+    assert np.median([s[0] for s in steps]) < 0.5 * dt
