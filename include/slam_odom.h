@@ -252,6 +252,31 @@ int odom_init_warmstart_input(odom_handle h, void *x);
 int odom_track_seeded(odom_handle h, const float *pred, float t_scale, float r_scale, const int iters[3],
                       int rgb_only, float icp_weight, int select);
 
+/* ---- an entry point that allocates nothing.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It is odom_icp_step on caller-provided maps: the pass's
+ * descriptor, state record, block partials and FP64 totals live in one
+ * file-scope device array of ODOM_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, and the call makes no hipMalloc/hipFree.  Every call on
+ * a GPU shares that GPU's array (its contents are rewritten by each call
+ * before they are read): calls that overlap in time on one GPU race. ---- */
+#define ODOM_NOALLOC_SCRATCH_BYTES 55296
+#define ODOM_NOALLOC_MAX_DIM 8192
+/* icpStep (reduce.cu:418-490, the JtJJtrSE3 sums of types.cuh:85-96) over
+ * device maps in the reference's layout (createVMap/createNMap, utils.cu:
+ * 109-204): planar float [3][height][width] (x, y, z planes, row pitch =
+ * width), NaN x = invalid; vcur/ncur in the current camera frame, vprev/nprev
+ * in the global frame.  fx..cy: the intrinsics of the maps' level.  out[29]
+ * as odom_icp_step.  width and height at most ODOM_NOALLOC_MAX_DIM
+ * (ODOM_ERR_ARG beyond).  Work runs on `stream` (a hipStream_t, 0 = the null
+ * stream); the call returns when out is written. */
+int odom_icp_step_noalloc(const float *vcur, const float *ncur, const float *vprev, const float *nprev,
+                          int width, int height, float fx, float fy, float cx, float cy,
+                          const float Rcurr[9], const float tcurr[3], const float Rprev_inv[9],
+                          const float tprev[3], float dist_thresh, float angle_thresh, double out[29],
+                          void *stream);
+
 /* Device name / arch of the current device, for logs. */
 int odom_device_info(char *name, int name_len, char *arch, int arch_len);
 

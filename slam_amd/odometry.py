@@ -63,7 +63,7 @@ EXPORTS = [
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
     "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
-    "odom_loop_check",
+    "odom_loop_check", "odom_icp_step_noalloc",
 ]
 
 _V = ctypes.c_void_p
@@ -128,6 +128,8 @@ def lib() -> ctypes.CDLL:
     L.odom_batch_get_step_shape.argtypes = [_V, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int)]
     L.odom_batch_profile_track.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, _V, _V, _V, _V, _P(ctypes.c_float)]
+    L.odom_icp_step_noalloc.argtypes = [_V, _V, _V, _V, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4 + \
+        [_V, _V, _V, _V, ctypes.c_float, ctypes.c_float, _V, _V]
     L._typed = True
     return L
 
