@@ -40,14 +40,15 @@ typedef struct efloop_params {
     int autotune;         /* after the first tracking call, time the step kernels' launch
                              shapes on its inputs and keep the fastest (odom_autotune).  The
                              shapes set the order of the block sums, and the choice follows
-                             timing, so poses can differ between runs in their last bits;
-                             0 keeps the fixed default shapes (reproducible runs). */
+                             timing, so poses can then differ between runs in their last
+                             bits; 0 (the default) keeps the library's table of shapes and
+                             reproducible runs. */
 } efloop_params;
 
 typedef struct efloop_ctx *efloop_handle;
 
 /* The reference configs' values (ef_file.cfg): 3 m, 20 m, 10, 200, icpWeight
- * 10, SO3 on; 4M surfels; graphs and autotuning on. */
+ * 10, SO3 on; 4M surfels; graphs on, autotuning off. */
 int efloop_default_params(efloop_params *p);
 int efloop_create(int width, int height, float cx, float cy, float fx, float fy, const efloop_params *p,
                   efloop_handle *out);
@@ -61,6 +62,40 @@ int efloop_process(efloop_handle h, const odom_frame_view *rgb, const odom_frame
  * fusion weighting. */
 int efloop_get_handles(efloop_handle h, odom_handle *odom, splat_handle *prediction, surfel_handle *model);
 int efloop_get_state(efloop_handle h, int *tick, float *last_weighting, float *last_icp_error);
+
+/* ---- n sequences in lockstep (no counterpart in the reference, whose frame
+ * loop tracks one sequence): every sequence has its own loop (model,
+ * prediction, odometry handle and stream) and the same per-frame work as
+ * efloop_process, except that the n tracking calls of a frame are one batched
+ * call (odom_track_batch_policy, under `pol`), and, with a network set, the
+ * learned start's input and forward pass (slam_cnn.h) run once for all n
+ * sequences.  A sequence's results are bit-identical to the same sequence
+ * run as a batch of one.  Autotuning is not done (the shapes are the
+ * library's table). ---- */
+typedef struct efloop_batch_s *efloop_batch;
+
+/* Per sequence, since creation. */
+typedef struct efloop_counters {
+    int frames_tracked;        /* tracking calls */
+    long long gn_iterations;   /* Gauss-Newton iterations of the kept refinements, summed */
+    long long alt_iterations;  /* Gauss-Newton iterations of refinements judged away, summed */
+    int predictions;           /* tracking calls started from the network's prediction */
+    int predictions_kept;      /* ... whose refinement the judge kept */
+} efloop_counters;
+
+int efloop_batch_create(int n, int width, int height, float cx, float cy, float fx, float fy,
+                        const efloop_params *p, const odom_policy *pol, efloop_batch *out);
+int efloop_batch_destroy(efloop_batch b);
+/* The learned start: the warm-start network's parameters (host fp32, the
+ * layout of cnn_param_layout) and output scales.  640x480 sequences only.
+ * Without it each call starts as `pol` says. */
+int efloop_batch_set_net(efloop_batch b, const float *params, size_t nparams, float t_scale, float r_scale);
+/* One frame of every sequence: rgb[i], depth[i] as in efloop_process;
+ * poses_out: n row-major 4x4 poses. */
+int efloop_batch_process(efloop_batch b, const odom_frame_view *rgb, const odom_frame_view *depth,
+                         float *poses_out);
+int efloop_batch_get_loop(efloop_batch b, int i, efloop_handle *out);
+int efloop_batch_get_counters(efloop_batch b, int i, efloop_counters *out);
 
 #ifdef __cplusplus
 }

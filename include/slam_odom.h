@@ -67,8 +67,9 @@ typedef struct odom_stats {
     int gn_iterations;         /* Gauss-Newton iterations run */
     int degenerate;            /* last solve had no information (identity update) */
     int reverted;              /* the 0.3 m revert was taken */
-    int seed_used;             /* odom_track_seeded: the last call started from the prediction */
-    int seed_calls, seed_kept; /* seeded calls so far, and how many kept the prediction */
+    int seed_used;             /* odom_track_seeded / odom_track_policy: the last call kept the prediction */
+    int seed_calls, seed_kept; /* predicted calls so far, and how many kept the prediction */
+    int alt_iterations;        /* a judged policy call: Gauss-Newton iterations of the refinement not kept */
 } odom_stats;
 
 typedef struct odom_ctx *odom_handle;
@@ -196,6 +197,44 @@ int odom_autotune(odom_handle h, int rgb_only, float icp_weight);
  * singular last system (no covariance) is not accepted.  Synchronous. */
 int odom_loop_check(odom_handle h, double cov_thresh, float icp_count_thresh, float icp_err_thresh, int *ok);
 
+/* ---- tracking policies: the start, the iteration schedule and the
+ * convergence test of a call, and the judge of a learned start (no
+ * counterpart in the reference, whose getIncrementalTransformation always
+ * starts from the previous pose, optionally rotated by the SO3
+ * pre-alignment, and runs a fixed 10/5/4 schedule, RGBDOdometryef.cpp:
+ * 382-420).  With the default policy odom_track_policy(h, NULL, ...) is
+ * getIncrementalTransformation(rgbOnly, icpWeight, pyramid, !fastOdom, so3). ---- */
+#define ODOM_START_PREVIOUS 0  /* the previous pose (the reference's start) */
+#define ODOM_START_VELOCITY 1  /* constant velocity: the previous pose moved by the last call's increment */
+#define ODOM_JUDGE_NONE 0      /* a prediction is kept */
+#define ODOM_JUDGE_COARSE 1    /* the prediction and the constant-velocity start are both refined through
+                                  the level-2 iterations; the better one continues through levels 1 and 0 */
+#define ODOM_JUDGE_FULL 2      /* both are refined through every level; the better one is the result */
+typedef struct odom_policy {
+    int iters[3];              /* Gauss-Newton iterations at levels 0, 1, 2 (at most 63 each) */
+    float conv_t, conv_r;      /* a level stops after a solve whose update is below both (m, rad);
+                                  conv_t = 0: every scheduled iteration runs */
+    int start;                 /* ODOM_START_* when there is no prediction */
+    int so3;                   /* the SO3 pre-alignment (only with ODOM_START_PREVIOUS and no prediction) */
+    int judge;                 /* ODOM_JUDGE_*, for a prediction */
+    float judge_tol;           /* the prediction falls back when its refinement's ICP residual
+                                  (lastICPError) is more than judge_tol above, or its inlier count
+                                  more than judge_tol below, the constant-velocity start's
+                                  (photometric error and count for RGB-only calls) */
+} odom_policy;
+/* iters 10/5/4, no convergence test, start previous, so3 on, judge full, tol 0.01 */
+int odom_default_policy(odom_policy *p);
+/* getIncrementalTransformation under a policy, asynchronously (the prior is
+ * the handle's device-resident pose, as odom_track_async).  pred: a device
+ * prediction of the camera motion since the previous pose (6 floats, t /
+ * t_scale and rotation vector / r_scale, slam_cnn.h) or NULL.  With a
+ * prediction and a judge, the constant-velocity start is refined beside it
+ * in the same launches (the handle's second state) and the judge keeps the
+ * better; odom_get_stats reports whether the prediction was kept and the
+ * iterations of both refinements. */
+int odom_track_policy(odom_handle h, const float *pred, float t_scale, float r_scale, const odom_policy *pol,
+                      int rgb_only, float icp_weight);
+
 /* ---- batches: many frame pairs (handles, of any size) tracked by the same
  * launches.  No counterpart in the reference, whose RGBDOdometryef tracks one
  * pair per object and call (RGBDOdometryef.cpp:267-595); this is the batched
@@ -214,7 +253,9 @@ int odom_batch_destroy(odom_batch b);
 int odom_track_batch(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3);
 int odom_batch_synchronize(odom_batch b);
 /* Blocks per step launch over the whole batch (default: 4 per compute unit),
- * shared out by pixel count, at most 256 per pair. */
+ * shared out by pixel count, at most 256 per pair.  0: every pair gets the
+ * block count of a single-pair call (its sums are then bit-identical to the
+ * pair tracked alone). */
 int odom_batch_set_blocks(odom_batch b, int target_blocks);
 int odom_batch_get_blocks(odom_batch b, int kernel, int level, int *total_blocks);
 int odom_batch_set_step_shape(odom_batch b, int kernel, int level, int threads, int pixels_per_thread);
@@ -232,6 +273,18 @@ int odom_batch_set_graph(odom_batch b, int enable);
 /* As odom_profile_track, for one batched call. */
 int odom_batch_profile_track(odom_batch b, int rgb_only, float icp_weight, int pyramid, int fast_odom, int so3,
                              float us_a[3], float us_b[3], int n_a[3], int n_b[3], float *us_total);
+/* The batch's HIP stream (hipStream_t): a caller enqueues the warm-start
+ * network (slam_cnn.h) there between odom_batch_warmstart_input and
+ * odom_track_batch_policy. */
+int odom_batch_get_stream(odom_batch b, void **stream);
+/* The warm-start network input of every pair (640x480 handles only): bf16
+ * [n][60][80][4] at device pointer x, sample i from handle i, on the batch's
+ * stream after each handle's preceding work. */
+int odom_batch_warmstart_input(odom_batch b, void *x);
+/* odom_track_policy on every pair of the batch; pred: 6 floats per handle
+ * (device) or NULL. */
+int odom_track_batch_policy(odom_batch b, const float *pred, float t_scale, float r_scale, const odom_policy *pol,
+                            int rgb_only, float icp_weight);
 
 /* ---- learned warm start (no counterpart in the reference, whose tracker
  * always starts from the previous pose, RGBDOdometryef.cpp:267-401) ---- */

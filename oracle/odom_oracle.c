@@ -790,6 +790,13 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
 void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
                   int so3, const double *seed)
 {
+    orc_track_run(o, t, R, rgbOnly, icpWeight, iters, so3, seed, 0.0f, 0.0f, NPYR - 1, 0, NULL, NULL);
+}
+
+void orc_track_run(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
+                   int so3, const double *seed, float convT, float convR, int hi, int lo, double resultRt_out[16],
+                   int *iters_done)
+{
     const int icp = !rgbOnly && icpWeight > 0;
     const int rgb = rgbOnly || icpWeight < 100;
     const float sobelScale = (float)(1.0 / pow(2.0, 3)); /* sobelSize 3, :34-35 */
@@ -805,6 +812,7 @@ void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWei
 
     double resultR[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     o->so3_iters = 0;
+    if (hi != NPYR - 1) so3 = 0;    /* the SO3 pre-alignment precedes the coarsest level */
     if (so3) { /* :294-380 */
         const int lvl = 2;
         float R_lr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -872,7 +880,8 @@ void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWei
     }
 
     float residual[2] = {0.0f, 0.0f};
-    for (int i = NPYR - 1; i >= 0; --i) {
+    int ndone = 0;
+    for (int i = hi; i >= lo; --i) {
         const orc_intr ki = orc_level_intr(o->intr, i);
         const int pw = o->pw[i], ph = o->ph[i];
         double K[9], Ki[9];
@@ -967,16 +976,39 @@ void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWei
             memcpy(resultRt, nrt, sizeof nrt);
             /* rgbOdom.inverse(): [R^T | -R^T t]; currentT = [Rprev|tprev] * inv (:567-575) */
             pose_from_result(resultRt, Rprev, tprev, Rcurr, tcurr);
+            ++ndone;
+            /* the convergence policy (no counterpart in the reference, whose
+             * schedule is fixed, :382-384, 420): an update below both
+             * thresholds ends the level */
+            if (convT > 0.0f) {
+                const double nt = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const double nr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+                if (nt < (double)convT && nr < (double)convR) break;
+            }
         }
+    }
+    if (iters_done) *iters_done = ndone;
+    if (lo > 0) {
+        /* a partial run (the coarse levels of a judged call): no revert,
+         * no image swap; the increment continues in a later run */
+        if (resultRt_out) memcpy(resultRt_out, resultRt, sizeof resultRt);
+        memcpy(t, tcurr, sizeof tcurr);
+        memcpy(R, Rcurr, sizeof Rcurr);
+        return;
     }
 
     /* :579-583 */
+    int reverted = 0;
     if (rgb) {
         const float d0 = tcurr[0] - tprev[0], d1 = tcurr[1] - tprev[1], d2 = tcurr[2] - tprev[2];
         if (sqrtf(d0 * d0 + d1 * d1 + d2 * d2) > 0.3f) {
             memcpy(Rcurr, Rprev, sizeof Rcurr);
             memcpy(tcurr, tprev, sizeof tcurr);
+            reverted = 1;
         }
+    }
+    if (resultRt_out) {
+        for (int a = 0; a < 16; ++a) resultRt_out[a] = reverted ? ((a % 5 == 0) ? 1.0 : 0.0) : resultRt[a];
     }
     /* :585-591 */
     if (so3)

@@ -161,6 +161,17 @@ void orc_track(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight
  * the warm-start form. */
 void orc_track_ex(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
                   int so3, const double *seed);
+/* orc_track_ex over levels hi..lo (2..0 is the whole call) with the
+ * convergence policy of the library's odom_track_policy (no counterpart in
+ * the reference, whose schedule is fixed): a level ends after a solve whose
+ * update has |dt| < convT and |dw| < convR (convT 0: off).  resultRt_out
+ * (optional): the increment at the end (identity after the 0.3 m revert);
+ * iters_done (optional): solves run.  With lo > 0 the run stops after level
+ * lo without the revert or the image swap, t/R = the current estimate, and
+ * a later run continues from resultRt_out as its seed. */
+void orc_track_run(orc_odom *o, float t[3], float R[9], int rgbOnly, float icpWeight, const int iters[3],
+                   int so3, const double *seed, float convT, float convR, int hi, int lo, double resultRt_out[16],
+                   int *iters_done);
 /* stats[0..5] = lastICPError, lastICPCount, lastRGBError, lastRGBCount,
  * lastSO3Error, lastSO3Count; stats[6] = number of SO3 iterations run. */
 void orc_get_stats(const orc_odom *o, float stats[7]);

@@ -70,6 +70,8 @@ def lib():
         L.orc_get_covariance.restype = ctypes.c_int
         L.orc_get_so3_rotation.argtypes = [_V, _V]
         L.orc_track_ex.argtypes = [_V, _V, _V, ctypes.c_int, ctypes.c_float, _V, ctypes.c_int, _V]
+        L.orc_track_run.argtypes = [_V, _V, _V, ctypes.c_int, ctypes.c_float, _V, ctypes.c_int, _V, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_int, ctypes.c_int, _V, _V]
         for n, rt in (("orc_buf_f32", _F), ("orc_buf_u8", _U8), ("orc_buf_s16", _S16), ("orc_buf_u16", _U16)):
             getattr(L, n).restype = rt
             getattr(L, n).argtypes = [_V, ctypes.c_char_p, ctypes.c_int]
@@ -324,6 +326,9 @@ class Odometry:
         self.w, self.h = width, height
         self._h = lib().orc_create(width, height, cx, cy, fx, fy, distThresh, angleThresh)
         lib().orc_set_precision(self._h, int(fp64))
+        # the increment of the last call (resultRt convention), which the
+        # constant-velocity start repeats; identity at a sequence's start
+        self.lastRt = np.eye(4)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -353,13 +358,88 @@ class Odometry:
     def initFirstRGB(self, rgb):
         r = _c(rgb, np.uint8)
         lib().orc_init_first_rgb(self._h, _p(r), r.shape[2])
+        self.lastRt = np.eye(4)
 
     def getIncrementalTransformation(self, trans, rot, rgbOnly=False, icpWeight=10.0, pyramid=True,
                                      fastOdom=False, so3=True):
+        iters = (3 if fastOdom else 10, 5 if pyramid else 0, 4 if pyramid else 0)
+        t, R, _ = self.run(trans, rot, iters, rgbOnly, icpWeight, so3)
+        return t, R
+
+    def run(self, trans, rot, iters=(10, 5, 4), rgbOnly=False, icpWeight=10.0, so3=False, seed=None,
+            conv=(0.0, 0.0), levels=(2, 0)):
+        """orc_track_run: levels hi..lo from the seeded increment with the
+        convergence test conv = (m, rad).  Returns t, R and a dict with the
+        increment at the end, the solves run and the last iteration's
+        statistics.  A run that ends at level 0 records its increment as the
+        last call's (the constant-velocity start of the next call)."""
         t = _c(trans, np.float32).copy()
         R = _c(rot, np.float32).copy()
-        lib().orc_track(self._h, _p(t), _p(R), int(rgbOnly), icpWeight, int(pyramid), int(fastOdom), int(so3))
-        return t, R
+        it = np.ascontiguousarray(iters, dtype=np.int32)
+        sd = None if seed is None else _c(seed, np.float64)
+        rt = np.zeros(16, np.float64)
+        nd = ctypes.c_int(0)
+        lib().orc_track_run(self._h, _p(t), _p(R), int(rgbOnly), icpWeight, _p(it), int(so3),
+                            None if sd is None else _p(sd), float(conv[0]), float(conv[1]), int(levels[0]),
+                            int(levels[1]), _p(rt), ctypes.byref(nd))
+        info = dict(self.stats(), resultRt=rt.reshape(4, 4), iterations=nd.value)
+        if levels[1] == 0:
+            self.lastRt = rt.reshape(4, 4).copy()
+        return t, R, info
+
+    def track_policy(self, trans, rot, pred=None, t_scale=0.05, r_scale=0.05, iters=(10, 5, 4), conv=(0.0, 0.0),
+                     start="previous", so3=False, judge="full", tol=0.01, rgbOnly=False, icpWeight=10.0):
+        """The library's odom_track_policy restated: the start (the predicted
+        motion pred = (t / t_scale, rotation vector / r_scale), else the
+        previous pose or the constant-velocity extrapolation of the last
+        call's increment), the schedule with the convergence test, and the
+        judge of a prediction against the constant-velocity start: both are
+        refined through level 2 ("coarse") or every level ("full"), and the
+        prediction falls back when its refinement's ICP residual is more than
+        tol above, or its inlier count more than tol below, the other's.
+        Returns t, R and a dict (kept, iterations, alt_iterations)."""
+        cv = self.lastRt.copy()
+        if pred is None:
+            seed = cv if start == "velocity" else None
+            t, R, info = self.run(trans, rot, iters, rgbOnly, icpWeight, so3 and seed is None, seed, conv)
+            return t, R, {"kept": None, "iterations": info["iterations"], "alt_iterations": 0}
+        q = np.asarray(pred, np.float64).reshape(6)
+        M = np.eye(4)
+        M[:3, :3] = rodrigues(q[3:] * r_scale)
+        M[:3, 3] = q[:3] * t_scale
+        # the seed is the inverse motion in the resultRt convention, built as
+        # the device builds it: [R^T | -R^T t] of the float prediction
+        pr = np.asarray(pred, np.float32).astype(np.float64).reshape(6)
+        Rm = rodrigues(pr[3:] * np.float64(np.float32(r_scale)))
+        tm = pr[:3] * np.float64(np.float32(t_scale))
+        seed = np.eye(4)
+        seed[:3, :3] = Rm.T
+        seed[:3, 3] = -(Rm.T @ tm)
+        if judge in (None, "none"):
+            t, R, info = self.run(trans, rot, iters, rgbOnly, icpWeight, False, seed, conv)
+            return t, R, {"kept": True, "iterations": info["iterations"], "alt_iterations": 0}
+
+        def worse(a, b):
+            ka = ("lastICPError", "lastICPCount") if not rgbOnly and icpWeight > 0 else ("lastRGBError", "lastRGBCount")
+            ea, ca, eb, cb = (np.float32(a[ka[0]]), np.float32(a[ka[1]]), np.float32(b[ka[0]]),
+                              np.float32(b[ka[1]]))
+            one = np.float32(1.0)
+            tl = np.float32(tol)
+            return bool(ea > eb * (one + tl) or ca < cb * (one - tl))
+
+        hi_lo = (2, 2) if judge == "coarse" and (iters[0] > 0 or iters[1] > 0) and iters[2] > 0 else (2, 0)
+        tp, Rp, ip = self.run(trans, rot, iters, rgbOnly, icpWeight, False, seed, conv, hi_lo)
+        tv, Rv, iv = self.run(trans, rot, iters, rgbOnly, icpWeight, False, cv, conv, hi_lo)
+        fall = worse(ip, iv)
+        win, lose = (iv, ip) if fall else (ip, iv)
+        if hi_lo[1] == 0:
+            t, R = (tv, Rv) if fall else (tp, Rp)
+            self.lastRt = win["resultRt"].copy()
+            return t, R, {"kept": not fall, "iterations": win["iterations"], "alt_iterations": lose["iterations"]}
+        # the finer levels continue from the kept refinement's increment
+        t, R, info = self.run(trans, rot, iters, rgbOnly, icpWeight, False, win["resultRt"], conv, (1, 0))
+        return t, R, {"kept": not fall, "iterations": win["iterations"] + info["iterations"],
+                      "alt_iterations": lose["iterations"]}
 
     def track_ex(self, trans, rot, iters=(10, 5, 4), rgbOnly=False, icpWeight=10.0, so3=False, seed=None):
         """Explicit schedule and optional seeded increment (4x4, resultRt

@@ -515,6 +515,16 @@ int fwd_splits(int M, int N, int K)
     return std::max(1, std::min(64 / tiles, K / 128));
 }
 
+// The split count of a forward GEMM of N samples with `rows` output rows
+// each.  For inference-sized batches (N <= 64) it is chosen from one
+// sample's rows, so the k-chunks and their summation order -- and with them
+// a sample's output, bit for bit -- do not depend on the batch the sample
+// runs in; training batches choose from all rows.
+int fwd_splits_n(int N, int rows, int cols, int K)
+{
+    return fwd_splits(N <= 64 ? rows : N * rows, cols, K);
+}
+
 int splits_for(int K, int MN)
 {
     // enough blocks along the reduction when the output tile count is small
@@ -698,17 +708,18 @@ int cnn_forward(cnn_net c, const void* x, int N, float* out)
         hipLaunchKernelGGL(k_im2col, g1(tot), dim3(256), 0, c->stream, in, N, s.hi, s.wi, s.cin, s.ho, s.wo, c->Kp[l],
                            c->xcol[l]);
         if ((rc = gemm(c, false, c->xcol[l], c->Kp[l], false, c->pb + c->convW[l].off, s.cout, nullptr, c->y[l], s.cout,
-                       M, s.cout, c->Kp[l], c->p + c->convB[l].off, true, fwd_splits(M, s.cout, c->Kp[l]))))
+                       M, s.cout, c->Kp[l], c->p + c->convB[l].off, true,
+                       fwd_splits_n(N, s.ho * s.wo, s.cout, c->Kp[l]))))
             return rc;
         in = c->y[l];
     }
     hipLaunchKernelGGL(k_gap_fwd, g1((size_t)N * kFc1In), dim3(256), 0, c->stream, c->y[kConvs - 1], N, 2, kFc1In,
                        c->gap);
     if ((rc = gemm(c, false, c->gap, kFc1In, false, c->pb + c->fc1W.off, kFc1Out, nullptr, c->h1, kFc1Out, N, kFc1Out,
-                   kFc1In, c->p + c->fc1B.off, true, fwd_splits(N, kFc1Out, kFc1In))))
+                   kFc1In, c->p + c->fc1B.off, true, fwd_splits_n(N, 1, kFc1Out, kFc1In))))
         return rc;
     if ((rc = gemm(c, false, c->h1, kFc1Out, false, c->pb + c->fc2W.off, kOutPad, c->out, nullptr, kOutPad, N, kOutPad,
-                   kFc1Out, c->p + c->fc2B.off, false, fwd_splits(N, kOutPad, kFc1Out))))
+                   kFc1Out, c->p + c->fc2B.off, false, fwd_splits_n(N, 1, kOutPad, kFc1Out))))
         return rc;
     if (out) CK(hipMemcpy2DAsync(out, kOut * 4, c->out, kOutPad * 4, kOut * 4, N, hipMemcpyDeviceToDevice, c->stream));
     return launch_check("forward");

@@ -72,6 +72,11 @@ struct IterState {
     int pad_;
 };
 
+// Hypotheses of one tracking call: a pair's main state (the call's result)
+// and, when a start is judged against the constant-velocity start, the state
+// of that second refinement (same inputs, same prior).
+constexpr int kMaxHyp = 2;
+
 // State of the SO3 pre-alignment loop (RGBDOdometryef.cpp:294-380) between
 // its launches; iteration k lives in slot k & 1.
 struct So3Slot {
@@ -110,6 +115,12 @@ struct TrackState {
     // counters of seeded calls and kept predictions
     double seedRt[16];
     int seed_used, seed_calls, seed_kept;
+    // the increment of the last completed call (resultRt convention: the
+    // pose moved from prev to prev * lastRt^-1; identity after a revert or
+    // a first frame), from which the constant-velocity start extrapolates
+    double lastRt[16];
+    int alt_iters;                   // iterations of the judged-away hypothesis
+    int judge_level;                 // level after which the last judge ran (-1 none)
 };
 
 // Uniform options of one track call.
@@ -120,7 +131,16 @@ struct TrackParams {
     float sobelScale;
     float maxDepthDelta;
     int iters[kLevels];
-    int seeded;                      // start from TrackState::seedRt (warm start)
+    int seeded;                      // start from TrackState::seedRt: 1 the warm start,
+                                     // 2 the warm start if its level-2 association keeps it,
+                                     // 3 a policy call's start (k_seed_policy; counted there
+                                     // or by the judge)
+    // convergence policy: a level stops after a solve whose update has
+    // |dt| < convT (m) and |dw| < convR (rad); 0 = the fixed schedule
+    float convT, convR;
+    float judgeTol;                  // a judged start falls back when its ICP residual is
+                                     // more than this fraction above, or its inlier count
+                                     // this fraction below, the constant-velocity start's
 };
 
 // Per pair: level maps, state, scratch.

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <vector>
@@ -41,6 +42,46 @@ int fmt_bpp(int fmt)
         case ODOM_FMT_RGBA32F: return 16;
         default: return 0;
     }
+}
+
+// Launch shapes of the two Gauss-Newton step kernels per level (threads,
+// pixels per thread): the role of the reference's GPUConfig table
+// (GPUConfig.h:30-98), for gfx950, the only target of this library.  The
+// rows were measured with odom_autotune / odom_batch_autotune on MI355X at
+// the benchmark image sizes (the modal choice over repeated runs) and are
+// fixed here, so a handle's shapes -- which set the order of its FP32 block
+// sums -- depend on its image size only, never on run-time timing.  A size
+// without a row takes the row of the nearest pixel count.
+struct ShapeRow {
+    int w, h;
+    StepShape a[kLevels], b[kLevels];   // ICP/association pass, photometric pass; levels 0, 1, 2
+};
+const ShapeRow kShapeTable[] = {
+    {320, 240, {{512, 1}, {256, 1}, {256, 1}}, {{512, 1}, {256, 1}, {256, 1}}},
+    {640, 480, {{512, 3}, {512, 1}, {256, 1}}, {{512, 3}, {512, 1}, {256, 1}}},
+    {848, 480, {{512, 4}, {512, 1}, {256, 1}}, {{512, 4}, {512, 1}, {256, 1}}},
+};
+// The batches' row: the 64-pair workload of uneven sizes (16 at 320x240, 32
+// at 640x480, 16 at 848x480) with 4 blocks per compute unit.
+const ShapeRow kBatchShapes = {0, 0, {{512, 2}, {256, 3}, {256, 1}}, {{512, 2}, {512, 2}, {512, 3}}};
+
+StepShape table_shape(int w, int h, int kernel, int level)
+{
+    const ShapeRow* best = &kShapeTable[0];
+    long long bd = -1;
+    for (const ShapeRow& r : kShapeTable) {
+        const long long d = std::llabs((long long)r.w * r.h - (long long)w * h);
+        if (bd < 0 || d < bd) {
+            bd = d;
+            best = &r;
+        }
+    }
+    return kernel == 0 ? best->a[level] : best->b[level];
+}
+
+StepShape batch_shape(int kernel, int level)
+{
+    return kernel == 0 ? kBatchShapes.a[level] : kBatchShapes.b[level];
 }
 
 }  // namespace
@@ -72,6 +113,14 @@ struct odom_ctx {
     float* part_a[2] = {nullptr, nullptr};
     float* part_b[2] = {nullptr, nullptr};
     unsigned int* part_ai[2] = {nullptr, nullptr};
+    // the second refinement of a judged call (odom_track_policy): its own
+    // state and block partials over the same pyramids
+    TrackState* st_alt = nullptr;
+    float* alt_a[2] = {nullptr, nullptr};
+    float* alt_b[2] = {nullptr, nullptr};
+    unsigned int* alt_ai[2] = {nullptr, nullptr};
+    PairDesc h_pair_alt[2]{};
+    odom_batch_s* self_batch = nullptr;   // this handle's policy calls, as a batch of one
     double* d_out11 = nullptr;
     float* d_c27 = nullptr;
     int maxblk = 0;
@@ -121,6 +170,7 @@ uint8_t* next_img(odom_ctx* c, int l) { return c->img[c->parity][l]; }
 uint8_t* last_next_img(odom_ctx* c, int l) { return c->img[c->parity ^ 1][l]; }
 
 void drop_graphs(odom_ctx* c);
+void drop_self_batch(odom_ctx* c);
 
 int upload_pair(odom_ctx* c)
 {
@@ -156,6 +206,14 @@ int upload_pair(odom_ctx* c)
         D.so3_last = c->img[q ^ 1][2];
         D.so3_next = c->img[q][2];
         HIPCHK(c, hipMemcpyAsync(c->d_pair[q], &D, sizeof(PairDesc), hipMemcpyHostToDevice, c->stream));
+        PairDesc& E = c->h_pair_alt[q];
+        E = D;
+        E.st = c->st_alt;
+        for (int i = 0; i < 2; ++i) {
+            E.part_a[i] = c->alt_a[i];
+            E.part_b[i] = c->alt_b[i];
+            E.part_ai[i] = c->alt_ai[i];
+        }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ODOM_OK;
@@ -476,8 +534,8 @@ int odom_create(const odom_params* p, odom_handle* out)
         c->fy[l] = p->fy / (float)div;
         c->cx[l] = p->cx / (float)div;
         c->cy[l] = p->cy / (float)div;
-        c->shape_a[l] = StepShape{256, 2};
-        c->shape_b[l] = StepShape{256, 2};
+        c->shape_a[l] = table_shape(p->width, p->height, 0, l);
+        c->shape_b[l] = table_shape(p->width, p->height, 1, l);
     }
     c->maxblk = kMaxBlk;
     // one arena for every device buffer
@@ -512,7 +570,11 @@ int odom_create(const odom_params* p, odom_handle* out)
         reqs.push_back({(void**)&c->part_a[i], (size_t)c->maxblk * kPartStride * 4});
         reqs.push_back({(void**)&c->part_b[i], (size_t)c->maxblk * kPartStride * 4});
         reqs.push_back({(void**)&c->part_ai[i], (size_t)c->maxblk * 2 * 4});
+        reqs.push_back({(void**)&c->alt_a[i], (size_t)c->maxblk * kPartStride * 4});
+        reqs.push_back({(void**)&c->alt_b[i], (size_t)c->maxblk * kPartStride * 4});
+        reqs.push_back({(void**)&c->alt_ai[i], (size_t)c->maxblk * 2 * 4});
     }
+    reqs.push_back({(void**)&c->st_alt, sizeof(TrackState)});
     reqs.push_back({(void**)&c->d_out11, 16 * 8});
     reqs.push_back({(void**)&c->d_c27, 32 * 4});
     size_t total = 0;
@@ -542,7 +604,10 @@ int odom_create(const odom_params* p, odom_handle* out)
     s.lastSO3Count = (float)(p->width * p->height);
     s.pose_out[0] = s.pose_out[5] = s.pose_out[10] = 1.0f;
     for (int k = 0; k < 9; ++k) s.so3R[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 16; ++k) s.lastRt[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    s.judge_level = -1;
     if (hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->st_alt, &s, sizeof s, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         upload_pair(c) != ODOM_OK) {
         odom_destroy(c);
         return ODOM_ERR_HIP;
@@ -556,6 +621,7 @@ int odom_destroy(odom_handle c)
 {
     if (!c) return ODOM_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_self_batch(c);
     drop_graphs(c);
     if (c->arena) (void)hipFree(c->arena);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -663,6 +729,10 @@ int odom_init_first_rgb(odom_handle c, const odom_frame_view* rgb)
 {
     if (!c) return ODOM_ERR_ARG;
     uint8_t* dst[kLevels] = {last_next_img(c, 0), last_next_img(c, 1), last_next_img(c, 2)};
+    // a sequence starts: no motion yet to extrapolate
+    double eye[16];
+    for (int k = 0; k < 16; ++k) eye[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    if (int rc = write_field(c, c->st->lastRt, eye, 16)) return rc;
     return enqueue_populate(c, rgb, nullptr, dst);
 }
 
@@ -753,6 +823,7 @@ int odom_set_graph(odom_handle c, int enable)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->use_graph = enable != 0;
     if (!c->use_graph) drop_graphs(c);
+    drop_self_batch(c);
     return ODOM_OK;
 }
 
@@ -817,6 +888,7 @@ int slam::odom_stats_last_read(odom_handle c, odom_stats* s)
     s->seed_used = t.seed_used;
     s->seed_calls = t.seed_calls;
     s->seed_kept = t.seed_kept;
+    s->alt_iterations = t.alt_iters;
     return ODOM_OK;
 }
 
@@ -997,6 +1069,7 @@ int odom_set_step_shape(odom_handle c, int kernel, int level, int threads, int p
     if (!step_shape_supported(sh)) return ODOM_ERR_ARG;
     if (kernel == 0) c->shape_a[level] = sh;
     else c->shape_b[level] = sh;
+    drop_self_batch(c);
     return upload_pair(c);
 }
 
@@ -1111,9 +1184,33 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
     if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
     const int threads[] = {256, 512};
     const int ppts[] = {1, 2, 3, 4};
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    // the events live until the function returns, and on any failure the
+    // shapes in place when the call started are restored
+    struct Guard {
+        odom_ctx* c;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        StepShape a[kLevels], b[kLevels];
+        bool done = false;
+        ~Guard()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (!done) {
+                for (int l = 0; l < kLevels; ++l) {
+                    c->shape_a[l] = a[l];
+                    c->shape_b[l] = b[l];
+                }
+                drop_self_batch(c);
+                (void)upload_pair(c);
+            }
+        }
+    } g{c};
+    for (int l = 0; l < kLevels; ++l) {
+        g.a[l] = c->shape_a[l];
+        g.b[l] = c->shape_b[l];
+    }
+    HIPCHK(c, hipEventCreate(&g.e0));
+    HIPCHK(c, hipEventCreate(&g.e1));
     int rc = odom_compute_derivatives(c);
     if (rc) return rc;
     const int zero = 0;
@@ -1128,22 +1225,21 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
                     if ((rc = odom_set_step_shape(c, kernel, level, t, q))) return rc;
                     const int reps = 20;
                     for (int r = 0; r < reps + 3; ++r) {
-                        if (r == 3) HIPCHK(c, hipEventRecord(e0, c->stream));
+                        if (r == 3) HIPCHK(c, hipEventRecord(g.e0, c->stream));
                         if (kernel == 0) launch_step_a(step_launch(c, 0, level), level, 0, 0, -1, P, 1, nullptr, c->stream);
                         else launch_step_b(step_launch(c, 1, level), level, 0, 0, P, 1, c->stream);
                     }
-                    HIPCHK(c, hipEventRecord(e1, c->stream));
-                    HIPCHK(c, hipEventSynchronize(e1));
+                    HIPCHK(c, hipEventRecord(g.e1, c->stream));
+                    HIPCHK(c, hipEventSynchronize(g.e1));
                     float ms = 0.0f;
-                    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+                    HIPCHK(c, hipEventElapsedTime(&ms, g.e0, g.e1));
                     if ((rc = check_launch(c))) return rc;
                     if (ms < best) { best = ms; best_sh = StepShape{t, q}; }
                 }
             if ((rc = odom_set_step_shape(c, kernel, level, best_sh.threads, best_sh.ppt))) return rc;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    g.done = true;
     return ODOM_OK;
 }
 
@@ -1151,25 +1247,36 @@ int odom_autotune(odom_handle c, int rgbOnly, float icpWeight)
 
 // ================================================================ batches
 // Many handles (frame pairs of any size) tracked by the same launches: one
-// launch per pass and Gauss-Newton iteration covers every pair, each pair
-// owning a share of the blocks proportional to its pixel count (at most
-// kMaxBlk).  The per-pair state, partials and descriptors are the handles'
-// own; the batch holds the descriptor arrays with its block split, the
-// block-to-pair tables and a stream.
+// launch per pass and Gauss-Newton iteration covers every pair.  A call runs
+// one refinement per handle, or two (a judged policy call: the prediction's
+// and the constant-velocity start's) through the levels before its judge;
+// each refinement is an entry with its own state and block partials.  The
+// entries share a launch's blocks in proportion to their pixel counts (at
+// most kMaxBlk each), or, with target 0, each gets the block count of a
+// single-pair call, which makes a batched entry's sums bit-identical to the
+// same pair tracked alone.  The batch holds the descriptor arrays with its
+// block split, the block-to-entry tables and a stream (a handle's own policy
+// calls run as a batch of one on the handle's stream).
+struct BatchPlan {
+    int* d_blk[2][kLevels] = {};
+    int total[2][kLevels] = {};
+    std::vector<int> nblk[2][kLevels], blk0[2][kLevels];   // per entry
+    std::map<std::vector<int>, PairDesc*> slots;           // descriptor arrays by the pairs' image parities
+    bool built = false;
+};
+
 struct odom_batch_s {
     std::vector<odom_ctx*> h;
     hipStream_t stream = nullptr;
+    bool own_stream = true;          // false: the single handle's stream, no cross-stream events
     hipEvent_t done = nullptr;
     std::vector<hipEvent_t> ready;
-    int target_blocks = 0;
+    int target_blocks = 0;           // 0: each entry gets its single-call block count
     StepShape shape[2][kLevels];
-    int* d_blk[2][kLevels] = {};
-    int total[2][kLevels] = {};
-    std::vector<int> nblk[2][kLevels], blk0[2][kLevels];
-    std::map<std::vector<int>, PairDesc*> slots;   // descriptor arrays by the pairs' image parities
-    PairDesc* h_stage = nullptr;                   // pinned upload buffer
+    BatchPlan plan[kMaxHyp];         // plan[k]: k + 1 entries per handle
+    PairDesc* h_stage = nullptr;     // pinned upload buffer (kMaxHyp entries per handle)
     bool use_graph = false;
-    bool split_solve = true;        // k_solve + unfused pass A (else the fused prologue in every block)
+    bool split_solve = true;         // k_solve + unfused pass A (else the fused prologue in every block)
     struct G { std::vector<int> key; hipGraphExec_t exec; };
     std::vector<G> graphs;
     hipError_t last_hip = hipSuccess;
@@ -1177,106 +1284,166 @@ struct odom_batch_s {
 
 namespace {
 
+// One batched tracking call.
+struct CallSpec {
+    TrackParams P{};
+    int hyp = 1;                     // refinements per handle before the judge
+    int judge = ODOM_JUDGE_NONE;
+    bool policy_start = false;       // k_seed_policy sets each entry's start
+    const float* pred = nullptr;     // device, 6 floats per handle
+    float t_scale = 1.0f, r_scale = 1.0f;
+};
+
 void batch_drop_graphs(odom_batch_s* b)
 {
     for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
     b->graphs.clear();
 }
 
-void batch_drop_slots(odom_batch_s* b)
+void plan_free(BatchPlan& pl)
 {
-    for (auto& kv : b->slots) (void)hipFree(kv.second);
-    b->slots.clear();
+    for (auto& kv : pl.slots) (void)hipFree(kv.second);
+    pl.slots.clear();
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < kLevels; ++l) {
+            if (pl.d_blk[k][l]) (void)hipFree(pl.d_blk[k][l]);
+            pl.d_blk[k][l] = nullptr;
+            pl.total[k][l] = 0;
+            pl.nblk[k][l].clear();
+            pl.blk0[k][l].clear();
+        }
+    pl.built = false;
 }
 
-// Split each pass and level's tiles over the pairs and upload the tables.
-int batch_plan(odom_batch_s* b)
+// Split each pass and level's tiles over the entries of plan `hyp` and
+// upload the tables.
+int plan_build(odom_batch_s* b, int hyp)
 {
-    batch_drop_graphs(b);
-    batch_drop_slots(b);
-    const int n = (int)b->h.size();
+    BatchPlan& pl = b->plan[hyp - 1];
+    plan_free(pl);
+    const int n = (int)b->h.size(), ne = n * hyp;
     for (int k = 0; k < 2; ++k)
         for (int l = 0; l < kLevels; ++l) {
             const int tile = b->shape[k][l].threads * b->shape[k][l].ppt;
-            std::vector<int> tiles(n);
+            std::vector<int> tiles(ne);
             long long T = 0;
-            for (int i = 0; i < n; ++i) {
-                tiles[i] = cdiv(b->h[i]->n[l], tile);
-                T += tiles[i];
+            for (int e = 0; e < ne; ++e) {
+                tiles[e] = cdiv(b->h[e / hyp]->n[l], tile);
+                T += tiles[e];
             }
-            auto& nb = b->nblk[k][l];
-            auto& b0 = b->blk0[k][l];
-            nb.assign(n, 1);
-            b0.assign(n, 0);
+            auto& nb = pl.nblk[k][l];
+            auto& b0 = pl.blk0[k][l];
+            nb.assign(ne, 1);
+            b0.assign(ne, 0);
             int tot = 0;
-            for (int i = 0; i < n; ++i) {
-                const double share = (double)tiles[i] * b->target_blocks / (double)std::max(T, 1LL);
-                nb[i] = std::max(1, std::min({(int)(share + 0.5), kMaxBlk, tiles[i]}));
-                b0[i] = tot;
-                tot += nb[i];
+            for (int e = 0; e < ne; ++e) {
+                if (b->target_blocks > 0) {
+                    const double share = (double)tiles[e] * b->target_blocks / (double)std::max(T, 1LL);
+                    nb[e] = std::max(1, std::min({(int)(share + 0.5), kMaxBlk, tiles[e]}));
+                } else {
+                    nb[e] = std::max(1, std::min(kMaxBlk, tiles[e]));
+                }
+                b0[e] = tot;
+                tot += nb[e];
             }
-            b->total[k][l] = tot;
+            pl.total[k][l] = tot;
             std::vector<int> table(tot);
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j < nb[i]; ++j) table[b0[i] + j] = i;
-            if (b->d_blk[k][l]) (void)hipFree(b->d_blk[k][l]);
-            b->d_blk[k][l] = nullptr;
-            hipError_t e = hipMalloc((void**)&b->d_blk[k][l], sizeof(int) * tot);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(b->d_blk[k][l], table.data(), sizeof(int) * tot, hipMemcpyHostToDevice, b->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-            if (e != hipSuccess) {
-                b->last_hip = e;
+            for (int e = 0; e < ne; ++e)
+                for (int j = 0; j < nb[e]; ++j) table[b0[e] + j] = e;
+            hipError_t err = hipMalloc((void**)&pl.d_blk[k][l], sizeof(int) * tot);
+            if (err == hipSuccess)
+                err = hipMemcpyAsync(pl.d_blk[k][l], table.data(), sizeof(int) * tot, hipMemcpyHostToDevice,
+                                     b->stream);
+            if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
+            if (err != hipSuccess) {
+                b->last_hip = err;
                 return ODOM_ERR_HIP;
             }
         }
+    pl.built = true;
     return ODOM_OK;
 }
 
-// The descriptor array for the pairs' current image parities (built once per
-// parity pattern: a batch tracked with so3 alternates between two).
-int batch_slot(odom_batch_s* b, PairDesc** out)
+// After a change of shapes or block target: graphs and plans are rebuilt
+// (the one-refinement plan now, the two-refinement plan on first use).
+int batch_plan(odom_batch_s* b)
 {
+    batch_drop_graphs(b);
+    for (auto& pl : b->plan) plan_free(pl);
+    return plan_build(b, 1);
+}
+
+// The descriptor array of plan `hyp` for the pairs' current image parities
+// (built once per parity pattern: a batch tracked with so3 alternates
+// between two).  Entry hyp * i + j is handle i's state j (0: its own, 1: the
+// second refinement's).
+int batch_slot(odom_batch_s* b, int hyp, PairDesc** out)
+{
+    BatchPlan& pl = b->plan[hyp - 1];
+    if (!pl.built)
+        if (int rc = plan_build(b, hyp)) return rc;
     const int n = (int)b->h.size();
     std::vector<int> key(n);
     for (int i = 0; i < n; ++i) key[i] = b->h[i]->parity;
-    auto it = b->slots.find(key);
-    if (it != b->slots.end()) {
+    auto it = pl.slots.find(key);
+    if (it != pl.slots.end()) {
         *out = it->second;
         return ODOM_OK;
     }
-    for (int i = 0; i < n; ++i) {
-        PairDesc D = b->h[i]->h_pair[key[i]];
+    for (int e = 0; e < n * hyp; ++e) {
+        const odom_ctx* c = b->h[e / hyp];
+        PairDesc D = (e % hyp == 0) ? c->h_pair[key[e / hyp]] : c->h_pair_alt[key[e / hyp]];
         for (int l = 0; l < kLevels; ++l) {
-            D.nblk_a[l] = b->nblk[0][l][i];
-            D.blk0_a[l] = b->blk0[0][l][i];
-            D.nblk_b[l] = b->nblk[1][l][i];
-            D.blk0_b[l] = b->blk0[1][l][i];
+            D.nblk_a[l] = pl.nblk[0][l][e];
+            D.blk0_a[l] = pl.blk0[0][l][e];
+            D.nblk_b[l] = pl.nblk[1][l][e];
+            D.blk0_b[l] = pl.blk0[1][l][e];
         }
-        b->h_stage[i] = D;
+        b->h_stage[e] = D;
     }
     PairDesc* d = nullptr;
-    hipError_t e = hipMalloc((void**)&d, sizeof(PairDesc) * n);
+    hipError_t e = hipMalloc((void**)&d, sizeof(PairDesc) * n * hyp);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d, b->h_stage, sizeof(PairDesc) * n, hipMemcpyHostToDevice, b->stream);
+        e = hipMemcpyAsync(d, b->h_stage, sizeof(PairDesc) * n * hyp, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         if (d) (void)hipFree(d);
         b->last_hip = e;
         return ODOM_ERR_HIP;
     }
-    b->slots[key] = d;
+    pl.slots[key] = d;
     *out = d;
     return ODOM_OK;
 }
 
-// The launches of one batched getIncrementalTransformation (the same
-// sequence as enqueue_track, over every pair at once).
-void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEvent_t* ev)
+StepLaunch plan_launch(odom_batch_s* b, int hyp, PairDesc* dp, int kernel, int level)
 {
+    const BatchPlan& pl = b->plan[hyp - 1];
+    StepLaunch L;
+    L.pairs = dp;
+    L.npairs = (int)b->h.size() * hyp;
+    L.blk_pair = pl.d_blk[kernel][level];
+    L.total_blocks = pl.total[kernel][level];
+    L.shape = b->shape[kernel][level];
+    return L;
+}
+
+// The launches of one batched getIncrementalTransformation (the sequence of
+// enqueue_track over every pair at once), with the start and judge of a
+// policy call.  dp[k]: the descriptor array of plan k + 1 (dp[1] only for
+// two refinements).  A coarse judge runs when the level-2 iterations are
+// done: the last of them is finished for both refinements (k_solve), the
+// judge keeps one state per handle, and the finer levels continue with one
+// entry per handle (their first pass A reads the state the solve stored).
+void batch_launches(odom_batch_s* b, const CallSpec& cs, PairDesc* const* dp, hipEvent_t* ev)
+{
+    const TrackParams& P = cs.P;
     const int n = (int)b->h.size();
-    if (P.so3) launch_so3(dp, n, &P, b->stream);  // begin rides on the last SO3 launch
-    else launch_track_begin(dp, n, P, b->stream);
+    int hyp = cs.hyp;
+    PairDesc* cur = dp[hyp - 1];
+    if (cs.policy_start) launch_seed_policy(cur, n * hyp, hyp, cs.pred, cs.t_scale, cs.r_scale, b->stream);
+    if (P.so3) launch_so3(cur, n * hyp, &P, b->stream);  // begin rides on the last SO3 launch
+    else launch_track_begin(cur, n * hyp, P, b->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
@@ -1284,34 +1451,36 @@ void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEven
     for (size_t k = 0; k < sched.size(); ++k) {
         const int level = sched[k] / 64, iter = sched[k] % 64;
         const int prev = k > 0 ? sched[k - 1] / 64 : -1;
-        StepLaunch La;
-        La.pairs = dp;
-        La.npairs = n;
-        La.blk_pair = b->d_blk[0][level];
-        La.total_blocks = b->total[0][level];
-        La.shape = b->shape[0][level];
+        bool state_ready = false;   // the state entering iteration k is in its slot already
+        if (hyp == 2 && cs.judge == ODOM_JUDGE_COARSE && prev == 2 && level < 2) {
+            launch_solve(cur, n * 2, level, iter, (int)k, prev, P, b->shape[0][level].threads, b->stream);
+            launch_judge(cur, n, P, (int)(k & 1), 2, b->stream);
+            hyp = 1;
+            cur = dp[0];
+            state_ready = true;
+        }
+        const StepLaunch La = plan_launch(b, hyp, cur, 0, level);
         if (ev) (void)hipEventRecord(ev[e++], b->stream);
-        if (b->split_solve) {
-            launch_solve(dp, n, level, iter, (int)k, prev, P, b->stream);
+        if (state_ready) {
+            launch_step_a(La, level, iter, (int)k, prev, P, 1, nullptr, b->stream);
+        } else if (b->split_solve) {
+            launch_solve(cur, n * hyp, level, iter, (int)k, prev, P, La.shape.threads, b->stream);
             launch_step_a(La, level, iter, (int)k, prev, P, 1, nullptr, b->stream);
         } else {
             launch_step_a(La, level, iter, (int)k, prev, P, 0, nullptr, b->stream);
         }
         if (ev) (void)hipEventRecord(ev[e++], b->stream);
         if (P.rgb) {
-            StepLaunch Lb;
-            Lb.pairs = dp;
-            Lb.npairs = n;
-            Lb.blk_pair = b->d_blk[1][level];
-            Lb.total_blocks = b->total[1][level];
-            Lb.shape = b->shape[1][level];
+            const StepLaunch Lb = plan_launch(b, hyp, cur, 1, level);
             if (ev) (void)hipEventRecord(ev[e++], b->stream);
             launch_step_b(Lb, level, iter, (int)k, P, 0, b->stream);
             if (ev) (void)hipEventRecord(ev[e++], b->stream);
         }
     }
     const int K = (int)sched.size();
-    launch_track_end(dp, n, P, K, K > 0 ? sched[K - 1] / 64 : 0, b->stream);
+    launch_track_end(cur, n * hyp, P, K, K > 0 ? sched[K - 1] / 64 : 0, b->stream);
+    // a full judge, or a coarse one whose schedule has no finer level
+    if (hyp == 2) launch_judge(cur, n, P, -1, 0, b->stream);
 }
 
 #define BCHK(b, x)                                                                                       \
@@ -1325,51 +1494,236 @@ void batch_launches(odom_batch_s* b, const TrackParams& P, PairDesc* dp, hipEven
     } while (0)
 
 // Everything before the batch's own launches: checks, the per-pair
-// derivative pass on each handle's stream, and the cross-stream waits.
-int batch_prepare(odom_batch_s* b, const TrackParams& P, PairDesc** dp)
+// derivative pass on each handle's stream, the cross-stream waits, and the
+// descriptor arrays the call uses.
+int batch_prepare(odom_batch_s* b, const CallSpec& cs, PairDesc** dp)
 {
     for (auto* c : b->h)
-        if (!inputs_ready(c, P)) return ODOM_ERR_STATE;
+        if (!inputs_ready(c, cs.P)) return ODOM_ERR_STATE;
     for (size_t i = 0; i < b->h.size(); ++i) {
         odom_ctx* c = b->h[i];
-        if (P.rgb) {
+        if (cs.P.rgb) {
             PrepLevels a{};
             prep_levels(c, a);
             launch_rgb_prep(a, c->stream);
             if (int rc = check_launch(c)) return rc;
         }
-        BCHK(b, hipEventRecord(b->ready[i], c->stream));
-        BCHK(b, hipStreamWaitEvent(b->stream, b->ready[i], 0));
+        if (b->own_stream) {
+            BCHK(b, hipEventRecord(b->ready[i], c->stream));
+            BCHK(b, hipStreamWaitEvent(b->stream, b->ready[i], 0));
+        }
     }
-    return batch_slot(b, dp);
+    dp[0] = dp[1] = nullptr;
+    if (int rc = batch_slot(b, 1, &dp[0])) return rc;
+    if (cs.hyp == 2)
+        if (int rc = batch_slot(b, 2, &dp[1])) return rc;
+    return ODOM_OK;
 }
 
 // Handles continue after the batch: their streams wait for it, and the image
 // swap of RGBDOdometryef.cpp:585-591 flips each pair's parity.
 int batch_finish(odom_batch_s* b, const TrackParams& P)
 {
-    BCHK(b, hipEventRecord(b->done, b->stream));
+    if (b->own_stream) BCHK(b, hipEventRecord(b->done, b->stream));
     for (auto* c : b->h) {
-        BCHK(b, hipStreamWaitEvent(c->stream, b->done, 0));
+        if (b->own_stream) BCHK(b, hipStreamWaitEvent(c->stream, b->done, 0));
         if (P.so3) c->parity ^= 1;
         c->have_state = false;
     }
     return ODOM_OK;
 }
 
-std::vector<int> graph_key(const TrackParams& P, const std::vector<int>& parities)
+template <class T>
+int bits(T v)
 {
-    std::vector<int> k = {P.icp, P.rgb, P.rgbOnly, P.so3, P.iters[0], P.iters[1], P.iters[2]};
+    static_assert(sizeof(T) == 4, "32-bit field");
     int w;
-    memcpy(&w, &P.icpWeight, 4);
-    k.push_back(w);
+    memcpy(&w, &v, 4);
+    return w;
+}
+
+std::vector<int> graph_key(const CallSpec& cs, const std::vector<int>& parities)
+{
+    const TrackParams& P = cs.P;
+    std::vector<int> k = {P.icp, P.rgb, P.rgbOnly, P.so3, P.iters[0], P.iters[1], P.iters[2], bits(P.icpWeight),
+                          P.seeded, bits(P.convT), bits(P.convR), bits(P.judgeTol), cs.hyp, cs.judge,
+                          cs.policy_start ? 1 : 0, bits(cs.t_scale), bits(cs.r_scale)};
+    uint64_t pp = (uint64_t)(uintptr_t)cs.pred;
+    k.push_back((int)(pp & 0xffffffffu));
+    k.push_back((int)(pp >> 32));
     k.insert(k.end(), parities.begin(), parities.end());
     return k;
+}
+
+// One batched call: prepared, then launched directly or replayed from the
+// hipGraph captured on its first use.
+int batch_call(odom_batch_s* b, const CallSpec& cs)
+{
+    PairDesc* dp[2];
+    int rc = batch_prepare(b, cs, dp);
+    if (rc) return rc;
+    if (!b->use_graph) {
+        batch_launches(b, cs, dp, nullptr);
+        hipError_t e = hipPeekAtLastError();
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            b->last_hip = e;
+            return ODOM_ERR_HIP;
+        }
+    } else {
+        std::vector<int> par;
+        for (auto* c : b->h) par.push_back(c->parity);
+        const std::vector<int> key = graph_key(cs, par);
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : b->graphs)
+            if (g.key == key) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+            batch_launches(b, cs, dp, nullptr);
+            BCHK(b, hipStreamEndCapture(b->stream, &graph));
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            BCHK(b, e);
+            b->graphs.push_back({key, exec});
+        }
+        BCHK(b, hipGraphLaunch(exec, b->stream));
+    }
+    return batch_finish(b, cs.P);
+}
+
+// The TrackParams and call shape of a policy call.
+int policy_spec(odom_ctx* c0, const float* pred, float t_scale, float r_scale, const odom_policy* pol, int rgbOnly,
+                float icpWeight, CallSpec& cs)
+{
+    if (!pol) return ODOM_ERR_ARG;
+    if (pol->start != ODOM_START_PREVIOUS && pol->start != ODOM_START_VELOCITY) return ODOM_ERR_ARG;
+    if (pol->judge < ODOM_JUDGE_NONE || pol->judge > ODOM_JUDGE_FULL) return ODOM_ERR_ARG;
+    if (pred && !(t_scale > 0.0f && r_scale > 0.0f)) return ODOM_ERR_ARG;
+    if (!(pol->conv_t >= 0.0f) || !(pol->conv_r >= 0.0f) || !(pol->judge_tol >= 0.0f) || pol->judge_tol >= 1.0f)
+        return ODOM_ERR_ARG;
+    // the SO3 pre-alignment seeds the reference's start only
+    if (pol->so3 && (pred || pol->start != ODOM_START_PREVIOUS)) return ODOM_ERR_ARG;
+    cs.P = make_params(c0, rgbOnly, icpWeight, 1, 0, pol->so3);
+    if (!cs.P.icp && !cs.P.rgb) return ODOM_ERR_ARG;
+    for (int l = 0; l < kLevels; ++l) {
+        if (pol->iters[l] < 0 || pol->iters[l] > 63) return ODOM_ERR_ARG;
+        cs.P.iters[l] = pol->iters[l];
+    }
+    cs.P.convT = pol->conv_t;
+    cs.P.convR = pol->conv_t > 0.0f ? pol->conv_r : 0.0f;
+    cs.P.judgeTol = pol->judge_tol;
+    cs.policy_start = pred != nullptr || pol->start == ODOM_START_VELOCITY;
+    cs.P.seeded = cs.policy_start ? 3 : 0;
+    cs.pred = pred;
+    cs.t_scale = pred ? t_scale : 1.0f;
+    cs.r_scale = pred ? r_scale : 1.0f;
+    cs.judge = pred ? pol->judge : ODOM_JUDGE_NONE;
+    cs.hyp = (pred && pol->judge != ODOM_JUDGE_NONE) ? 2 : 1;
+    return ODOM_OK;
+}
+
+int batch_new(const odom_handle* handles, int n, bool own_stream, odom_batch_s** out);
+
+// The handle's own batch of one for its policy calls: the handle's stream,
+// shapes and graph setting, the single-call block counts and the fused
+// prologue, so its launches are those of odom_track_async.
+int self_batch(odom_ctx* c, odom_batch_s** out)
+{
+    if (!c->self_batch) {
+        odom_batch_s* b = nullptr;
+        if (int rc = batch_new(&c, 1, false, &b)) return rc;
+        c->self_batch = b;
+    }
+    *out = c->self_batch;
+    return ODOM_OK;
+}
+
+}  // namespace
+
+namespace {
+
+void drop_self_batch(odom_ctx* c)
+{
+    if (c->self_batch) {
+        odom_batch_destroy(c->self_batch);
+        c->self_batch = nullptr;
+    }
+}
+
+int batch_new(const odom_handle* handles, int n, bool own_stream, odom_batch_s** out)
+{
+    odom_batch_s* b = new odom_batch_s();
+    b->h.assign(handles, handles + n);
+    b->own_stream = own_stream;
+    if (own_stream) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+                cus = prop.multiProcessorCount;
+        }
+        b->target_blocks = 4 * cus;
+        for (int k = 0; k < 2; ++k)
+            for (int l = 0; l < kLevels; ++l) b->shape[k][l] = batch_shape(k, l);
+    } else {
+        odom_ctx* c = handles[0];
+        b->target_blocks = 0;
+        b->split_solve = false;
+        b->use_graph = c->use_graph;
+        b->stream = c->stream;
+        for (int l = 0; l < kLevels; ++l) {
+            b->shape[0][l] = c->shape_a[l];
+            b->shape[1][l] = c->shape_b[l];
+        }
+    }
+    b->ready.resize(n, nullptr);
+    bool ok = hipHostMalloc((void**)&b->h_stage, sizeof(PairDesc) * n * kMaxHyp, 0) == hipSuccess;
+    if (ok && own_stream)
+        ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && own_stream && i < n; ++i)
+        ok = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok || batch_plan(b) != ODOM_OK) {
+        odom_batch_destroy(b);
+        return ODOM_ERR_HIP;
+    }
+    *out = b;
+    return ODOM_OK;
 }
 
 }  // namespace
 
 extern "C" {
+
+int odom_default_policy(odom_policy* p)
+{
+    if (!p) return ODOM_ERR_ARG;
+    p->iters[0] = 10;
+    p->iters[1] = 5;
+    p->iters[2] = 4;
+    p->conv_t = 0.0f;
+    p->conv_r = 0.0f;
+    p->start = ODOM_START_PREVIOUS;
+    p->so3 = 1;
+    p->judge = ODOM_JUDGE_FULL;
+    p->judge_tol = 0.01f;
+    return ODOM_OK;
+}
+
+int odom_track_policy(odom_handle c, const float* pred, float t_scale, float r_scale, const odom_policy* pol,
+                      int rgbOnly, float icpWeight)
+{
+    if (!c) return ODOM_ERR_ARG;
+    CallSpec cs;
+    int rc = policy_spec(c, pred, t_scale, r_scale, pol, rgbOnly, icpWeight, cs);
+    if (rc) return rc;
+    if (!inputs_ready(c, cs.P)) return ODOM_ERR_STATE;
+    odom_batch_s* b = nullptr;
+    if ((rc = self_batch(c, &b))) return rc;
+    if ((rc = batch_call(b, cs))) return rc;
+    return check_launch(c);
+}
 
 int odom_batch_create(const odom_handle* handles, int n, odom_batch* out)
 {
@@ -1383,29 +1737,7 @@ int odom_batch_create(const odom_handle* handles, int n, odom_batch* out)
         for (int j = 0; j < i; ++j)
             if (handles[j] == handles[i]) return ODOM_ERR_ARG;
     }
-    odom_batch_s* b = new odom_batch_s();
-    b->h.assign(handles, handles + n);
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
-    }
-    b->target_blocks = 4 * cus;
-    for (int k = 0; k < 2; ++k)
-        for (int l = 0; l < kLevels; ++l) b->shape[k][l] = StepShape{256, 2};
-    b->ready.resize(n, nullptr);
-    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess &&
-              hipHostMalloc((void**)&b->h_stage, sizeof(PairDesc) * n, 0) == hipSuccess;
-    for (int i = 0; ok && i < n; ++i)
-        ok = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) == hipSuccess;
-    if (!ok || batch_plan(b) != ODOM_OK) {
-        odom_batch_destroy(b);
-        return ODOM_ERR_HIP;
-    }
-    *out = b;
-    return ODOM_OK;
+    return batch_new(handles, n, true, out);
 }
 
 int odom_batch_destroy(odom_batch b)
@@ -1413,22 +1745,19 @@ int odom_batch_destroy(odom_batch b)
     if (!b) return ODOM_ERR_ARG;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graphs(b);
-    batch_drop_slots(b);
-    for (int k = 0; k < 2; ++k)
-        for (int l = 0; l < kLevels; ++l)
-            if (b->d_blk[k][l]) (void)hipFree(b->d_blk[k][l]);
+    for (auto& pl : b->plan) plan_free(pl);
     for (auto e : b->ready)
         if (e) (void)hipEventDestroy(e);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->h_stage) (void)hipHostFree(b->h_stage);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return ODOM_OK;
 }
 
 int odom_batch_set_blocks(odom_batch b, int target_blocks)
 {
-    if (!b || target_blocks < 1 || target_blocks > 1 << 20) return ODOM_ERR_ARG;
+    if (!b || target_blocks < 0 || target_blocks > 1 << 20) return ODOM_ERR_ARG;
     BCHK(b, hipStreamSynchronize(b->stream));
     b->target_blocks = target_blocks;
     return batch_plan(b);
@@ -1446,7 +1775,7 @@ int odom_batch_set_step_shape(odom_batch b, int kernel, int level, int threads, 
 int odom_batch_get_blocks(odom_batch b, int kernel, int level, int* total_blocks)
 {
     if (!b || !total_blocks || kernel < 0 || kernel > 1 || level < 0 || level >= kLevels) return ODOM_ERR_ARG;
-    *total_blocks = b->total[kernel][level];
+    *total_blocks = b->plan[0].total[kernel][level];
     return ODOM_OK;
 }
 
@@ -1468,42 +1797,56 @@ int odom_batch_set_graph(odom_batch b, int enable)
     return ODOM_OK;
 }
 
+int odom_batch_get_stream(odom_batch b, void** stream)
+{
+    if (!b || !stream) return ODOM_ERR_ARG;
+    *stream = (void*)b->stream;
+    return ODOM_OK;
+}
+
 int odom_track_batch(odom_batch b, int rgbOnly, float icpWeight, int pyramid, int fastOdom, int so3)
 {
     if (!b) return ODOM_ERR_ARG;
-    const TrackParams P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
-    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;  // RGBDOdometryef.cpp:558-561
-    PairDesc* dp = nullptr;
-    int rc = batch_prepare(b, P, &dp);
+    CallSpec cs;
+    cs.P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!cs.P.icp && !cs.P.rgb) return ODOM_ERR_ARG;  // RGBDOdometryef.cpp:558-561
+    return batch_call(b, cs);
+}
+
+int odom_track_batch_policy(odom_batch b, const float* pred, float t_scale, float r_scale, const odom_policy* pol,
+                            int rgbOnly, float icpWeight)
+{
+    if (!b) return ODOM_ERR_ARG;
+    CallSpec cs;
+    const int rc = policy_spec(b->h[0], pred, t_scale, r_scale, pol, rgbOnly, icpWeight, cs);
     if (rc) return rc;
-    if (!b->use_graph) {
-        batch_launches(b, P, dp, nullptr);
-        hipError_t e = hipPeekAtLastError();
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            b->last_hip = e;
-            return ODOM_ERR_HIP;
-        }
-    } else {
-        std::vector<int> par;
-        for (auto* c : b->h) par.push_back(c->parity);
-        const std::vector<int> key = graph_key(P, par);
-        hipGraphExec_t exec = nullptr;
-        for (auto& g : b->graphs)
-            if (g.key == key) exec = g.exec;
-        if (!exec) {
-            hipGraph_t graph = nullptr;
-            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-            batch_launches(b, P, dp, nullptr);
-            BCHK(b, hipStreamEndCapture(b->stream, &graph));
-            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            BCHK(b, e);
-            b->graphs.push_back({key, exec});
-        }
-        BCHK(b, hipGraphLaunch(exec, b->stream));
+    return batch_call(b, cs);
+}
+
+int odom_batch_warmstart_input(odom_batch b, void* x)
+{
+    if (!b || !x) return ODOM_ERR_ARG;
+    for (size_t i = 0; i < b->h.size(); ++i) {
+        odom_ctx* c = b->h[i];
+        if (c->w[2] != 2 * 80 || c->h[2] != 2 * 60) return ODOM_ERR_ARG;  // 640x480 frames
+        if (!(c->icp_cur && c->rgb_last && c->rgb_next)) return ODOM_ERR_STATE;
     }
-    return batch_finish(b, P);
+    for (size_t i = 0; i < b->h.size(); ++i) {
+        odom_ctx* c = b->h[i];
+        if (b->own_stream) {
+            BCHK(b, hipEventRecord(b->ready[i], c->stream));
+            BCHK(b, hipStreamWaitEvent(b->stream, b->ready[i], 0));
+        }
+        cnn_launch_prepare(c->lastDepth[2], c->lastImage[2], c->depth[2], next_img(c, 2), 1, (size_t)c->n[2],
+                           (char*)x + i * (size_t)(80 * 60 * 4 * 2), b->stream);
+    }
+    hipError_t e = hipPeekAtLastError();
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b->last_hip = e;
+        return ODOM_ERR_HIP;
+    }
+    return ODOM_OK;
 }
 
 // Times every supported shape of the two step kernels per level on the
@@ -1539,22 +1882,17 @@ int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
                     b->shape[kernel][level] = StepShape{t, q};
                     if ((rc = batch_plan(b))) break;
                     PairDesc* dp = nullptr;
-                    if ((rc = batch_slot(b, &dp))) break;
-                    StepLaunch Ls;
-                    Ls.pairs = dp;
-                    Ls.npairs = (int)b->h.size();
-                    Ls.blk_pair = b->d_blk[kernel][level];
-                    Ls.total_blocks = b->total[kernel][level];
-                    Ls.shape = b->shape[kernel][level];
+                    if ((rc = batch_slot(b, 1, &dp))) break;
+                    const StepLaunch Ls = plan_launch(b, 1, dp, kernel, level);
                     const int reps = 10;
                     for (int r = 0; r < reps + 2; ++r) {
-                        if (r == 2) BCHK(b, hipEventRecord(e0, b->stream));
+                        if (r == 2) (void)hipEventRecord(e0, b->stream);
                         if (kernel == 0) launch_step_a(Ls, level, 0, 0, -1, P, 1, nullptr, b->stream);
                         else launch_step_b(Ls, level, 0, 0, P, 1, b->stream);
                     }
-                    BCHK(b, hipEventRecord(e1, b->stream));
-                    BCHK(b, hipEventSynchronize(e1));
-                    hipError_t e = hipPeekAtLastError();
+                    (void)hipEventRecord(e1, b->stream);
+                    hipError_t e = hipEventSynchronize(e1);
+                    if (e == hipSuccess) e = hipPeekAtLastError();
                     if (e != hipSuccess) {
                         (void)hipGetLastError();
                         b->last_hip = e;
@@ -1562,7 +1900,7 @@ int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
                         break;
                     }
                     float ms = 0.0f;
-                    BCHK(b, hipEventElapsedTime(&ms, e0, e1));
+                    (void)hipEventElapsedTime(&ms, e0, e1);
                     if (ms < best) {
                         best = ms;
                         best_sh = StepShape{t, q};
@@ -1596,18 +1934,20 @@ int odom_batch_profile_track(odom_batch b, int rgbOnly, float icpWeight, int pyr
                              float us_a[3], float us_b[3], int n_a[3], int n_b[3], float* us_total)
 {
     if (!b || !us_a || !us_b || !n_a || !n_b || !us_total) return ODOM_ERR_ARG;
-    const TrackParams P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
-    if (!P.icp && !P.rgb) return ODOM_ERR_ARG;
-    PairDesc* dp = nullptr;
-    int rc = batch_prepare(b, P, &dp);
+    CallSpec cs;
+    cs.P = make_params(b->h[0], rgbOnly, icpWeight, pyramid, fastOdom, so3);
+    if (!cs.P.icp && !cs.P.rgb) return ODOM_ERR_ARG;
+    PairDesc* dp[2];
+    int rc = batch_prepare(b, cs, dp);
     if (rc) return rc;
+    const TrackParams& P = cs.P;
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
     std::vector<hipEvent_t> ev(sched.size() * (P.rgb ? 4 : 2) + 2, nullptr);
     for (auto& e : ev) BCHK(b, hipEventCreate(&e));
     BCHK(b, hipEventRecord(ev[ev.size() - 2], b->stream));
-    batch_launches(b, P, dp, ev.data());
+    batch_launches(b, cs, dp, ev.data());
     BCHK(b, hipEventRecord(ev[ev.size() - 1], b->stream));
     BCHK(b, hipEventSynchronize(ev[ev.size() - 1]));
     for (int l = 0; l < kLevels; ++l) { us_a[l] = 0; us_b[l] = 0; n_a[l] = 0; n_b[l] = 0; }

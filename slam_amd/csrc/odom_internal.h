@@ -110,10 +110,18 @@ void cnn_launch_prepare(const float* depth_a, const uint8_t* image_a, const uint
 void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_level, const TrackParams& P, int mode,
                    uint8_t* dbg_mask, hipStream_t s);
 void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s);
-// Finish iteration k-1 and store the state of iteration k, one workgroup per
-// pair (followed by launch_step_a with mode 1).
+// Finish iteration k-1 and store the state of iteration k, one workgroup of
+// `threads` (256 or 512: those of the pass A that follows) per pair
+// (followed by launch_step_a with mode 1).
 void launch_solve(const PairDesc* pairs, int npairs, int level, int iter, int k, int prev_level, const TrackParams& P,
-                  hipStream_t s);
+                  int threads, hipStream_t s);
+// Policy calls: the start of every entry (hyp entries per handle; the
+// prediction or the constant-velocity extrapolation), and the judge of two
+// refinements per handle (slot >= 0: after the coarse levels, from state
+// slot `slot`; slot < 0: after the call).
+void launch_seed_policy(const PairDesc* pairs, int nentries, int hyp, const float* pred, float t_scale, float r_scale,
+                        hipStream_t s);
+void launch_judge(const PairDesc* pairs, int nhandles, const TrackParams& P, int slot, int level, hipStream_t s);
 // finishes the last iteration (K-1, at last_level), revert, result
 void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s);
 // FP64 totals of a parity pass: which 0 = pass A (ICP sums, count, sigma), 1 = pass B

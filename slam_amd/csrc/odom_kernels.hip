@@ -1078,8 +1078,10 @@ __device__ void store_system_stats(IterState& S, const TrackParams& P, const dou
     }
 }
 
+// level: the level of the iteration being finished; with the convergence
+// policy (P.convT > 0) an update below both thresholds stops that level.
 __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const double* icp_sums,
-                                     const double* rgb_sums, const float* Rprev, const float* tprev)
+                                     const double* rgb_sums, const float* Rprev, const float* tprev, int level)
 {
     const bool lead = (threadIdx.x & 63) == 0;
     double A[36], b[6], x[6];
@@ -1144,6 +1146,11 @@ __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const d
         }
         S.degenerate = (zeros == 6);
         S.iters_done++;
+        if (P.convT > 0.0f) {
+            const double nt = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+            const double nr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+            if (nt < (double)P.convT && nr < (double)P.convR) S.break_level = level;
+        }
     }
 }
 
@@ -1365,7 +1372,7 @@ __device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, 
     if (run) {
         partials_sum<NT>(PL, na, nb, W.tot, W.tot1, W.red, W.fin2);
         if (wave == 0) {
-            solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t);
+            solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t, prev_level);
         } else if (wave == 1) {
             store_system_stats(W.S, P, W.tot1, W.tot1 + kSE3, lane);
             if (lane == 0 && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
@@ -1541,18 +1548,22 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
 // pair: finish iteration k-1 and store the state entering iteration k, which
 // the unfused pass A (FUSED = false) then reads.  Used by batches, where it
 // keeps the solve's registers out of the pixel kernel.
-__global__ __launch_bounds__(256) void k_solve(const PairDesc* __restrict__ pairs, int level, int iter, int k,
-                                               int prev_level, TrackParams P)
+// NT: the thread count of the pass A that follows, whose fused prologue
+// would sum the partials in this same order (so the state is the same bits
+// either way).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_solve(const PairDesc* __restrict__ pairs, int level, int iter, int k,
+                                              int prev_level, TrackParams P)
 {
-    __shared__ Prolog<256> W;
+    __shared__ Prolog<NT> W;
     const PairDesc& D = pairs[blockIdx.x];
     TrackState* st = D.st;
     PrevPose pp;
     load_prev(st, pp);
-    finish_previous<256>(D, P, k, prev_level, pp, W);
+    finish_previous<NT>(D, P, k, prev_level, pp, W);
     if (threadIdx.x == 0) enter_iteration(W.S, D.lv[level], level, iter);
     __syncthreads();
-    copy_out<256>(&st->it[k & 1], &W.S);
+    copy_out<NT>(&st->it[k & 1], &W.S);
 }
 
 // ======================================================== step kernel B
@@ -1763,9 +1774,12 @@ void launch_step_a(const StepLaunch& L, int level, int iter, int k, int prev_lev
 }
 
 void launch_solve(const PairDesc* pairs, int npairs, int level, int iter, int k, int prev_level, const TrackParams& P,
-                  hipStream_t s)
+                  int threads, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_solve, dim3(npairs), dim3(256), 0, s, pairs, level, iter, k, prev_level, P);
+    if (threads == 512)
+        hipLaunchKernelGGL((k_solve<512>), dim3(npairs), dim3(512), 0, s, pairs, level, iter, k, prev_level, P);
+    else
+        hipLaunchKernelGGL((k_solve<256>), dim3(npairs), dim3(256), 0, s, pairs, level, iter, k, prev_level, P);
 }
 
 void launch_step_b(const StepLaunch& L, int level, int iter, int k, const TrackParams& P, int mode, hipStream_t s)
@@ -1943,7 +1957,9 @@ __device__ void track_begin_pair(const PairDesc& D, const TrackParams& P, const 
         st->Rprev_inv[a] = Rprev_inv[a];
     }
     for (int a = 0; a < 3; ++a) st->tprev[a] = tprev[a];
-    if (P.seeded) {
+    st->alt_iters = 0;
+    st->judge_level = -1;
+    if (P.seeded == 1 || P.seeded == 2) {
         st->seed_used = keep;
         st->seed_calls = st->seed_calls + 1;
         st->seed_kept = st->seed_kept + keep;
@@ -2028,6 +2044,7 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     g->lastRGBCount = S.lastRGBCount;
     g->iters_done = S.iters_done;
     g->degenerate = S.degenerate;
+    for (int a = 0; a < 16; ++a) g->lastRt[a] = reverted ? ((a % 5 == 0) ? 1.0 : 0.0) : S.resultRt[a];
     if (plog && plog_cap > 0) {
         GLOBAL_AS float* dst = gp(plog) + (size_t)(frame % plog_cap) * 12;
         for (int a = 0; a < 12; ++a) dst[a] = pose[a];
@@ -2038,6 +2055,168 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
 void launch_track_end(const PairDesc* pairs, int npairs, const TrackParams& P, int K, int last_level, hipStream_t s)
 {
     hipLaunchKernelGGL(k_track_end, dim3(npairs), dim3(256), 0, s, pairs, P, K, last_level);
+}
+
+// ============================================== starts and the judge
+// The start of a policy call, one workgroup per entry (hyp entries per
+// handle, entry hyp*i + h).  Entry h = 0 is the handle's own state: it starts
+// from the prediction (pred, 6 floats per handle: t / t_scale, rotation
+// vector / r_scale) when there is one, else from the constant-velocity
+// extrapolation.  Entry h = 1 is the second refinement of a judged call: it
+// takes the handle's prior (pose and the statistics a call carries) and
+// starts from the constant-velocity extrapolation.  The extrapolation repeats
+// the last call's increment (TrackState::lastRt).  The seed is stored in the
+// entry's seedRt for k_track_begin (seeded 3).
+__device__ void seed_from_motion(const float* q, float t_scale, float r_scale, double* seed)
+{
+    double w[3], R[9], t[3];
+    for (int r = 0; r < 3; ++r) {
+        t[r] = (double)q[r] * (double)t_scale;
+        w[r] = (double)q[3 + r] * (double)r_scale;
+    }
+    rodrigues(w, R);
+    // M = [R | t] (previous -> current camera); seed = M^-1 = [R^T | -R^T t]
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) seed[r * 4 + c] = R[c * 3 + r];
+        seed[r * 4 + 3] = -(R[0 * 3 + r] * t[0] + R[1 * 3 + r] * t[1] + R[2 * 3 + r] * t[2]);
+    }
+    seed[12] = seed[13] = seed[14] = 0.0;
+    seed[15] = 1.0;
+}
+
+__global__ __launch_bounds__(64) void k_seed_policy(const PairDesc* __restrict__ pairs, int hyp,
+                                                    const float* __restrict__ pred, float t_scale, float r_scale)
+{
+    if (threadIdx.x != 0) return;
+    const int e = blockIdx.x, i = e / hyp, h = e - i * hyp;
+    GLOBAL_AS TrackState* st = gp(pairs[e].st);
+    const GLOBAL_AS TrackState* own = gp(pairs[e - h].st);   // the handle's state
+    double seed[16];
+    if (h == 0 && pred) {
+        seed_from_motion(pred + 6 * i, t_scale, r_scale, seed);
+    } else {
+        for (int a = 0; a < 16; ++a) seed[a] = own->lastRt[a];
+    }
+    if (h > 0) {
+        // the handle's prior and carried statistics (RGBDOdometryef.cpp:
+        // 278-282: the call starts from the pose and keeps lastA/lastb and
+        // the errors of the previous call until it overwrites them)
+        float pose[12], errs[4];
+        double A[36], b[6], lr[16];
+        for (int a = 0; a < 12; ++a) pose[a] = own->pose_out[a];
+        for (int a = 0; a < 36; ++a) A[a] = own->lastA[a];
+        for (int a = 0; a < 6; ++a) b[a] = own->lastb[a];
+        for (int a = 0; a < 16; ++a) lr[a] = own->lastRt[a];
+        errs[0] = own->lastICPError;
+        errs[1] = own->lastICPCount;
+        errs[2] = own->lastRGBError;
+        errs[3] = own->lastRGBCount;
+        for (int a = 0; a < 12; ++a) st->pose_out[a] = pose[a];
+        for (int a = 0; a < 36; ++a) st->lastA[a] = A[a];
+        for (int a = 0; a < 6; ++a) st->lastb[a] = b[a];
+        for (int a = 0; a < 16; ++a) st->lastRt[a] = lr[a];
+        st->lastICPError = errs[0];
+        st->lastICPCount = errs[1];
+        st->lastRGBError = errs[2];
+        st->lastRGBCount = errs[3];
+        st->pose_log = nullptr;
+        st->pose_log_cap = 0;
+    }
+    for (int a = 0; a < 16; ++a) st->seedRt[a] = seed[a];
+    if (h == 0 && pred && hyp == 1) {
+        // an unjudged prediction is always kept
+        st->seed_used = 1;
+        st->seed_calls = st->seed_calls + 1;
+        st->seed_kept = st->seed_kept + 1;
+    }
+}
+
+void launch_seed_policy(const PairDesc* pairs, int nentries, int hyp, const float* pred, float t_scale, float r_scale,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seed_policy, dim3(nentries), dim3(64), 0, s, pairs, hyp, pred, t_scale, r_scale);
+}
+
+// True when refinement a (error, count) is worse than refinement b by more
+// than the relative margin tol.
+__device__ __forceinline__ bool refinement_worse(float err_a, float cnt_a, float err_b, float cnt_b, float tol)
+{
+    return (err_a > err_b * (1.0f + tol)) || (cnt_a < cnt_b * (1.0f - tol));
+}
+
+// The judge of a call with two refinements, one workgroup per handle
+// (entries 2i: the prediction's, 2i+1: the constant-velocity start's).  The
+// prediction's refinement is kept unless its ICP residual or inlier count
+// (the photometric ones for RGB-only calls) is worse than the other's.
+// mid (slot >= 0): after the coarse levels, from the statistics of the last
+// finished iteration in state slot `slot`; on a fall-back the other state of
+// that slot replaces the handle's, and the finer levels continue from it.
+// final (slot < 0): after the call; on a fall-back the other call's results
+// replace the handle's (pose, system, statistics, the logged pose).
+__global__ __launch_bounds__(256) void k_judge(const PairDesc* __restrict__ pairs, TrackParams P, int slot,
+                                               int level)
+{
+    __shared__ int s_fall;
+    GLOBAL_AS TrackState* own = gp(pairs[2 * blockIdx.x].st);
+    GLOBAL_AS TrackState* alt = gp(pairs[2 * blockIdx.x + 1].st);
+    if (threadIdx.x == 0) {
+        float ea, ca, eb, cb;
+        if (slot >= 0) {
+            const GLOBAL_AS IterState& A = own->it[slot];
+            const GLOBAL_AS IterState& B = alt->it[slot];
+            ea = P.icp ? A.lastICPError : A.lastRGBError;
+            ca = P.icp ? A.lastICPCount : A.lastRGBCount;
+            eb = P.icp ? B.lastICPError : B.lastRGBError;
+            cb = P.icp ? B.lastICPCount : B.lastRGBCount;
+        } else {
+            ea = P.icp ? own->lastICPError : own->lastRGBError;
+            ca = P.icp ? own->lastICPCount : own->lastRGBCount;
+            eb = P.icp ? alt->lastICPError : alt->lastRGBError;
+            cb = P.icp ? alt->lastICPCount : alt->lastRGBCount;
+        }
+        const int fall = refinement_worse(ea, ca, eb, cb, P.judgeTol) ? 1 : 0;
+        s_fall = fall;
+        const int iters_own = (slot >= 0) ? own->it[slot].iters_done : own->iters_done;
+        const int iters_alt = (slot >= 0) ? alt->it[slot].iters_done : alt->iters_done;
+        own->alt_iters = fall ? iters_own : iters_alt;
+        own->seed_used = fall ? 0 : 1;
+        own->seed_calls = own->seed_calls + 1;
+        own->seed_kept = own->seed_kept + (fall ? 0 : 1);
+        own->judge_level = level;
+    }
+    __syncthreads();
+    if (!s_fall) return;
+    if (slot >= 0) {
+        const GLOBAL_AS unsigned int* src = (const GLOBAL_AS unsigned int*)&alt->it[slot];
+        GLOBAL_AS unsigned int* dst = (GLOBAL_AS unsigned int*)&own->it[slot];
+        for (int i = threadIdx.x; i < (int)(sizeof(IterState) / 4); i += 256) dst[i] = src[i];
+        return;
+    }
+    if (threadIdx.x != 0) return;
+    float pose[12];
+    for (int a = 0; a < 12; ++a) pose[a] = alt->pose_out[a];
+    for (int a = 0; a < 12; ++a) own->pose_out[a] = pose[a];
+    for (int a = 0; a < 36; ++a) own->lastA[a] = alt->lastA[a];
+    for (int a = 0; a < 6; ++a) own->lastb[a] = alt->lastb[a];
+    for (int a = 0; a < 16; ++a) own->lastRt[a] = alt->lastRt[a];
+    own->lastICPError = alt->lastICPError;
+    own->lastICPCount = alt->lastICPCount;
+    own->lastRGBError = alt->lastRGBError;
+    own->lastRGBCount = alt->lastRGBCount;
+    own->iters_done = alt->iters_done;
+    own->degenerate = alt->degenerate;
+    own->reverted = alt->reverted;
+    float* const plog = own->pose_log;
+    const int cap = own->pose_log_cap;
+    if (plog && cap > 0) {
+        GLOBAL_AS float* d = gp(plog) + (size_t)((own->frame_count - 1) % cap) * 12;
+        for (int a = 0; a < 12; ++a) d[a] = pose[a];
+    }
+}
+
+void launch_judge(const PairDesc* pairs, int nhandles, const TrackParams& P, int slot, int level, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_judge, dim3(nhandles), dim3(256), 0, s, pairs, P, slot, level);
 }
 
 // ================================================================= SO3

@@ -4,9 +4,11 @@ model update, and the next prediction (elastic_fusion_file.cpp:335-420,
 551-602, with closeLoops off).
 
 ``FrameLoop`` drives the native loop of ``include/slam_loop.h``
-(``efloop_process``, C++ over the odometry, splat and surfel entry points);
-with the learned warm start it runs the same steps from Python, adding the
-network call.  Every stage runs on one HIP stream (the odometry's); the host
+(``efloop_process``, C++ over the odometry, splat and surfel entry points).
+``BatchFrameLoop`` runs n sequences in lockstep (``efloop_batch``), their
+tracking calls batched and, with the learned start, one network forward for
+all of them; a FrameLoop with any other tracking policy is a batch of one.
+Every stage of a sequence runs on one HIP stream (its odometry's); the host
 reads back only what the reference's loop reads: the tracked pose and the
 model's surfel count.  The prediction that the reference makes right after
 tracking (elastic_fusion_file.cpp:399) feeds only the loop-closure search, so
@@ -20,18 +22,25 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import splat, surfels
-from .odometry import (FMT_DEPTH_U16, FMT_RGB8, MEM_DEVICE, RGBDOdometry, Stats, _check, _p, frame_view,
+from .odometry import (FMT_DEPTH_U16, FMT_RGB8, JUDGE_FULL, START_PREVIOUS, Policy, Stats, _check, _p, frame_view,
                        lib as odom_lib)
 
 # Every symbol include/slam_loop.h declares.
 EXPORTS = ["efloop_default_params", "efloop_create", "efloop_destroy", "efloop_process", "efloop_get_handles",
-           "efloop_get_state"]
+           "efloop_get_state", "efloop_batch_create", "efloop_batch_destroy", "efloop_batch_set_net",
+           "efloop_batch_process", "efloop_batch_get_loop", "efloop_batch_get_counters"]
 
 
 class _Params(ctypes.Structure):
     _fields_ = [("depth_cutoff", ctypes.c_float), ("max_depth", ctypes.c_float), ("confidence", ctypes.c_float),
                 ("time_delta", ctypes.c_int), ("icp_weight", ctypes.c_float), ("so3", ctypes.c_int),
                 ("max_surfels", ctypes.c_int), ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("frames_tracked", ctypes.c_int), ("gn_iterations", ctypes.c_longlong),
+                ("alt_iterations", ctypes.c_longlong), ("predictions", ctypes.c_int),
+                ("predictions_kept", ctypes.c_int)]
 
 
 def lib() -> ctypes.CDLL:
@@ -48,6 +57,14 @@ def lib() -> ctypes.CDLL:
     L.efloop_process.argtypes = [V, P(odom_frame_view_type()), P(odom_frame_view_type()), V]
     L.efloop_get_handles.argtypes = [V, P(V), P(V), P(V)]
     L.efloop_get_state.argtypes = [V, P(ctypes.c_int), P(ctypes.c_float), P(ctypes.c_float)]
+    from .odometry import Policy
+    L.efloop_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4 + \
+        [P(_Params), P(Policy), P(V)]
+    L.efloop_batch_destroy.argtypes = [V]
+    L.efloop_batch_set_net.argtypes = [V, V, ctypes.c_size_t, ctypes.c_float, ctypes.c_float]
+    L.efloop_batch_process.argtypes = [V, V, V, V]
+    L.efloop_batch_get_loop.argtypes = [V, ctypes.c_int, P(V)]
+    L.efloop_batch_get_counters.argtypes = [V, ctypes.c_int, P(Counters)]
     L._loop_typed = True
     return L
 
@@ -68,6 +85,7 @@ class _OdoView:
         _check("odom_get_stats", odom_lib().odom_get_stats(self._h, ctypes.byref(s)))
         return {"lastICPError": s.last_icp_error, "lastICPCount": s.last_icp_count,
                 "lastRGBError": s.last_rgb_error, "lastRGBCount": s.last_rgb_count,
+                "gnIterations": s.gn_iterations, "altIterations": s.alt_iterations,
                 "seedCalls": s.seed_calls, "seedKept": s.seed_kept}
 
 
@@ -92,7 +110,8 @@ class _ModelView:
 
 @dataclass
 class LoopParams:
-    """The reference configs' values (src/configs/ef_file.cfg)."""
+    """The reference configs' values (src/configs/ef_file.cfg), and the
+    tracking policy of the batched loop."""
     depth_cutoff: float = 3.0        # depthCutoff: bilateral and metric depth window (m)
     max_depth: float = 20.0          # maxDepthProcessed
     confidence: float = 10.0
@@ -100,35 +119,55 @@ class LoopParams:
     icp_weight: float = 10.0
     so3: bool = True
     max_surfels: int = 4_000_000
-    use_graph: bool = True           # replay the tracking call from a hipGraph (native loop)
-    autotune: bool = True            # step-kernel launch shapes timed on the first tracked frame
-    learned: bool = False            # seed each call from the warm-start network (no SO3 pass then)
-    learned_iters: tuple = (10, 5, 4)
+    use_graph: bool = True           # replay the tracking call from a hipGraph
+    autotune: bool = False           # step-kernel launch shapes timed on the first tracked frame (the
+                                     # library's table otherwise; timing makes runs differ in their last bits)
+    learned: bool = False            # start each call from the warm-start network's prediction, judged
+                                     # against the constant-velocity start (no SO3 pass then)
+    iters: tuple = (10, 5, 4)        # the policy's schedule (levels 0, 1, 2)
+    conv_t: float = 0.0              # the policy's convergence test (m, rad); 0 = the fixed schedule
+    conv_r: float = 0.0
+    start: int = START_PREVIOUS      # the start without a prediction
+    judge: int = JUDGE_FULL
+    judge_tol: float = 0.01
+
+    def policy(self) -> Policy:
+        so3 = bool(self.so3) and not self.learned and self.start == START_PREVIOUS
+        return Policy.make(self.iters, self.conv_t, self.conv_r, self.start, so3, self.judge, self.judge_tol)
+
+    def is_reference(self) -> bool:
+        """The reference's getIncrementalTransformation: previous-pose start,
+        fixed 10/5/4 schedule, no prediction."""
+        return (not self.learned and self.start == START_PREVIOUS and tuple(self.iters) == (10, 5, 4)
+                and self.conv_t == 0.0)
 
 
-def _rodrigues_norm(R: np.ndarray) -> float:
-    c = (np.trace(R) - 1.0) / 2.0
-    return float(np.arccos(np.clip(c, -1.0, 1.0)))
+def _native_params(p: LoopParams) -> _Params:
+    P = _Params()
+    _check("efloop_default_params", lib().efloop_default_params(ctypes.byref(P)))
+    P.depth_cutoff, P.max_depth, P.confidence = p.depth_cutoff, p.max_depth, p.confidence
+    P.time_delta, P.icp_weight, P.so3 = p.time_delta, p.icp_weight, int(p.so3)
+    P.max_surfels = p.max_surfels
+    P.use_graph = int(p.use_graph)
+    P.autotune = int(p.autotune)
+    return P
 
 
 class FrameLoop:
-    """Processes frames (uint16 depth in mm, RGB8), host or device."""
+    """Processes frames (uint16 depth in mm, RGB8), host or device.  The
+    reference's loop runs natively per frame (efloop_process); any other
+    tracking policy (the learned start, the constant-velocity start, the
+    convergence test) runs as a batched loop of one sequence."""
 
     def __new__(cls, width, height, cx, cy, fx, fy, params: LoopParams | None = None):
-        if params is not None and params.learned:
-            return super().__new__(_PythonFrameLoop)
+        if params is not None and not params.is_reference():
+            return super().__new__(_BatchOfOne)
         return super().__new__(cls)
 
     def __init__(self, width: int, height: int, cx: float, cy: float, fx: float, fy: float,
                  params: LoopParams | None = None):
         self.p = params or LoopParams()
-        P = _Params()
-        _check("efloop_default_params", lib().efloop_default_params(ctypes.byref(P)))
-        P.depth_cutoff, P.max_depth, P.confidence = self.p.depth_cutoff, self.p.max_depth, self.p.confidence
-        P.time_delta, P.icp_weight, P.so3 = self.p.time_delta, self.p.icp_weight, int(self.p.so3)
-        P.max_surfels = self.p.max_surfels
-        P.use_graph = int(self.p.use_graph)
-        P.autotune = int(self.p.autotune)
+        P = _native_params(self.p)
         h = ctypes.c_void_p()
         _check("efloop_create", lib().efloop_create(width, height, cx, cy, fx, fy, ctypes.byref(P), ctypes.byref(h)))
         self._h = h
@@ -160,94 +199,95 @@ class FrameLoop:
         return T
 
 
-class _PythonFrameLoop(FrameLoop):
-    """The same loop driven from Python, with the learned warm start seeding
-    each tracking call (no SO3 pass then)."""
+class BatchFrameLoop:
+    """``n`` sequences of one size in lockstep (include/slam_loop.h
+    efloop_batch): per frame one batched tracking call for all of them under
+    the params' policy, with the learned start's network run once for the
+    whole batch.  Each sequence's results equal those of the same sequence
+    run alone."""
+
+    def __init__(self, n: int, width: int, height: int, cx: float, cy: float, fx: float, fy: float,
+                 params: LoopParams | None = None, weights: str | None = None):
+        self.p = params or LoopParams()
+        self.n = n
+        P = _native_params(self.p)
+        pol = self.p.policy()
+        h = ctypes.c_void_p()
+        _check("efloop_batch_create", lib().efloop_batch_create(n, width, height, cx, cy, fx, fy, ctypes.byref(P),
+                                                                ctypes.byref(pol), ctypes.byref(h)))
+        self._h = h
+        if self.p.learned:
+            from . import warmstart as ws
+            with np.load(weights or ws.WEIGHTS, allow_pickle=False) as z:
+                params = np.ascontiguousarray(z["params"], dtype=np.float32)
+                if float(z["t_scale"]) != ws.T_SCALE or float(z["r_scale"]) != ws.R_SCALE:
+                    raise ValueError("weights were trained with other output scales")
+            _check("efloop_batch_set_net", lib().efloop_batch_set_net(h, params.ctypes.data, params.size, ws.T_SCALE,
+                                                                      ws.R_SCALE))
+        self.odos, self.models = [], []
+        for i in range(n):
+            loop = ctypes.c_void_p()
+            _check("efloop_batch_get_loop", lib().efloop_batch_get_loop(h, i, ctypes.byref(loop)))
+            o, m = ctypes.c_void_p(), ctypes.c_void_p()
+            _check("efloop_get_handles", lib().efloop_get_handles(loop, ctypes.byref(o), None, ctypes.byref(m)))
+            self.odos.append(_OdoView(o))
+            self.models.append(_ModelView(m))
+        self.poses: list[list[np.ndarray]] = [[] for _ in range(n)]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().efloop_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+    def process(self, rgbs, depths) -> list[np.ndarray]:
+        """One frame of every sequence; returns the n tracked poses (4x4)."""
+        FV = odom_frame_view_type()
+        rv, dv, keep = (FV * self.n)(), (FV * self.n)(), []
+        for i in range(self.n):
+            r, kr = frame_view(rgbs[i], FMT_RGB8)
+            d, kd = frame_view(depths[i], FMT_DEPTH_U16)
+            rv[i], dv[i] = r, d
+            keep += [kr, kd]
+        out = np.zeros((self.n, 16), np.float32)
+        _check("efloop_batch_process", lib().efloop_batch_process(self._h, rv, dv, _p(out)))
+        poses = [out[i].reshape(4, 4).copy() for i in range(self.n)]
+        for i, T in enumerate(poses):
+            self.poses[i].append(T)
+        return poses
+
+    def counters(self, i: int) -> dict:
+        c = Counters()
+        _check("efloop_batch_get_counters", lib().efloop_batch_get_counters(self._h, i, ctypes.byref(c)))
+        return {"frames_tracked": c.frames_tracked, "gn_iterations": c.gn_iterations,
+                "alt_iterations": c.alt_iterations, "predictions": c.predictions,
+                "predictions_kept": c.predictions_kept}
+
+
+class _BatchOfOne(FrameLoop):
+    """FrameLoop under a non-reference policy: a batched loop of one."""
 
     def __init__(self, width: int, height: int, cx: float, cy: float, fx: float, fy: float,
                  params: LoopParams | None = None):
-        import torch
         self.p = params or LoopParams()
-        self.w, self.h = width, height
-        self.odo = RGBDOdometry(width, height, cx, cy, fx, fy)
-        stream = self.odo.stream
-        self.pred = splat.IndexMap(width, height, cx, cy, fx, fy)
-        self.pred.set_stream(stream)
-        self.model = surfels.SurfelModel(width, height, cx, cy, fx, fy, self.p.max_surfels)
-        self.model.set_stream(stream)
-        self.filtered = torch.empty((height, width), dtype=torch.uint16, device="cuda")
-        self.pose = np.eye(4, dtype=np.float32)
-        self.odo.set_pose(self.pose[:3, 3], self.pose[:3, :3])
-        self.net = None
+        self._b = BatchFrameLoop(1, width, height, cx, cy, fx, fy, self.p)
         self._h = None
-        if self.p.learned:
-            from . import warmstart as ws
-            self.ws = ws
-            self.net = ws.WarmStartNet(max_batch=1)
-            self.net.load()
-            self.net.set_stream(stream)
-            self.x = torch.zeros((1, 60, 80, 4), dtype=torch.bfloat16, device="cuda")
-            self.pred6 = torch.zeros((1, 6), dtype=torch.float32, device="cuda")
-        self.tick = 0
-        self.last_icp_error = 0.0
-        self.poses: list[np.ndarray] = []
+        self.odo = self._b.odos[0]
+        self.model = self._b.models[0]
+        self.poses = self._b.poses[0]
 
-    def _bilateral(self, depth):
-        v, keep = frame_view(depth, FMT_DEPTH_U16)
-        _check("odom_bilateral_depth", odom_lib().odom_bilateral_depth(
-            self.odo.handle, ctypes.byref(v), self.p.depth_cutoff, ctypes.c_void_p(self.filtered.data_ptr()),
-            MEM_DEVICE))
-        return self.filtered
-
-    def _predict(self):
-        """predict() (elastic_fusion_file.cpp:17-44): the active splat at the
-        current pose, then the fill-in from the filtered depth and colour."""
-        self.pred.combinedPredict(self.pose, self.model.device_model(), self.p.max_depth, self.p.confidence,
-                                  self.tick, self.tick, self.p.time_delta)
+    def close(self) -> None:
+        if getattr(self, "_b", None) is not None:
+            self._b.close()
+            self._b = None
 
     def process(self, rgb, depth) -> np.ndarray:
-        """One frame; returns the tracked camera-to-model pose (4x4)."""
-        self.tick += 1
-        p = self.p
-        filtered = self._bilateral(depth)
-        if self.tick == 1:
-            self.model.initialise(rgb, depth, filtered, p.depth_cutoff, self.tick, p.max_depth)
-            self.odo.initFirstRGB(rgb)
-        else:
-            last = self.pose.copy()
-            tracking_ok = self.last_icp_error < 1e-4      # the previous call's error, as the loop reads it
-            self.odo.initICPModel(self.pred.device(splat.FILL_VERTEX), self.pred.device(splat.FILL_NORMAL),
-                                  p.max_depth, self.pose)
-            self.odo.initRGBModel(self.pred.device(splat.FILL_IMAGE))
-            self.odo.initICP(filtered, p.max_depth)
-            self.odo.initRGB(rgb)
-            if self.net is not None:
-                # the learned warm start: the network's motion seeds the call,
-                # which keeps it only if its association is at least as good
-                self.odo.initWarmStartInput(self.x)
-                self.ws._lib().cnn_forward(self.net.handle, self.x.data_ptr(), 1, self.pred6.data_ptr())
-                self.odo.track_seeded(self.pred6, p.learned_iters, False, p.icp_weight, True, self.ws.T_SCALE,
-                                      self.ws.R_SCALE)
-                t, R = self.odo.get_pose()
-            else:
-                t, R = self.odo.getIncrementalTransformation(self.pose[:3, 3], self.pose[:3, :3], False,
-                                                             p.icp_weight, True, False, p.so3)
-            self.pose = np.eye(4, dtype=np.float32)
-            self.pose[:3, :3] = np.asarray(R, np.float32).reshape(3, 3)
-            self.pose[:3, 3] = np.asarray(t, np.float32).reshape(3)
-            self.last_icp_error = self.odo.stats()["lastICPError"]
-            if self.tick == 2 and p.autotune:
-                self.odo.autotune(False, p.icp_weight)  # launch shapes from the first tracked frame
-            # weight by velocity (elastic_fusion_file.cpp:377-398)
-            diff = np.linalg.inv(self.pose.astype(np.float64)) @ last.astype(np.float64)
-            weighting = max(float(np.linalg.norm(diff[:3, 3])), _rodrigues_norm(diff[:3, :3]))
-            weighting = max(1.0 - min(weighting, 0.01) / 0.01, 0.5)
-            if tracking_ok:
-                self.model.predictIndices(self.pose, self.tick, p.max_depth, p.time_delta)
-                self.model.fuse(self.pose, self.tick, rgb, depth, filtered, p.depth_cutoff, p.max_depth, weighting)
-                self.model.predictIndices(self.pose, self.tick, p.max_depth, p.time_delta)
-                self.model.clean(self.pose, self.tick, p.confidence, p.time_delta)
-        self._predict()
-        self.pred.fillIn(filtered, rgb)
-        self.poses.append(self.pose.copy())
-        return self.pose
+        return self._b.process([rgb], [depth])[0]
+
+    def counters(self) -> dict:
+        return self._b.counters(0)

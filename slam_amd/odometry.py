@@ -47,7 +47,27 @@ class Stats(ctypes.Structure):
                 ("last_so3_error", ctypes.c_float), ("last_so3_count", ctypes.c_float),
                 ("so3_iterations", ctypes.c_int), ("gn_iterations", ctypes.c_int),
                 ("degenerate", ctypes.c_int), ("reverted", ctypes.c_int), ("seed_used", ctypes.c_int),
-                ("seed_calls", ctypes.c_int), ("seed_kept", ctypes.c_int)]
+                ("seed_calls", ctypes.c_int), ("seed_kept", ctypes.c_int), ("alt_iterations", ctypes.c_int)]
+
+
+START_PREVIOUS, START_VELOCITY = 0, 1
+JUDGE_NONE, JUDGE_COARSE, JUDGE_FULL = 0, 1, 2
+
+
+class Policy(ctypes.Structure):
+    """odom_policy: schedule, convergence test, start and judge of a call."""
+    _fields_ = [("iters", ctypes.c_int * 3), ("conv_t", ctypes.c_float), ("conv_r", ctypes.c_float),
+                ("start", ctypes.c_int), ("so3", ctypes.c_int), ("judge", ctypes.c_int),
+                ("judge_tol", ctypes.c_float)]
+
+    @classmethod
+    def make(cls, iters=(10, 5, 4), conv_t=0.0, conv_r=0.0, start=START_PREVIOUS, so3=False, judge=JUDGE_FULL,
+             judge_tol=0.01) -> "Policy":
+        p = cls()
+        p.iters[:] = [int(v) for v in iters]
+        p.conv_t, p.conv_r = float(conv_t), float(conv_r)
+        p.start, p.so3, p.judge, p.judge_tol = int(start), int(so3), int(judge), float(judge_tol)
+        return p
 
 
 # Every symbol include/slam_odom.h declares.
@@ -63,7 +83,8 @@ EXPORTS = [
     "odom_batch_set_blocks", "odom_batch_get_blocks", "odom_batch_set_step_shape", "odom_batch_set_graph",
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
     "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
-    "odom_loop_check", "odom_icp_step_noalloc",
+    "odom_loop_check", "odom_icp_step_noalloc", "odom_default_policy", "odom_track_policy",
+    "odom_batch_get_stream", "odom_batch_warmstart_input", "odom_track_batch_policy",
 ]
 
 _V = ctypes.c_void_p
@@ -130,6 +151,12 @@ def lib() -> ctypes.CDLL:
                                            ctypes.c_int, _V, _V, _V, _V, _P(ctypes.c_float)]
     L.odom_icp_step_noalloc.argtypes = [_V, _V, _V, _V, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4 + \
         [_V, _V, _V, _V, ctypes.c_float, ctypes.c_float, _V, _V]
+    L.odom_default_policy.argtypes = [_P(Policy)]
+    L.odom_track_policy.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _P(Policy), ctypes.c_int, ctypes.c_float]
+    L.odom_batch_get_stream.argtypes = [_V, _P(_V)]
+    L.odom_batch_warmstart_input.argtypes = [_V, _V]
+    L.odom_track_batch_policy.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _P(Policy), ctypes.c_int,
+                                          ctypes.c_float]
     L._typed = True
     return L
 
@@ -316,6 +343,15 @@ class RGBDOdometry:
                                                             it.ctypes.data, int(rgbOnly), float(icpWeight),
                                                             int(select)))
 
+    def track_policy(self, pred=None, policy: Policy | None = None, rgbOnly=False, icpWeight=10.0, t_scale=0.05,
+                     r_scale=0.05) -> None:
+        """Asynchronous tracking call under a policy (odom_track_policy):
+        pred is a device tensor of 6 floats or None."""
+        pol = policy if policy is not None else Policy.make()
+        ptr = None if pred is None else pred.data_ptr()
+        _check("odom_track_policy", lib().odom_track_policy(self._h, ptr, float(t_scale), float(r_scale),
+                                                            ctypes.byref(pol), int(rgbOnly), float(icpWeight)))
+
     def track_async(self, rgbOnly=False, icpWeight=10.0, pyramid=True, fastOdom=False, so3=True) -> None:
         _check("odom_track_async", lib().odom_track_async(self._h, int(rgbOnly), float(icpWeight), int(pyramid),
                                                           int(fastOdom), int(so3)))
@@ -358,7 +394,7 @@ class RGBDOdometry:
                 "lastSO3Error": s.last_so3_error, "lastSO3Count": s.last_so3_count,
                 "so3Iterations": s.so3_iterations, "gnIterations": s.gn_iterations,
                 "degenerate": s.degenerate, "reverted": s.reverted, "seedUsed": s.seed_used,
-                "seedCalls": s.seed_calls, "seedKept": s.seed_kept}
+                "seedCalls": s.seed_calls, "seedKept": s.seed_kept, "altIterations": s.alt_iterations}
 
     @property
     def lastA(self) -> np.ndarray:
@@ -488,6 +524,26 @@ class Batch:
 
     def synchronize(self) -> None:
         _check("odom_batch_synchronize", lib().odom_batch_synchronize(self._b))
+
+    def track_policy(self, pred=None, policy: Policy | None = None, rgbOnly=False, icpWeight=10.0, t_scale=0.05,
+                     r_scale=0.05) -> None:
+        """odom_track_batch_policy: pred is a device tensor [n][6] or None."""
+        pol = policy if policy is not None else Policy.make()
+        ptr = None if pred is None else pred.data_ptr()
+        _check("odom_track_batch_policy", lib().odom_track_batch_policy(self._b, ptr, float(t_scale), float(r_scale),
+                                                                        ctypes.byref(pol), int(rgbOnly),
+                                                                        float(icpWeight)))
+
+    def warmstart_input(self, x) -> None:
+        """The network input of every pair into x (bf16 [n][60][80][4])."""
+        _check("odom_batch_warmstart_input", lib().odom_batch_warmstart_input(self._b, x.data_ptr()))
+
+    @property
+    def stream(self) -> int:
+        """The batch's HIP stream (as an integer pointer)."""
+        st = _V()
+        _check("odom_batch_get_stream", lib().odom_batch_get_stream(self._b, ctypes.byref(st)))
+        return st.value or 0
 
     def set_blocks(self, target_blocks: int) -> None:
         _check("odom_batch_set_blocks", lib().odom_batch_set_blocks(self._b, int(target_blocks)))

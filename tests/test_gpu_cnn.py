@@ -97,3 +97,22 @@ def test_training_reduces_loss():
     assert last["train_loss"] < 0.7 * first["train_loss"], (first, last)
     assert last["val_loss"] < first["val_loss"], (first, last)
     assert last["val_rot_err_deg"] < 0.8 * last["zero_guess_rot_deg"], last
+
+
+def test_forward_is_the_same_in_any_inference_batch():
+    """A sample's prediction does not depend on the batch it runs in (up to
+    64 samples): the frame loop batches sequences without changing any of
+    them.  Batch 8 against 8 forwards of batch 1, and batch 3, bit for bit."""
+    import torch
+    from slam_amd.warmstart import WarmStartNet
+    net = WarmStartNet(max_batch=64)
+    net.init_params(seed=9)
+    rng = np.random.default_rng(10)
+    x = torch.from_numpy(rng.uniform(0, 1, size=(8, 60, 80, 4)).astype(np.float32)).to(torch.bfloat16).cuda()
+    full = net.forward(x.contiguous()).cpu().numpy()
+    for i in range(8):
+        one = net.forward(x[i:i + 1].contiguous()).cpu().numpy()
+        assert np.array_equal(one[0], full[i]), i
+    three = net.forward(x[2:5].contiguous()).cpu().numpy()
+    assert np.array_equal(three, full[2:5])
+    net.close()
