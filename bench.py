@@ -1,21 +1,26 @@
 #!/usr/bin/env python
 """Frames/s of dense RGB-D odometry on one MI355X.
 
-Workload: a 640x480 frame-to-model sequence (seeded procedural room, Perlin
-textures, depth noise 1 mm * z^2, 2 % dropout) on a smooth spline of up to
-1 cm / 0.5 deg per frame.  Each step tracks one frame exactly as the
-reference's frame loop does (elastic_fusion_file.cpp:342-374): bilateral
-filter, initICPModel (model maps of the previous frame at the current
-estimate), initRGBModel, initICP, initRGB, then getIncrementalTransformation
-with ICP+RGB (icpWeight 10), 3 levels at 10/5/4 iterations and the SO(3)
-pre-alignment.  Inputs are resident in HBM before timing starts; every call
-is enqueued without host synchronisation and the pose chain stays on the
-device.
+Workload (the headline): a 640x480 frame-to-model sequence (seeded procedural
+room, Perlin textures, depth noise 1 mm * z^2, 2 % dropout) on a smooth
+spline of up to 1 cm / 0.5 deg per frame.  Each step tracks one frame exactly
+as the reference's frame loop does (elastic_fusion_file.cpp:342-374):
+bilateral filter, initICPModel (model maps of the previous frame at the
+current estimate), initRGBModel, initICP, initRGB, then
+getIncrementalTransformation with ICP+RGB (icpWeight 10), 3 levels at 10/5/4
+iterations and the SO(3) pre-alignment.  Inputs are resident in HBM before
+timing starts; every call is enqueued without host synchronisation and the
+pose chain stays on the device.  Launch shapes come from the library's fixed
+table, so the run does not depend on timing.
 
 Prints one JSON line: throughput, the roofline of the dominant step kernel
-(HIP events on the library's stream), the CPU restatement's frames/s on a
-bounded sample of the same sequence, and the pose error against the FP64
-oracle on that sample.
+(HIP events on the library's stream) against the 8 TB/s specification and a
+copy peak measured in the same run, the CPU restatement's frames/s (1 core
+and all cores, the 320x240 ICP-only and the 640x480 ICP+RGB configurations,
+median and p90 after warm-up), the pose error against the FP64 oracle, and
+the other configurations' legs: the learned start against the plain forms
+under the same iteration policy, the 64-pair batch, the splat prediction,
+the surfel fusion and the 8-sequence fast-motion frame loop.
 """
 from __future__ import annotations
 
@@ -43,16 +48,433 @@ BYTES_RGB = 14   # I_next 1, dIdx 2, dIdy 2, z_next 4, z_last 4, I_last 1
 # k_step_b then reads the gradients (the rest of the RGB bytes)
 BYTES_A = BYTES_ICP + 10
 BYTES_B = BYTES_RGB - 10
+# the iteration policy of the learned start and its plain control: a level
+# ends once an update is below 0.1 mm and 0.1 mrad
+CONV_T = 1e-4
+CONV_R = 1e-4
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+class Status:
+    """ORs the status codes of the C calls a timed loop makes; checked after
+    the timing, so a refused call cannot silently shorten the work."""
+
+    def __init__(self, what: str):
+        self.what = what
+        self.bad = 0
+        self.first = None
+
+    def __call__(self, rc: int) -> None:
+        if rc != 0 and self.first is None:
+            self.first = rc
+        self.bad |= rc != 0
+
+    def check(self) -> None:
+        if self.bad:
+            raise RuntimeError(f"{self.what}: a C call returned {self.first}")
+
+
+def pose12(p):
+    T = np.eye(4)
+    T[:3, :4] = np.asarray(p).reshape(3, 4)
+    return T
+
+
+def stream_copy_peak(gib: float = 1.0, reps: int = 10) -> dict:
+    """STREAM-style copy: a device-to-device copy of `gib` GiB, timed with HIP
+    events; bytes moved = 2 x size per copy."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2 * n * 4 / (ms * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"GBs": round(gbs, 1), "bytes_per_copy": 2 * n * 4, "ms_per_copy": round(ms, 4),
+            "kernel": "torch device copy (elementwise), 1 GiB -> 1 GiB"}
+
+
+class DeviceSequence:
+    """A rendered sequence resident on the device with prebuilt frame views,
+    so a timed loop is only C calls."""
+
+    def __init__(self, cam, poses, scene, noise_base, keep_host=0, what="sequence"):
+        import torch
+        from slam_amd.odometry import frame_view, FMT_DEPTH_U16, FMT_RGB8, FMT_RGBA8, FMT_RGBA32F
+        self.cam, self.poses = cam, poses
+        self.dev = {k: [] for k in ("depth", "rgb", "v4", "n4", "rgba")}
+        self.host = []
+        t0 = time.time()
+        for i, T in enumerate(poses):
+            f = scene.render(T, cam, noise_seed=noise_base + i)
+            self.dev["depth"].append(torch.from_numpy(f.depth).cuda())
+            self.dev["rgb"].append(torch.from_numpy(f.rgb).cuda())
+            self.dev["v4"].append(torch.from_numpy(f.vert4).cuda())
+            self.dev["n4"].append(torch.from_numpy(f.norm4).cuda())
+            self.dev["rgba"].append(torch.from_numpy(f.rgba).cuda())
+            if i < keep_host:
+                self.host.append(f)
+        torch.cuda.synchronize()
+        self.views = {k: [frame_view(t, fmt)[0] for t in self.dev[k]] for k, fmt in
+                      (("depth", FMT_DEPTH_U16), ("rgb", FMT_RGB8), ("v4", FMT_RGBA32F), ("n4", FMT_RGBA32F),
+                       ("rgba", FMT_RGBA8))}
+        log(f"{what}: {len(poses)} frames {cam.width}x{cam.height} resident in {time.time() - t0:.1f}s")
+
+    def feed(self, L, h, k, st: Status, icp_only=False) -> None:
+        """The init calls of frame k against frame k-1's model maps at the
+        handle's current estimate."""
+        by = ctypes.byref
+        v = self.views
+        st(L.odom_init_icp_model_current(h, by(v["v4"][k - 1]), by(v["n4"][k - 1]), CUTOFF))
+        if not icp_only:
+            st(L.odom_init_rgb_model(h, by(v["rgba"][k - 1])))
+        st(L.odom_init_icp_raw_depth(h, by(v["depth"][k]), MAXD, CUTOFF))
+        if not icp_only:
+            st(L.odom_init_rgb(h, by(v["rgb"][k])))
+
+
+class Forms:
+    """The tracking calls compared on one handle: the reference's, and the
+    iteration policy with the constant-velocity start or the learned start
+    (network input, forward pass and the judged call, all on the handle's
+    stream)."""
+
+    def __init__(self, L, odo, net=None):
+        from slam_amd.odometry import JUDGE_COARSE, JUDGE_FULL, START_VELOCITY, Policy
+        import torch
+        self.L, self.odo, self.h = L, odo, odo.handle
+        self.net = net
+        self.pol_ref_conv = Policy.make((10, 5, 4), CONV_T, CONV_R, so3=True)
+        self.pol_cv = Policy.make((10, 5, 4), CONV_T, CONV_R, start=START_VELOCITY, so3=False)
+        self.pol_learned = Policy.make((10, 5, 4), CONV_T, CONV_R, judge=JUDGE_COARSE)
+        self.pol_learned_full = Policy.make((10, 5, 4), CONV_T, CONV_R, judge=JUDGE_FULL)
+        self.x = torch.zeros((1, 60, 80, 4), dtype=torch.bfloat16, device="cuda")
+        self.pred = torch.zeros((1, 6), dtype=torch.float32, device="cuda")
+
+    def call(self, form: str, st: Status) -> None:
+        from slam_amd import warmstart as ws
+        L, h, by = self.L, self.h, ctypes.byref
+        if form == "reference":
+            st(L.odom_track_async(h, 0, ICP_WEIGHT, 1, 0, 1))
+        elif form == "reference_conv":
+            st(L.odom_track_policy(h, None, 1.0, 1.0, by(self.pol_ref_conv), 0, ICP_WEIGHT))
+        elif form == "plain_policy":
+            st(L.odom_track_policy(h, None, 1.0, 1.0, by(self.pol_cv), 0, ICP_WEIGHT))
+        elif form in ("learned", "learned_full"):
+            st(L.odom_init_warmstart_input(h, self.x.data_ptr()))
+            st(ws._lib().cnn_forward(self.net.handle, self.x.data_ptr(), 1, self.pred.data_ptr()))
+            pol = self.pol_learned if form == "learned" else self.pol_learned_full
+            st(L.odom_track_policy(h, self.pred.data_ptr(), ws.T_SCALE, ws.R_SCALE, by(pol), 0, ICP_WEIGHT))
+        else:
+            raise ValueError(form)
+
+
+def run_form(seq: DeviceSequence, forms: Forms, form: str, nsteps: int, warmup: int, trace: bool = False) -> dict:
+    """Frames 1..nsteps of the sequence with one form: warm-up frames, then
+    the timed frames enqueued without host synchronisation (poses logged on
+    the device).  With `trace`, the same frames again with the statistics
+    read after every call (iterations, whether the prediction was kept) and
+    the network's predictions; the runs are deterministic, so the trace
+    repeats the timed run."""
+    import torch
+    L, odo, h = forms.L, forms.odo, forms.h
+    T0 = seq.poses[0]
+
+    def reset():
+        odo.set_pose(T0[:3, 3], T0[:3, :3])
+        odo.initFirstRGB(seq.dev["rgb"][0])
+
+    st = Status(form)
+    reset()
+    for k in range(1, 1 + warmup):
+        seq.feed(L, h, k, st)
+        forms.call(form, st)
+    odo.synchronize()
+    log_t = torch.zeros((nsteps + 1, 12), dtype=torch.float32, device="cuda")
+    odo.set_pose_log(log_t.data_ptr(), nsteps + 1)
+    reset()
+    torch.cuda.synchronize()
+    odo.synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, 1 + nsteps):
+        seq.feed(L, h, k, st)
+        forms.call(form, st)
+    odo.synchronize()
+    el = time.perf_counter() - t0
+    st.check()
+    est = [pose12(p) for p in log_t.cpu().numpy()[:nsteps]]
+    odo.set_pose_log(0, 0)
+    r = {"frames_per_s": round(nsteps / el, 2), "ms_per_frame": round(1e3 * el / nsteps, 4), "poses": est}
+    if trace:
+        reset()
+        its, alts, kept, preds = [], [], [], []
+        for k in range(1, 1 + nsteps):
+            seq.feed(L, h, k, st)
+            forms.call(form, st)
+            s = odo.stats()
+            its.append(s["gnIterations"])
+            alts.append(s["altIterations"])
+            kept.append(s["seedUsed"])
+            if form.startswith("learned"):
+                preds.append(forms.pred.cpu().numpy()[0].copy())
+        st.check()
+        r.update({"iters_per_frame": round(float(np.mean(its)), 3),
+                  "alt_iters_per_frame": round(float(np.mean(alts)), 3), "_kept": kept, "_preds": preds})
+        if form.startswith("learned"):
+            r["fallback_rate"] = round(1.0 - float(np.mean(kept)), 4)
+    return r
+
+
+def error_stats(est, ref, key: str) -> dict:
+    from slam_amd import synth
+    e = [synth.pose_error(a, b) for a, b in zip(est, ref)]
+    return {f"{key}_mean_mm": round(1e3 * float(np.mean([q[0] for q in e])), 5),
+            f"{key}_max_mm": round(1e3 * float(np.max([q[0] for q in e])), 5),
+            f"{key}_mean_deg": round(float(np.mean([q[1] for q in e])), 6),
+            f"{key}_max_deg": round(float(np.max([q[1] for q in e])), 6)}
+
+
+def oracle_sequence(seq: DeviceSequence, n: int, fp64: bool, call):
+    """The oracle's tracker over frames 1..n of the sequence (host frames),
+    with the per-frame call(o, T, k) -> (t, R, info)."""
+    from oracle import oracle as orc
+    from slam_amd import synth
+    cam, host = seq.cam, seq.host
+    o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, fp64=fp64)
+    T = seq.poses[0].copy()
+    o.initFirstRGB(host[0].rgb)
+    out, times = [], []
+    for k in range(1, n + 1):
+        t_start = time.perf_counter()
+        o.initICPModel(host[k - 1].vert4, host[k - 1].norm4, CUTOFF, T.astype(np.float32))
+        o.initRGBModel(host[k - 1].rgba)
+        o.initICP(orc.bilateral(host[k].depth, MAXD), CUTOFF)
+        o.initRGB(host[k].rgb)
+        t, R, _ = call(o, T, k)
+        times.append(time.perf_counter() - t_start)
+        T = synth.make_pose(np.asarray(R, np.float64).reshape(3, 3), np.asarray(t, np.float64).reshape(3))
+        out.append(T)
+    return out, times
+
+
+def reference_call(o, T, k):
+    t, R = o.getIncrementalTransformation(T[:3, 3], T[:3, :3], False, ICP_WEIGHT, True, False, True)
+    return t, R, None
+
+
+def cpu_baseline(args, seq_c2: DeviceSequence) -> dict:
+    """The reference algorithm's CPU path: the FP32 C restatement
+    (oracle/odom_oracle.c, OpenMP over rows) on this host, in this process,
+    on the same inputs -- 640x480 ICP+RGB+SO3 (the headline's first frames)
+    and 320x240 ICP-only (a 1 cm / 0.5 deg sequence of the same kind of room)
+    -- on 1 thread and on all threads: 3 warm-up frames, then the median and
+    p90 of the per-frame time (bilateral filter, the init calls and
+    getIncrementalTransformation)."""
+    from oracle import oracle as orc
+    from slam_amd import synth
+    n = args.cpu_frames
+    warm = 3
+    hw = orc.hardware_concurrency()
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        nproc = os.cpu_count() or 1
+    all_threads = orc.get_threads()
+    omp_env = os.environ.get("OMP_NUM_THREADS")
+    log(f"cpu baseline: nproc {nproc}, hardware_concurrency {hw}, OMP_NUM_THREADS {omp_env}, "
+        f"all-core run on {all_threads} threads")
+
+    # C1: 320x240 ICP-only, icpWeight 100 (RGBDOdometryef.cpp:276), no SO3
+    cam1 = synth.CAM_320
+    scene1 = synth.Scene(args.seed + 100)
+    poses1 = synth.spline_trajectory(n + warm + 1, seed=args.seed + 100)
+    host1 = [scene1.render(T, cam1, noise_seed=7000 + i) for i, T in enumerate(poses1)]
+
+    def run_c1(threads):
+        orc.set_threads(threads)
+        o = orc.Odometry(cam1.width, cam1.height, cam1.cx, cam1.cy, cam1.fx, cam1.fy)
+        T = poses1[0].copy()
+        times = []
+        for k in range(1, n + warm + 1):
+            t_start = time.perf_counter()
+            o.initICPModel(host1[k - 1].vert4, host1[k - 1].norm4, CUTOFF, T.astype(np.float32))
+            o.initICP(orc.bilateral(host1[k].depth, MAXD), CUTOFF)
+            t, R = o.getIncrementalTransformation(T[:3, 3], T[:3, :3], False, 100.0, True, False, False)
+            times.append(time.perf_counter() - t_start)
+            T = synth.make_pose(np.asarray(R, np.float64), np.asarray(t, np.float64))
+        return times[warm:]
+
+    def run_c2(threads):
+        orc.set_threads(threads)
+        _, times = oracle_sequence(seq_c2, n + warm, False, reference_call)
+        return times[warm:]
+
+    def summary(times, threads):
+        med = float(np.median(times))
+        return {"threads": threads, "frames": len(times), "median_ms": round(1e3 * med, 3),
+                "p90_ms": round(1e3 * float(np.percentile(times, 90)), 3), "frames_per_s": round(1.0 / med, 3)}
+
+    out = {"nproc": nproc, "hardware_concurrency": hw, "OMP_NUM_THREADS": omp_env, "warmup_frames": warm}
+    for name, fn in (("c1_320x240_icp", run_c1), ("c2_640x480_icp_rgb_so3", run_c2)):
+        out[name] = {}
+        for label, threads in (("1_core", 1), ("all_cores", all_threads)):
+            t0 = time.time()
+            out[name][label] = summary(fn(threads), threads)
+            log(f"cpu baseline {name} {label}: {out[name][label]} ({time.time() - t0:.1f}s)")
+    orc.set_threads(all_threads)
+    return out
+
+
+def c1_gpu(args) -> dict:
+    """The 320x240 ICP-only configuration on the GPU: the same sequence form
+    as the headline (icpWeight 100, no SO3, 10/5/4), frames/s."""
+    from slam_amd import synth
+    from slam_amd.odometry import RGBDOdometry, lib
+    cam = synth.CAM_320
+    n = args.steps
+    poses = synth.spline_trajectory(n + 1, seed=args.seed + 100)
+    seq = DeviceSequence(cam, poses, synth.Scene(args.seed + 100), 7000, what="C1")
+    L = lib()
+    odo = RGBDOdometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    odo.set_graph(True)
+    h = odo.handle
+    st = Status("C1")
+
+    def run():
+        odo.set_pose(poses[0][:3, 3], poses[0][:3, :3])
+        for k in range(1, n + 1):
+            seq.feed(L, h, k, st, icp_only=True)
+            st(L.odom_track_async(h, 0, 100.0, 1, 0, 0))
+        odo.synchronize()
+
+    run()
+    t0 = time.perf_counter()
+    run()
+    el = time.perf_counter() - t0
+    st.check()
+    odo.close()
+    return {"workload": "320x240 ICP-only frame-to-model, icpWeight 100, no SO3, 10/5/4, 1 cm / 0.5 deg spline",
+            "frames_per_s": round(n / el, 2), "ms_per_frame": round(1e3 * el / n, 4), "frames": n}
+
+
+FORMS = ("reference", "reference_conv", "plain_policy", "learned", "learned_full")
+FORM_DOC = {
+    "reference": "previous-pose start, SO3 pre-alignment, fixed 10/5/4 (getIncrementalTransformation)",
+    "reference_conv": "previous-pose start, SO3, 10/5/4 with the convergence test",
+    "plain_policy": "constant-velocity start, no SO3, 10/5/4 with the convergence test (the learned form "
+                    "without the network)",
+    "learned": "network start (bf16 MFMA, batch 1) judged against the constant-velocity start after the "
+               "level-2 refinement of both, convergence test",
+    "learned_full": "as learned, both starts refined through every level before the judge",
+}
+
+
+def learned_vs_plain(args, seq_c2: DeviceSequence, L, odo, ref64) -> dict | None:
+    """The learned start against the plain forms, on the headline's sequence
+    and on a fast-motion sequence (bursts up to 8 cm / 6 deg per frame, the
+    warm-start network's training motions): frames/s, Gauss-Newton
+    iterations per frame (the kept refinement's, and the judged-away one's),
+    the fall-back rate, and the pose error against the FP64 oracle of the
+    reference schedule and of the same policy (the oracle fed the network's
+    predictions the device used)."""
+    from slam_amd import evaluate as ev, synth, warmstart as ws
+    if not os.path.exists(ws.WEIGHTS):
+        log("learned: no trained weights; skipped")
+        return None
+    net = ws.WarmStartNet(max_batch=1)
+    net.load()
+    net.set_stream(odo.stream)
+    forms = Forms(L, odo, net)
+    nsteps, warm = args.steps, args.warmup
+    res = {"forms": FORM_DOC, "convergence_test": {"dt_m": CONV_T, "dw_rad": CONV_R}, "judge_tol": 0.01}
+
+    def public(r):
+        return {k: v for k, v in r.items() if k not in ("poses", "_kept", "_preds")}
+
+    # ---- the headline's sequence (smooth, up to 1 cm / 0.5 deg per frame)
+    c2 = {f: run_form(seq_c2, forms, f, nsteps, warm, trace=True) for f in FORMS}
+    if ref64 is not None:
+        m = min(len(ref64), nsteps)
+        for f in FORMS:
+            c2[f].update(error_stats(c2[f]["poses"][:m], ref64[:m], "vs_fp64_reference"))
+            e = c2[f]
+            c2[f]["accuracy_gate_1mm_0.1deg"] = bool(e["vs_fp64_reference_mean_mm"] <= 1.0 and
+                                                      e["vs_fp64_reference_mean_deg"] <= 0.1)
+        # the same policy in FP64 (the oracle's restatement, the device's predictions)
+        mp = min(args.policy_oracle_frames, m)
+        calls = {
+            "reference_conv": lambda o, T, k: o.track_policy(T[:3, 3], T[:3, :3], conv=(CONV_T, CONV_R), so3=True),
+            "plain_policy": lambda o, T, k: o.track_policy(T[:3, 3], T[:3, :3], conv=(CONV_T, CONV_R),
+                                                           start="velocity"),
+            "learned": lambda o, T, k: o.track_policy(T[:3, 3], T[:3, :3], c2["learned"]["_preds"][k - 1],
+                                                      ws.T_SCALE, ws.R_SCALE, conv=(CONV_T, CONV_R), judge="coarse"),
+            "learned_full": lambda o, T, k: o.track_policy(T[:3, 3], T[:3, :3], c2["learned_full"]["_preds"][k - 1],
+                                                           ws.T_SCALE, ws.R_SCALE, conv=(CONV_T, CONV_R),
+                                                           judge="full"),
+        }
+        for f, call in calls.items():
+            t0 = time.time()
+            ref_p, _ = oracle_sequence(seq_c2, mp, True, call)
+            c2[f].update(error_stats(c2[f]["poses"][:mp], ref_p, "vs_fp64_same_policy"))
+            log(f"learned: {f} vs the FP64 oracle of its policy on {mp} frames ({time.time() - t0:.1f}s)")
+    base = c2["reference"]["frames_per_s"]
+    ctrl = c2["plain_policy"]["frames_per_s"]
+    for f in FORMS:
+        c2[f]["speedup_vs_reference"] = round(c2[f]["frames_per_s"] / base, 3)
+        c2[f]["speedup_vs_plain_policy"] = round(c2[f]["frames_per_s"] / ctrl, 3)
+    res["smooth_640x480"] = {"workload": f"the headline's sequence, frames 1..{nsteps}",
+                             **{f: public(c2[f]) for f in FORMS}}
+    log(f"learned smooth {res['smooth_640x480']}")
+
+    # ---- fast motion: bursts up to 8 cm / 6 deg per frame
+    nb = args.burst_frames
+    if nb > 1:
+        cam = seq_c2.cam
+        bposes = synth.spline_trajectory(nb, seed=args.seed + 1, max_step_m=0.08, max_step_deg=6.0, bursts=True)
+        bseq = DeviceSequence(cam, bposes, synth.Scene(args.seed + 1), 5000, what="bursts")
+        steps = [synth.pose_error(bposes[k], bposes[k - 1]) for k in range(1, nb)]
+        out = {"workload": f"{nb} frames 640x480, bursts up to {1e3 * max(q[0] for q in steps):.1f} mm / "
+                           f"{max(q[1] for q in steps):.2f} deg per frame",
+               "frames": nb}
+        st = [k / 30.0 for k in range(1, nb)]
+        gt = ev.trajectory(st, bposes[1:nb])
+        for f in FORMS:
+            r = run_form(bseq, forms, f, nb - 1, min(warm, nb - 1), trace=True)
+            est = r["poses"]
+            errs = [synth.pose_error(est[k - 1], bposes[k]) for k in range(1, nb)]
+            rpe = [synth.pose_error(np.linalg.inv(est[k - 2]) @ est[k - 1],
+                                    np.linalg.inv(bposes[k - 1]) @ bposes[k])[0] for k in range(2, nb)]
+            r.update({"ate_mean_mm": round(1e3 * float(np.mean([q[0] for q in errs])), 2),
+                      "ate_rmse_aligned_mm": round(1e3 * ev.ate(gt, ev.trajectory(st, est))["rmse"], 2),
+                      "rpe_mean_mm": round(1e3 * float(np.mean(rpe)), 3),
+                      "frames_lost_gt_5cm": int(sum(1 for q in rpe if q > 0.05))})
+            out[f] = public(r)
+        base_b = out["reference"]["frames_per_s"]
+        for f in FORMS:
+            out[f]["speedup_vs_reference"] = round(out[f]["frames_per_s"] / base_b, 3)
+        res["bursts_640x480"] = out
+        log(f"learned bursts {out}")
+        del bseq
+    net.close()
+    return res
+
+
 BATCH_LAYOUT = (("CAM_320", 16), ("CAM_640", 32), ("CAM_848", 16))
 
 
-def batched_pairs(args) -> dict:
+def batched_pairs(args, peak_copy=None) -> dict:
     """64 independent pairs of uneven size (16 at 320x240, 32 at 640x480, 16
     at 848x480), seeded rooms, per-pair motion up to 3 cm / 2 deg, tracked by
     one batched call (ICP+RGB+SO3, 10/5/4) replayed as a hipGraph.  Reports
@@ -127,8 +549,7 @@ def batched_pairs(args) -> dict:
                "vs_fp64_oracle_max_deg": round(float(np.max([e[1] for e in errs])), 6)}
         log(f"batch accuracy {acc}")
 
-    # launch shapes for the batch's own launches
-    b.autotune(False, ICP_WEIGHT)
+    # launch shapes: the library's table row for batches
     bshapes = {f"{'ab'[k]}{l}": list(b.get_step_shape(k, l)) for k in range(2) for l in range(3)}
     log(f"batch shapes {bshapes}")
 
@@ -155,15 +576,25 @@ def batched_pairs(args) -> dict:
     l0_gbs = BYTES_A * npx[0] / (us_a[0] * 1e-6) / 1e9
     joint = sum(iters[l] * (BYTES_ICP + BYTES_RGB) * npx[l] for l in range(3)) / \
         (sum(iters[l] * (us_a[l] + us_b[l]) for l in range(3)) * 1e-6) / 1e9
+    l0_joint = (BYTES_ICP + BYTES_RGB) * npx[0] / ((us_a[0] + us_b[0]) * 1e-6) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_step_batch_traffic_64pairs.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)
     out = {"workload": "64 pairs: 16x320x240 + 32x640x480 + 16x848x480, <=3cm/2deg, ICP+RGB+SO3 10/5/4, "
                        "one batched call per step (hipGraph)",
            "pairs_per_s": round(len(odos) / call_s, 1), "ms_per_call": round(call_s * 1e3, 3),
            "roofline": {"bound": "hbm", "kernel": "k_step_a (batched)", "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "peak_measured": peak_copy, "frac_measured": round(ach / peak_copy, 4) if peak_copy else None,
                         "level0_GBs": round(l0_gbs, 1), "bytes_per_launch": int(bytes_a),
-                        "avg_launch_us": round(avg_a, 2)},
+                        "avg_launch_us": round(avg_a, 2), "traffic": traffic},
            "per_level_us": {"k_step_a": [round(x, 2) for x in us_a], "k_step_b": [round(x, 2) for x in us_b]},
            "joint_iteration_gbs": round(joint, 1),
+           "joint_frac": round(joint / HBM_PEAK_GBS, 4),
+           "joint_frac_measured": round(joint / peak_copy, 4) if peak_copy else None,
+           "level0_joint_gbs": round(l0_joint, 1),
            "blocks_per_launch": [b.get_blocks(0, l) for l in range(3)],
            "shapes": bshapes,
            "accuracy": acc}
@@ -172,6 +603,7 @@ def batched_pairs(args) -> dict:
     for o in odos:
         o.close()
     return out
+
 
 
 def _splat_traffic():
@@ -327,77 +759,105 @@ def surfel_fusion(args) -> dict:
     return res
 
 
+
+
 def frame_loop_sequence(args) -> dict:
-    """The reference's frame loop without loop closure on the device
-    (slam_amd/frame_loop.py): per frame the bilateral filter, frame-to-model
-    ICP+RGB+SO3 tracking against the filled-in splat prediction of the surfel
-    model, the fusion and cleaning, and the next prediction, with the
-    reference configs' values (depth cutoff 3 m, confidence 10, timeDelta
-    200).  A 640x480 sequence of one room with bursts up to 10 cm / 8 deg per
-    frame, frames resident on the device; reports the whole loop's frames/s
-    (host reads of the pose and the surfel count included, as the reference
-    loop makes them), the model size, and ATE / RPE against the generated
-    ground truth."""
+    """The reference's frame loop without loop closure on the device, for n
+    sequences in lockstep (slam_amd/frame_loop.py BatchFrameLoop): per frame
+    each sequence's bilateral filter, its init calls, one batched tracking
+    call for all of them (with the learned start: one network forward of
+    batch n first), then each sequence's fusion, cleaning and next
+    prediction, with the reference configs' values (depth cutoff 3 m,
+    confidence 10, timeDelta 200).  Eight 640x480 sequences of seeded rooms
+    with bursts up to 10 cm / 8 deg per frame, frames resident on the device;
+    reports the whole loop's sequence-frames/s (host reads of each pose and
+    surfel count included, as the reference loop makes them), the model
+    sizes, ATE / RPE against the generated ground truth, the fall-back rate
+    and the Gauss-Newton iterations per frame, plain against learned."""
     import torch
     from slam_amd import evaluate, synth
-    from slam_amd.frame_loop import FrameLoop
+    from slam_amd.frame_loop import BatchFrameLoop, FrameLoop, LoopParams
+    from slam_amd.odometry import JUDGE_COARSE
 
     cam = synth.CAM_640
-    n = args.loop_frames
-    poses = synth.spline_trajectory(n, seed=920, max_step_m=0.10, max_step_deg=8.0, bursts=True)
-    T0i = np.linalg.inv(poses[0])
-    gt = [T0i @ T for T in poses]
-    scene = synth.Scene(920)
+    n, ns = args.loop_frames, args.loop_seqs
+    seqs = []
     t0 = time.time()
-    frames = []
-    for k, T in enumerate(poses):
-        f = scene.render(T, cam, noise_seed=92000 + k, model=False)
-        frames.append((torch.from_numpy(f.rgb).cuda(), torch.from_numpy(f.depth).cuda()))
-        if (k + 1) % 500 == 0:
-            log(f"frame loop: rendered {k + 1}/{n} frames")
+    maxdt = maxdr = 0.0
+    for i in range(ns):
+        seed = 920 + i
+        poses = synth.spline_trajectory(n, seed=seed, max_step_m=0.10, max_step_deg=8.0, bursts=True)
+        steps = [synth.pose_error(poses[k + 1], poses[k]) for k in range(n - 1)]
+        maxdt = max(maxdt, max(q[0] for q in steps))
+        maxdr = max(maxdr, max(q[1] for q in steps))
+        T0i = np.linalg.inv(poses[0])
+        scene = synth.Scene(seed)
+        frames = []
+        for k, T in enumerate(poses):
+            f = scene.render(T, cam, noise_seed=92000 + 5000 * i + k, model=False)
+            frames.append((torch.from_numpy(f.rgb).cuda(), torch.from_numpy(f.depth).cuda()))
+        seqs.append(([T0i @ T for T in poses], frames))
+        log(f"frame loop: sequence {i + 1}/{ns} resident ({time.time() - t0:.1f}s)")
     torch.cuda.synchronize()
-    log(f"frame loop: {n} frames resident in {time.time() - t0:.1f}s")
-    steps = [np.linalg.inv(gt[k]) @ gt[k + 1] for k in range(n - 1)]
-    from slam_amd.frame_loop import LoopParams
 
     def run(learned: bool) -> dict:
-        loop = FrameLoop(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, LoopParams(learned=learned))
+        p = LoopParams(learned=learned, conv_t=CONV_T if learned else 0.0, conv_r=CONV_R if learned else 0.0,
+                       judge=JUDGE_COARSE, max_surfels=args.loop_surfels)
+        loop = BatchFrameLoop(ns, cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, p)
         warm = min(5, n)
         for k in range(warm):
-            loop.process(*frames[k])
+            loop.process([s[1][k][0] for s in seqs], [s[1][k][1] for s in seqs])
         torch.cuda.synchronize()
         t1 = time.time()
         for k in range(warm, n):
-            loop.process(*frames[k])
+            loop.process([s[1][k][0] for s in seqs], [s[1][k][1] for s in seqs])
         torch.cuda.synchronize()
         el = time.time() - t1
         stamps = [float(k) for k in range(n)]
-        gtt = evaluate.trajectory(stamps, gt)
-        est = evaluate.trajectory(stamps, [T.astype(np.float64) for T in loop.poses])
-        ate = evaluate.ate(gtt, est)
-        rpe = evaluate.rpe(gtt, est, delta=1, delta_unit="f")
-        M = loop.model.model()
-        st = loop.odo.stats()
-        r = {
-            "frames_per_s": round((n - warm) / el, 1),
-            "ms_per_frame": round(1e3 * el / (n - warm), 3),
-            "surfels_final": int(M.shape[0]),
-            "surfels_stable": int((M[:, 3] > 10.0).sum()),
-            "ate_rmse_mm": round(1e3 * ate["rmse"], 2),
-            "ate_max_mm": round(1e3 * ate["max"], 2),
-            "rpe_rmse_mm": round(1e3 * rpe["translational"]["rmse"], 3),
-            "rpe_rmse_deg": round(rpe["rotational_deg"]["rmse"], 4),
-            "frames_lost_gt_5cm": int(sum(np.linalg.norm((np.linalg.inv(g) @ e.astype(np.float64))[:3, 3]) > 0.05
-                                          for g, e in zip(gt, loop.poses))),
-        }
-        lost = [k for k, (g, e) in enumerate(zip(gt, loop.poses))
-                if np.linalg.norm((np.linalg.inv(g) @ e.astype(np.float64))[:3, 3]) > 0.05]
-        r["first_lost_frame"] = lost[0] if lost else None
+        per = []
+        for i, (gt, _) in enumerate(seqs):
+            gtt = evaluate.trajectory(stamps, gt)
+            est = evaluate.trajectory(stamps, [T.astype(np.float64) for T in loop.poses[i]])
+            ate = evaluate.ate(gtt, est)
+            rpe = evaluate.rpe(gtt, est, delta=1, delta_unit="f")
+            lost = [k for k, (g, e) in enumerate(zip(gt, loop.poses[i]))
+                    if np.linalg.norm((np.linalg.inv(g) @ e.astype(np.float64))[:3, 3]) > 0.05]
+            c = loop.counters(i)
+            per.append({"ate_rmse_mm": round(1e3 * ate["rmse"], 2), "ate_max_mm": round(1e3 * ate["max"], 2),
+                        "rpe_rmse_mm": round(1e3 * rpe["translational"]["rmse"], 3),
+                        "rpe_rmse_deg": round(rpe["rotational_deg"]["rmse"], 4),
+                        "frames_lost_gt_5cm": len(lost), "first_lost_frame": lost[0] if lost else None,
+                        "surfels_final": loop.models[i].lastCount(),
+                        "iters_per_frame": round(c["gn_iterations"] / max(1, c["frames_tracked"]), 3),
+                        "alt_iters_per_frame": round(c["alt_iterations"] / max(1, c["frames_tracked"]), 3),
+                        "fallback_rate": (round(1.0 - c["predictions_kept"] / c["predictions"], 4)
+                                          if c["predictions"] else None)})
+        loop.close()
+        agg = {"sequence_frames_per_s": round(ns * (n - warm) / el, 1),
+               "ms_per_frame_of_all_sequences": round(1e3 * el / (n - warm), 3),
+               "ate_rmse_mm_median": float(np.median([q["ate_rmse_mm"] for q in per])),
+               "ate_rmse_mm_mean": round(float(np.mean([q["ate_rmse_mm"] for q in per])), 2),
+               "rpe_rmse_mm_median": float(np.median([q["rpe_rmse_mm"] for q in per])),
+               "frames_lost_gt_5cm_total": int(sum(q["frames_lost_gt_5cm"] for q in per)),
+               "iters_per_frame": round(float(np.mean([q["iters_per_frame"] for q in per])), 3),
+               "alt_iters_per_frame": round(float(np.mean([q["alt_iters_per_frame"] for q in per])), 3)}
         if learned:
-            r["prediction_kept_frac"] = round(st["seedKept"] / max(1, st["seedCalls"]), 3)
-        return r
+            agg["fallback_rate"] = round(float(np.mean([q["fallback_rate"] for q in per])), 4)
+        log(f"frame loop {'learned' if learned else 'plain'}: {agg}")
+        return {**agg, "per_sequence": per}
+
+    res = {"workload": f"{ns} sequences x {n} frames 640x480 (frame loop: track, fuse, clean, predict), bursts up to "
+                       f"{1e3 * maxdt:.0f} mm / {maxdr:.1f} deg per frame, no loop closure; one batched tracking "
+                       f"call per frame",
+           "sequences": ns, "frames": n, "max_step_mm": round(1e3 * maxdt, 1), "max_step_deg": round(maxdr, 2),
+           "plain": run(False)}
+    from slam_amd import warmstart as ws
+    res["learned"] = run(True) if os.path.exists(ws.WEIGHTS) else None
+    del seqs
+    torch.cuda.empty_cache()
 
     # the loop's own speed on a sequence it tracks throughout (smooth, 300 frames)
+    scene = synth.Scene(921)
     sposes = synth.spline_trajectory(300, seed=921, max_step_m=0.02, max_step_deg=1.0)
     sframes = [(torch.from_numpy(f.rgb).cuda(), torch.from_numpy(f.depth).cuda()) for f in
                (scene.render(T, cam, noise_seed=93000 + k, model=False) for k, T in enumerate(sposes))]
@@ -413,156 +873,15 @@ def frame_loop_sequence(args) -> dict:
     s0i = np.linalg.inv(sposes[0])
     sgt = evaluate.trajectory([float(k) for k in range(len(sposes))], [s0i @ T for T in sposes])
     sest = evaluate.trajectory([float(k) for k in range(len(sposes))], [T.astype(np.float64) for T in sloop.poses])
-    smooth = {"frames": len(sposes), "frames_per_s": round((len(sposes) - 5) / sel, 1),
-              "ms_per_frame": round(1e3 * sel / (len(sposes) - 5), 3), "surfels_final": sloop.model.lastCount(),
-              "ate_rmse_mm": round(1e3 * evaluate.ate(sgt, sest)["rmse"], 2),
-              "workload": "300 frames on a <=2 cm / 1 deg spline, the same room, native loop"}
+    res["smooth"] = {"frames": len(sposes), "frames_per_s": round((len(sposes) - 5) / sel, 1),
+                     "ms_per_frame": round(1e3 * sel / (len(sposes) - 5), 3),
+                     "surfels_final": sloop.model.lastCount(),
+                     "ate_rmse_mm": round(1e3 * evaluate.ate(sgt, sest)["rmse"], 2),
+                     "workload": "one sequence, 300 frames on a <=2 cm / 1 deg spline, native loop"}
     sloop.close()
     del sframes
-    res = {
-        "smooth": smooth,
-        "workload": f"{n}-frame 640x480 frame loop (track, fuse, clean, predict), bursts up to "
-                    f"{1e3 * max(np.linalg.norm(s[:3, 3]) for s in steps):.0f} mm / "
-                    f"{max(np.degrees(np.arccos(np.clip((np.trace(s[:3, :3]) - 1) / 2, -1, 1))) for s in steps):.1f}"
-                    " deg per frame, no loop closure",
-        "frames": n,
-        "plain": run(False),
-        "learned": run(True) if os.path.exists(os.path.join(ROOT, "slam_amd", "weights", "warmstart.npz")) else None,
-    }
-    log(f"frame loop {res}")
+    log(f"frame loop {res['smooth']}")
     return res
-
-
-def learned_vs_plain(args, cam, poses, views, L, h, odo, reset, plain_fps, plain_est, ref64) -> dict | None:
-    """The learned form on the same sequence: per frame the same init calls,
-    then the network input from the handle's level-2 buffers, the network
-    (bf16 MFMA) and a seeded call with a short schedule and the keep-or-fall-
-    back rule, all enqueued on the handle's stream.  Reports its frames/s next
-    to the plain form's, its pose error against the FP64 oracle (plain init,
-    full schedule) on the CPU sample, and -- on a fast-motion sequence with
-    bursts up to 8 cm / 6 deg per frame -- both forms' error against the
-    rendered ground truth."""
-    import torch
-    from slam_amd import synth, warmstart as ws
-    if not os.path.exists(ws.WEIGHTS):
-        log("learned: no trained weights; skipped")
-        return None
-    iters = [int(v) for v in args.learned_iters.split(",")]
-    net = ws.WarmStartNet(max_batch=1)
-    net.load()
-    net.set_stream(odo.stream)
-    x = torch.zeros((1, 60, 80, 4), dtype=torch.bfloat16, device="cuda")
-    pred = torch.zeros((1, 6), dtype=torch.float32, device="cuda")
-    it = np.ascontiguousarray(iters, dtype=np.int32)
-    by = ctypes.byref
-
-    def run_learned(k0, k1, vw):
-        for k in range(k0, k1):
-            L.odom_init_icp_model_current(h, by(vw["v4"][k - 1]), by(vw["n4"][k - 1]), CUTOFF)
-            L.odom_init_rgb_model(h, by(vw["rgba"][k - 1]))
-            L.odom_init_icp_raw_depth(h, by(vw["depth"][k]), MAXD, CUTOFF)
-            L.odom_init_rgb(h, by(vw["rgb"][k]))
-            L.odom_init_warmstart_input(h, x.data_ptr())
-            ws._lib().cnn_forward(net.handle, x.data_ptr(), 1, pred.data_ptr())
-            L.odom_track_seeded(h, pred.data_ptr(), ctypes.c_float(ws.T_SCALE), ctypes.c_float(ws.R_SCALE),
-                                it.ctypes.data, 0, ctypes.c_float(ICP_WEIGHT), 1)
-
-    nsteps = args.steps
-
-    def pose12(p):
-        T = np.eye(4)
-        T[:3, :4] = p.reshape(3, 4)
-        return T
-
-    def timed(schedule) -> dict:
-        it[:] = schedule
-        reset()
-        run_learned(1, 1 + args.warmup, views)
-        odo.synchronize()
-        log_t = torch.zeros((nsteps + 1, 12), dtype=torch.float32, device="cuda")
-        odo.set_pose_log(log_t.data_ptr(), nsteps + 1)
-        reset()
-        torch.cuda.synchronize()
-        odo.synchronize()
-        s0 = odo.stats()
-        t0 = time.perf_counter()
-        run_learned(1, 1 + nsteps, views)
-        odo.synchronize()
-        el = time.perf_counter() - t0
-        s1 = odo.stats()
-        fps = nsteps / el
-        est = log_t.cpu().numpy()[:nsteps]
-        kept = (s1["seedKept"] - s0["seedKept"]) / max(1, s1["seedCalls"] - s0["seedCalls"])
-        r = {"schedule": list(schedule), "frames_per_s": round(fps, 2), "speedup_vs_plain": round(fps / plain_fps, 3),
-             "prediction_kept_frac": round(kept, 3)}
-        if ref64 is not None:
-            n = len(ref64)
-            e = [synth.pose_error(pose12(est[k]), ref64[k]) for k in range(n)]
-            r["vs_fp64_oracle_mean_mm"] = round(1e3 * float(np.mean([q[0] for q in e])), 4)
-            r["vs_fp64_oracle_max_mm"] = round(1e3 * float(np.max([q[0] for q in e])), 4)
-            r["vs_fp64_oracle_mean_deg"] = round(float(np.mean([q[1] for q in e])), 5)
-        dt_l = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, nsteps + 1)]
-        r["drift_vs_truth_mean_mm"] = round(1e3 * float(np.mean([d[0] for d in dt_l])), 3)
-        return r
-
-    out = {"workload": "same 640x480 sequence, network init + seeded ICP+RGB, no SO3, fallback to the previous "
-                       "pose when the level-2 association from the prediction is worse",
-           "short_schedule": timed(iters), "full_schedule": timed([10, 5, 4])}
-    it[:] = iters
-
-    # fast motion: bursts up to 8 cm / 6 deg per frame, both forms vs truth
-    nb = args.burst_frames
-    if nb > 1:
-        bposes = synth.spline_trajectory(nb, seed=args.seed + 1, max_step_m=0.08, max_step_deg=6.0, bursts=True)
-        scene = synth.Scene(args.seed + 1)
-        bdev = {k: [] for k in ("depth", "rgb", "v4", "n4", "rgba")}
-        for i, T in enumerate(bposes):
-            f = scene.render(T, cam, noise_seed=5000 + i)
-            bdev["depth"].append(torch.from_numpy(f.depth).cuda())
-            bdev["rgb"].append(torch.from_numpy(f.rgb).cuda())
-            bdev["v4"].append(torch.from_numpy(f.vert4).cuda())
-            bdev["n4"].append(torch.from_numpy(f.norm4).cuda())
-            bdev["rgba"].append(torch.from_numpy(f.rgba).cuda())
-        from slam_amd.odometry import frame_view, FMT_DEPTH_U16, FMT_RGB8, FMT_RGBA8, FMT_RGBA32F
-        bv = {k: [frame_view(t, fmt)[0] for t in bdev[k]] for k, fmt in
-              (("depth", FMT_DEPTH_U16), ("rgb", FMT_RGB8), ("v4", FMT_RGBA32F), ("n4", FMT_RGBA32F),
-               ("rgba", FMT_RGBA8))}
-        res = {}
-        for mode in ("plain", "learned"):
-            T0 = bposes[0]
-            odo.set_pose(T0[:3, 3], T0[:3, :3])
-            odo.initFirstRGB(bdev["rgb"][0])
-            lg = torch.zeros((nb, 12), dtype=torch.float32, device="cuda")
-            odo.set_pose_log(lg.data_ptr(), nb)
-            if mode == "plain":
-                for k in range(1, nb):
-                    L.odom_init_icp_model_current(h, by(bv["v4"][k - 1]), by(bv["n4"][k - 1]), CUTOFF)
-                    L.odom_init_rgb_model(h, by(bv["rgba"][k - 1]))
-                    L.odom_init_icp_raw_depth(h, by(bv["depth"][k]), MAXD, CUTOFF)
-                    L.odom_init_rgb(h, by(bv["rgb"][k]))
-                    L.odom_track_async(h, 0, ICP_WEIGHT, 1, 0, 1)
-            else:
-                run_learned(1, nb, bv)
-            odo.synchronize()
-            e = lg.cpu().numpy()[: nb - 1]
-            errs = [synth.pose_error(pose12(e[k - 1]), bposes[k]) for k in range(1, nb)]
-            # relative pose error per frame: estimated step vs true step
-            rpe = [synth.pose_error(np.linalg.inv(pose12(e[k - 2])) @ pose12(e[k - 1]),
-                                    np.linalg.inv(bposes[k - 1]) @ bposes[k])[0] for k in range(2, nb)]
-            from slam_amd import evaluate as ev
-            st = [k / 30.0 for k in range(1, nb)]
-            aligned = ev.ate(ev.trajectory(st, bposes[1:nb]), ev.trajectory(st, [pose12(e[k - 1]) for k in range(1, nb)]))
-            res[mode] = {"ate_mean_mm": round(1e3 * float(np.mean([q[0] for q in errs])), 2),
-                         "ate_rmse_aligned_mm": round(1e3 * aligned["rmse"], 2),
-                         "rpe_mean_mm": round(1e3 * float(np.mean(rpe)), 3),
-                         "frames_lost_gt_5cm": int(sum(1 for r in rpe if r > 0.05))}
-        steps = [synth.pose_error(bposes[k], bposes[k - 1]) for k in range(1, nb)]
-        out["bursts"] = {"frames": nb, "max_step_mm": round(1e3 * max(q[0] for q in steps), 1),
-                         "max_step_deg": round(max(q[1] for q in steps), 2), **{k: v for k, v in res.items()}}
-    log(f"learned {out}")
-    odo.set_pose_log(0, 0)
-    net.close()
-    return out
 
 
 def main() -> int:
@@ -572,11 +891,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-frames", type=int, default=20, help="frames of the CPU baseline / FP64 oracle sample")
+    ap.add_argument("--policy-oracle-frames", type=int, default=8,
+                    help="frames of each policy checked against the FP64 oracle of the same policy")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-pair measurement")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched 64-pair measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start measurement")
-    ap.add_argument("--learned-iters", default="5,2,2", help="schedule of the learned form (levels 0,1,2)")
     ap.add_argument("--burst-frames", type=int, default=150)
     ap.add_argument("--no-graph", action="store_true", help="launch tracking calls kernel by kernel")
     ap.add_argument("--pose-log", default=None, help="write the timed frames' poses in the frame loop's log format")
@@ -588,6 +909,8 @@ def main() -> int:
     ap.add_argument("--no-loop", action="store_true", help="skip the full frame-loop sequences")
     ap.add_argument("--loop-only", action="store_true", help="run only the full frame-loop sequences")
     ap.add_argument("--loop-frames", type=int, default=2000)
+    ap.add_argument("--loop-seqs", type=int, default=8)
+    ap.add_argument("--loop-surfels", type=int, default=2_500_000)
     ap.add_argument("--fusion-only", action="store_true", help="measure only the surfel fusion (profiling)")
     ap.add_argument("--fusion-frames", type=int, default=10)
     ap.add_argument("--fusion-reps", type=int, default=20)
@@ -623,94 +946,40 @@ def main() -> int:
     if args.loop_only:
         print(json.dumps({"frame_loop": frame_loop_sequence(args)}), flush=True)
         return 0
+    if args.batch_only:
+        print(json.dumps({"batched": batched_pairs(args)}), flush=True)
+        return 0
 
     import torch
     from slam_amd import synth
-    from slam_amd.odometry import RGBDOdometry, device_info, lib, frame_view, FMT_DEPTH_U16, FMT_RGB8, \
-        FMT_RGBA8, FMT_RGBA32F
+    from slam_amd.odometry import RGBDOdometry, device_info, lib
 
     name, arch = device_info()
     log(f"device {name} {arch}")
+    copy_peak = stream_copy_peak()
+    log(f"copy peak {copy_peak}")
+    peak_copy = copy_peak["GBs"]
     cam = synth.CAM_640
     nframes = max(args.steps, args.warmup) + 1
-    scene = synth.Scene(args.seed)
-    poses = synth.spline_trajectory(nframes, seed=args.seed)
-    keep_host = args.cpu_frames + 1 if not args.no_cpu else 0
-    dev = {k: [] for k in ("depth", "rgb", "v4", "n4", "rgba")}
-    host = []
-    t0 = time.time()
-    for i, T in enumerate(poses):
-        f = scene.render(T, cam, noise_seed=1000 + i)
-        dev["depth"].append(torch.from_numpy(f.depth).cuda())
-        dev["rgb"].append(torch.from_numpy(f.rgb).cuda())
-        dev["v4"].append(torch.from_numpy(f.vert4).cuda())
-        dev["n4"].append(torch.from_numpy(f.norm4).cuda())
-        dev["rgba"].append(torch.from_numpy(f.rgba).cuda())
-        if i < keep_host:
-            host.append(f)
-        if (i + 1) % 100 == 0:
-            log(f"rendered {i + 1}/{nframes} frames")
-    torch.cuda.synchronize()
-    log(f"inputs resident: {nframes} frames in {time.time() - t0:.1f}s")
+    poses = synth.spline_trajectory(max(nframes, args.cpu_frames + 5), seed=args.seed)
+    keep_host = args.cpu_frames + 5 if not args.no_cpu else 0
+    seq = DeviceSequence(cam, poses, synth.Scene(args.seed), 1000, keep_host=keep_host, what="headline")
 
-    # prebuilt frame views: the timed loop is only C calls
-    views = {k: [frame_view(t, fmt)[0] for t in dev[k]] for k, fmt in
-             (("depth", FMT_DEPTH_U16), ("rgb", FMT_RGB8), ("v4", FMT_RGBA32F), ("n4", FMT_RGBA32F),
-              ("rgba", FMT_RGBA8))}
     L = lib()
     odo = RGBDOdometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
-    h = odo.handle
     odo.set_graph(not args.no_graph)
-    by = ctypes.byref
-
-    def reset():
-        T0 = poses[0]
-        odo.set_pose(T0[:3, 3], T0[:3, :3])
-        odo.initFirstRGB(dev["rgb"][0])
-
-    def run(k0: int, k1: int) -> None:
-        for k in range(k0, k1):
-            L.odom_init_icp_model_current(h, by(views["v4"][k - 1]), by(views["n4"][k - 1]), CUTOFF)
-            L.odom_init_rgb_model(h, by(views["rgba"][k - 1]))
-            L.odom_init_icp_raw_depth(h, by(views["depth"][k]), MAXD, CUTOFF)
-            L.odom_init_rgb(h, by(views["rgb"][k]))
-            L.odom_track_async(h, 0, ICP_WEIGHT, 1, 0, 1)
-
-    # shapes: autotune on the first frame's inputs
-    reset()
-    run(1, 2)
-    odo.synchronize()
-    odo.autotune(False, ICP_WEIGHT)
     shapes = {f"{'ab'[k]}{l}": list(odo.get_step_shape(k, l)) for k in range(2) for l in range(3)}
-    log(f"autotuned shapes {shapes}")
+    log(f"launch shapes (library table) {shapes}")
+    forms = Forms(L, odo)
 
-    reset()
-    run(1, 1 + args.warmup)
-    odo.synchronize()
-
-    pose_log = torch.zeros((nframes, 12), dtype=torch.float32, device="cuda")
-    odo.set_pose_log(pose_log.data_ptr(), nframes)
-    reset()
-    torch.cuda.synchronize()
-    odo.synchronize()
-    t0 = time.perf_counter()
-    run(1, 1 + args.steps)
-    enqueued = time.perf_counter() - t0
-    odo.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    fps = args.steps / elapsed
-    log(f"timed {args.steps} frames in {elapsed * 1e3:.1f} ms: {fps:.1f} frames/s "
-        f"(host enqueue {enqueued * 1e3:.1f} ms)")
-    est = pose_log.cpu().numpy()[: args.steps]
+    head = run_form(seq, forms, "reference", args.steps, args.warmup)
+    fps = head["frames_per_s"]
+    elapsed = args.steps / fps
+    est = head["poses"]
+    log(f"timed {args.steps} frames: {fps:.1f} frames/s")
     stats = odo.stats()
 
-    def pose12(p):
-        T = np.eye(4)
-        T[:3, :4] = p.reshape(3, 4)
-        return T
-
-    drift = [synth.pose_error(pose12(est[k - 1]), poses[k]) for k in range(1, args.steps + 1)]
+    drift = [synth.pose_error(est[k - 1], poses[k]) for k in range(1, args.steps + 1)]
     drift_t = float(np.mean([d[0] for d in drift]))
     drift_r = float(np.mean([d[1] for d in drift]))
     # the reference's benchmark measures: ATE after Horn alignment and the
@@ -718,9 +987,8 @@ def main() -> int:
     from slam_amd import dataio
     from slam_amd import evaluate as ev
     stamps = [k / 30.0 for k in range(1, args.steps + 1)]
-    est_poses = [pose12(est[k - 1]) for k in range(1, args.steps + 1)]
     gt_traj = ev.trajectory(stamps, [poses[k] for k in range(1, args.steps + 1)])
-    est_traj = ev.trajectory(stamps, est_poses)
+    est_traj = ev.trajectory(stamps, est)
     ate_r = ev.ate(gt_traj, est_traj)
     rpe_r = ev.rpe(gt_traj, est_traj, delta=1, delta_unit="f")
     trajectory_eval = {"ate_rmse_mm": round(1e3 * ate_r["rmse"], 3), "ate_max_mm": round(1e3 * ate_r["max"], 3),
@@ -729,7 +997,7 @@ def main() -> int:
                        "evaluator": "slam_amd/evaluate.py (benchmark/evaluate_ate.py, evaluate_rpe.py: "
                                     "Horn-aligned ATE, 1-frame RPE)"}
     if args.pose_log:
-        dataio.write_pose_log(args.pose_log, list(range(1, args.steps + 1)), est_poses)
+        dataio.write_pose_log(args.pose_log, list(range(1, args.steps + 1)), est)
 
     # roofline of the step kernels: per level, the last iteration of a
     # completed call relaunched back to back between one HIP event pair on the
@@ -759,6 +1027,7 @@ def main() -> int:
             traffic = int(json.load(fh)["traffic_bytes_per_launch_10_5_4"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "peak_measured": peak_copy, "frac_measured": round(achieved / peak_copy, 4),
                 "kernel": "k_step_a (ICP products + photometric association)" if dom == 0 else
                 "k_step_b (photometric products)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_us, 2),
@@ -769,47 +1038,29 @@ def main() -> int:
     cpu = None
     accuracy = None
     ref64 = None
+    cpu_detail = None
     if not args.no_cpu and args.cpu_frames > 0:
-        from oracle import oracle as orc
         n = min(args.cpu_frames, args.steps)
-
-        def oracle_run(fp64: bool):
-            o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, fp64=fp64)
-            T = poses[0].copy()
-            o.initFirstRGB(host[0].rgb)
-            out = []
-            t_start = time.perf_counter()
-            for k in range(1, n + 1):
-                o.initICPModel(host[k - 1].vert4, host[k - 1].norm4, CUTOFF, T.astype(np.float32))
-                o.initRGBModel(host[k - 1].rgba)
-                o.initICP(orc.bilateral(host[k].depth, MAXD), CUTOFF)
-                o.initRGB(host[k].rgb)
-                t, R = o.getIncrementalTransformation(T[:3, 3], T[:3, :3], False, ICP_WEIGHT, True, False, True)
-                T = synth.make_pose(R.astype(np.float64), t.astype(np.float64))
-                out.append(T)
-            return out, time.perf_counter() - t_start
-
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ref32, t32 = oracle_run(False)
-        log(f"cpu baseline (FP32 restatement, {cores} threads): {n} frames in {t32:.1f}s")
-        ref64, t64 = oracle_run(True)
-        errs = [synth.pose_error(pose12(est[k]), ref64[k]) for k in range(n)]
-        errs32 = [synth.pose_error(pose12(est[k]), ref32[k]) for k in range(n)]
-        accuracy = {"frames": n,
-                    "vs_fp64_oracle_mean_mm": round(1e3 * float(np.mean([e[0] for e in errs])), 5),
-                    "vs_fp64_oracle_max_mm": round(1e3 * float(np.max([e[0] for e in errs])), 5),
-                    "vs_fp64_oracle_mean_deg": round(float(np.mean([e[1] for e in errs])), 6),
-                    "vs_fp64_oracle_max_deg": round(float(np.max([e[1] for e in errs])), 6),
-                    "vs_fp32_oracle_max_mm": round(1e3 * float(np.max([e[0] for e in errs32])), 5)}
-        cpu = {"value": round(n / t32, 3), "unit": "frames/s", "cores": cores, "kind": "port",
-               "sample": f"first {n} frames of the same 640x480 sequence, same calls, FP32 C restatement "
-                         f"(oracle/odom_oracle.c) with OpenMP over rows"}
+        cpu_detail = cpu_baseline(args, seq)
+        c2 = cpu_detail["c2_640x480_icp_rgb_so3"]["all_cores"]
+        cpu = {"value": c2["frames_per_s"], "unit": "frames/s", "cores": c2["threads"], "kind": "port",
+               "sample": f"640x480 ICP+RGB+SO3 frames of the same sequence (3 warm-up, then the median of "
+                         f"{c2['frames']}), the FP32 C restatement (oracle/odom_oracle.c) with OpenMP over rows "
+                         f"on {c2['threads']} threads; 1 core: "
+                         f"{cpu_detail['c2_640x480_icp_rgb_so3']['1_core']['frames_per_s']} frames/s",
+               "median_ms": c2["median_ms"], "p90_ms": c2["p90_ms"], "detail": cpu_detail}
+        ref64, t64 = oracle_sequence(seq, n, True, reference_call)
+        ref32, _ = oracle_sequence(seq, n, False, reference_call)
+        accuracy = {"frames": n, **error_stats(est[:n], ref64, "vs_fp64_oracle"),
+                    "vs_fp32_oracle_max_mm": error_stats(est[:n], ref32, "x")["x_max_mm"]}
         log(f"accuracy {accuracy}")
 
-    learned = None
-    if not args.no_learned:
-        learned = learned_vs_plain(args, cam, poses, views, L, h, odo, reset, fps, est, ref64)
-    batched = None if args.no_batch else batched_pairs(args)
+    c1 = c1_gpu(args)
+    log(f"C1 {c1}")
+    learned = None if args.no_learned else learned_vs_plain(args, seq, L, odo, ref64)
+    del seq
+    torch.cuda.empty_cache()
+    batched = None if args.no_batch else batched_pairs(args, peak_copy)
     splat_res = None if args.no_splat else splat_prediction(args)
     fusion_res = None if args.no_fusion else surfel_fusion(args)
     loop_res = None if args.no_loop else frame_loop_sequence(args)
@@ -834,10 +1085,12 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "copy_peak": copy_peak,
         "drift_vs_truth": {"mean_mm": round(1e3 * drift_t, 3), "mean_deg": round(drift_r, 4)},
         "trajectory": trajectory_eval,
         "last_stats": stats,
         "shapes": shapes,
+        "c1_gpu": c1,
         "learned": learned,
         "batched": batched,
         "splat": splat_res,

@@ -25,6 +25,35 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the OpenMP row loops (the CPU baseline's 1-core and all-core
+ * runs), and the host's online processors. */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int orc_hardware_concurrency(void)
+{
+    return (int)sysconf(_SC_NPROCESSORS_ONLN);
+}
 
 static const float ORC_NAN = NAN;
 

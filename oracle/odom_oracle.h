@@ -41,6 +41,10 @@ orc_intr orc_level_intr(orc_intr k, int level);
  * Cody-Waite reduction by ln2 and a degree-7 Taylor polynomial.  Returns 0
  * for x < -87. */
 float orc_bexpf(float x);
+/* threads of the oracle's OpenMP row loops; the host's online processors */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+int orc_hardware_concurrency(void);
 
 /* a1: depth_bilateral.frag:30-76 */
 void orc_bilateral_u16(const uint16_t *src, uint16_t *dst, int w, int h, float maxD);

@@ -51,6 +51,9 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_level_intr.restype = Intr
         L.orc_level_intr.argtypes = [Intr, ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        L.orc_hardware_concurrency.restype = ctypes.c_int
         L.orc_bexpf.restype = ctypes.c_float
         L.orc_bexpf.argtypes = [ctypes.c_float]
         L.orc_create.restype = _V
@@ -308,6 +311,20 @@ def rodrigues(r):
     R = np.zeros(9, np.float64)
     lib().orc_rodrigues(_p(r), _p(R))
     return R.reshape(3, 3)
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP row loops."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
+
+def hardware_concurrency() -> int:
+    """The host's online processors (std::thread::hardware_concurrency)."""
+    return int(lib().orc_hardware_concurrency())
 
 
 def bexpf(x: float) -> float:
