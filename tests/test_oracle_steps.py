@@ -371,3 +371,110 @@ def test_cross_call_state(orc):
     o.getIncrementalTransformation(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32), so3=True)
     assert np.array_equal(o.buf("lastNextImage", 0), nxt)
     assert np.array_equal(o.buf("nextImage", 0), orc.intensity(f0.rgb))
+
+
+def _so3_unpack(sums):
+    """JtJJtrSO3 packing (types.cuh:138-165): the upper triangle of the
+    products of [J0 J1 J2 r] row by row, then r^2 and the count."""
+    A = np.zeros((3, 3))
+    b = np.zeros(3)
+    s = 0
+    for i in range(3):
+        for j in range(i, 4):
+            if j == 3:
+                b[i] = sums[s]
+            else:
+                A[i, j] = A[j, i] = sums[s]
+            s += 1
+    return A, b
+
+
+def test_so3_step_jacobian_is_rotation_derivative(orc):
+    """so3Step's row J of a pixel p is minus the derivative of
+    g . pi(K exp([w]x) K^-1 p) at w = 0, with g the step's image gradient
+    (the mean of both images' central differences, written back minus fore).
+    A next image that is an integer ramp makes that gradient exact and the
+    same at every pixel; a constant last image gives r = I_last - I_next.
+    The sums must equal the sums of the finite-difference rows over the
+    interior pixels."""
+    k = kin(orc)
+    Kf = np.array([[k.fx, 0, k.cx], [0, k.fy, k.cy], [0, 0, 1]], np.float64)
+    Ki = np.linalg.inv(Kf)
+    yy, xx = np.mgrid[0:H, 0:W]
+    nxt = (20 + xx + 2 * yy).astype(np.uint8)
+    last = np.full((H, W), 90, np.uint8)
+    sums, mask = orc.so3_step(last, nxt, I3, Ki.astype(np.float32), Kf.astype(np.float32))
+    A, b = _so3_unpack(sums)
+    # the step's gradient: ((back + a) / 2 - (fore + a) / 2) of each image, averaged
+    g = 0.5 * np.array([(-1.0 - 0.0), (-2.0 - 0.0)])
+
+    def f(w, px):
+        X = synth.so3_exp(w) @ (Ki @ np.array([px[0], px[1], 1.0]))
+        u = Kf @ X
+        return g @ (u[:2] / u[2])
+
+    eps = 1e-6
+    A_fd = np.zeros((3, 3))
+    b_fd = np.zeros(3)
+    n = 0
+    for y in range(1, H - 1):
+        for x in range(1, W - 1):
+            J = np.zeros(3)
+            for q in range(3):
+                e = np.zeros(3)
+                e[q] = eps
+                J[q] = -(f(e, (x, y)) - f(-e, (x, y))) / (2 * eps)
+            r = -(float(nxt[y, x]) - 90.0)
+            A_fd += np.outer(J, J)
+            b_fd += J * r
+            n += 1
+    assert sums[10] == n and mask.sum() == n
+    # entries that cancel (|A02|, |A12| ~ 1e-8 of the diagonal) are FP32 noise
+    np.testing.assert_allclose(A, A_fd, rtol=1e-4, atol=1e-7 * np.abs(A_fd).max())
+    np.testing.assert_allclose(b, b_fd, rtol=1e-4, atol=1e-6 * np.abs(b_fd).max())
+    rr = [(90.0 - float(nxt[y, x])) ** 2 for y in range(1, H - 1) for x in range(1, W - 1)]
+    assert abs(sums[9] - sum(rr)) <= 1e-6 * sum(rr)
+
+
+def test_seeded_call_with_no_iterations_returns_the_seed(orc):
+    """orc_track_ex's seed path: with no Gauss-Newton iteration the call ends
+    at prev * seed^-1 (the resultRt convention) exactly, up to the float pose
+    arithmetic of the update."""
+    cam = synth.CAM_320
+    T0, T1, f0, f1 = make_pair(cam, 8)
+    o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
+    feed(o, f0, f1, T0, orc.bilateral(f1.depth, MAXD))
+    M = np.linalg.inv(T0) @ T1                 # the camera motion prev -> current
+    seed = np.linalg.inv(M)
+    t, R = o.track_ex(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32), iters=(0, 0, 0), seed=seed)
+    got = pose_of(t, R)
+    exp = T0.astype(np.float32).astype(np.float64) @ M
+    dt, dr = synth.pose_error(got, exp)
+    assert dt < 1e-6 and dr < 1e-4, (dt, dr)
+
+
+def test_seeded_call_from_the_true_motion_stays_on_it(orc):
+    """A seed equal to the rendered motion on a noise-free pair (raw depth,
+    ICP only) is already the optimum: the seeded refinement (FP64) ends
+    within 0.1 mm of the truth and where the plain 10/5/4 call ends.  (With
+    the photometric term the joint optimum of nearest-pixel warps sits about
+    a millimetre off the truth, from either start.)"""
+    cam = synth.CAM_320
+    scene = synth.Scene(12, 20)
+    rng = np.random.default_rng(12)
+    T0 = synth.base_pose()
+    T1 = synth.perturb(T0, rng, 0.03, 2.0)
+    f0 = scene.render(T0, cam, noise_mm=0.0, dropout=0.0, noise_seed=1)
+    f1 = scene.render(T1, cam, noise_mm=0.0, dropout=0.0, noise_seed=2)
+    seed = np.linalg.inv(np.linalg.inv(T0) @ T1)
+    out = []
+    for sd, it in ((seed, (5, 2, 2)), (None, (10, 5, 4))):
+        o = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy, fp64=True)
+        feed(o, f0, f1, T0, f1.depth)
+        t, R = o.track_ex(T0[:3, 3].astype(np.float32), T0[:3, :3].astype(np.float32), iters=it, icpWeight=100.0,
+                          seed=sd)
+        out.append(pose_of(t, R))
+    dt, dr = synth.pose_error(out[0], T1)
+    assert dt < 1e-4 and dr < 0.01, (dt, dr)
+    dt, dr = synth.pose_error(out[0], out[1])
+    assert dt < 1e-6 and dr < 1e-4, (dt, dr)
