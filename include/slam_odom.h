@@ -330,6 +330,13 @@ int odom_icp_step_noalloc(const float *vcur, const float *ncur, const float *vpr
                           const float tprev[3], float dist_thresh, float angle_thresh, double out[29],
                           void *stream);
 
+/* The measured HBM copy peak the benchmark reports beside the 8 TB/s
+ * specification (no counterpart in the reference): `reps` device copies of
+ * `bytes` (a multiple of 16, 16-byte aligned pointers) from src to dst on
+ * `stream` after one untimed copy; ms_per_copy gets the mean duration (HIP
+ * events).  Bandwidth = 2 * bytes / duration. */
+int odom_stream_copy(const void *src, void *dst, size_t bytes, int reps, float *ms_per_copy, void *stream);
+
 /* Device name / arch of the current device, for logs. */
 int odom_device_info(char *name, int name_len, char *arch, int arch_len);
 

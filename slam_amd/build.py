@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["odom_kernels.hip", "odom_host.cpp", "cnn_kernels.hip", "splat_kernels.hip", "surfel_kernels.hip",
-               "frame_loop.cpp", "noalloc.hip"]
+               "frame_loop.cpp", "noalloc.hip", "measure.hip"]
 HIP_HEADERS = ["odom_device.h", "odom_internal.h"]
 
 HIP_FLAGS = [

@@ -84,7 +84,7 @@ EXPORTS = [
     "odom_batch_profile_track", "odom_batch_set_split_solve", "odom_batch_autotune", "odom_batch_get_step_shape",
     "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
     "odom_loop_check", "odom_icp_step_noalloc", "odom_default_policy", "odom_track_policy",
-    "odom_batch_get_stream", "odom_batch_warmstart_input", "odom_track_batch_policy",
+    "odom_batch_get_stream", "odom_batch_warmstart_input", "odom_track_batch_policy", "odom_stream_copy",
 ]
 
 _V = ctypes.c_void_p
@@ -152,6 +152,7 @@ def lib() -> ctypes.CDLL:
     L.odom_icp_step_noalloc.argtypes = [_V, _V, _V, _V, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4 + \
         [_V, _V, _V, _V, ctypes.c_float, ctypes.c_float, _V, _V]
     L.odom_default_policy.argtypes = [_P(Policy)]
+    L.odom_stream_copy.argtypes = [_V, _V, ctypes.c_size_t, ctypes.c_int, _P(ctypes.c_float), _V]
     L.odom_track_policy.argtypes = [_V, _V, ctypes.c_float, ctypes.c_float, _P(Policy), ctypes.c_int, ctypes.c_float]
     L.odom_batch_get_stream.argtypes = [_V, _P(_V)]
     L.odom_batch_warmstart_input.argtypes = [_V, _V]

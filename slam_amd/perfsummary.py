@@ -14,6 +14,9 @@
         pyramid level, from FETCH_SIZE and WRITE_SIZE counter runs (KiB per
         dispatch), each scaled by what the counter reads for the 256 MiB
         device copies of ``bench.py --calibrate-copy``.
+    python -m slam_amd.perfsummary batch-traffic FETCH_DIR WRITE_DIR [--out JSON]
+        the same for the 64-pair batch's passes A (unfused) and B per level,
+        with the joint (A + B) traffic against the 62 B/px contract
     python -m slam_amd.perfsummary kernel-traffic FETCH_DIR WRITE_DIR --kernel NAME --algorithmic BYTES
         the same for every dispatch of the kernels whose name contains NAME:
         mean FETCH_SIZE and WRITE_SIZE, copy-scaled bytes, and their ratio to
@@ -204,6 +207,67 @@ def cmd_kernel_traffic(fetch_dir: str, write_dir: str, name: str, algorithmic: f
             "ratio": round((fb + wb) / algorithmic, 3)}
 
 
+# The 64-pair batch (16 at 320x240, 32 at 640x480, 16 at 848x480): pixels per
+# level over all pairs, and the schedule's order of levels within a call.
+BATCH_LEVEL_PX = (17_571_840, 4_392_960, 1_098_240)
+CALL_LEVELS = [2] * 4 + [1] * 5 + [0] * 10
+
+
+def _batch_levels(path: str, kernel: str):
+    """Counter values of the batched calls' dispatches of `kernel` (the
+    unfused pass A or pass B), by level: every batched call launches each
+    pass 19 times in the schedule's order (4 at level 2, 5 at 1, 10 at 0)."""
+    rows = list(csv.DictReader(open(path)))
+    copy = [float(r["Counter_Value"]) for r in rows
+            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    if "Dispatch_Id" in rows[0]:
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    if kernel == "a":
+        sel = [r for r in rows if "k_step_a" in r["Kernel_Name"] and "false>" in r["Kernel_Name"]]
+    else:
+        sel = [r for r in rows if "k_step_b" in r["Kernel_Name"]]
+    if not copy or not sel or len(sel) % len(CALL_LEVELS):
+        raise ValueError(f"{path}: {len(copy)} calibration copies, {len(sel)} dispatches of pass {kernel}")
+    lv = defaultdict(list)
+    for i, r in enumerate(sel):
+        lv[CALL_LEVELS[i % len(CALL_LEVELS)]].append(float(r["Counter_Value"]))
+    return sum(copy) / len(copy), {k: sum(v) / len(v) for k, v in lv.items()}, {k: len(v) for k, v in lv.items()}
+
+
+def cmd_batch_traffic(fetch_dir: str, write_dir: str) -> dict:
+    """HBM bytes per dispatch of the batched passes A and B per level, copy-scaled,
+    against the algorithmic bytes (A 58 B/px: the ICP maps and the photometric
+    association reads; B 4 B/px: the gradients; joint 62 B/px)."""
+    out = {"workload": "64 pairs (16 x 320x240, 32 x 640x480, 16 x 848x480), batched ICP+RGB+SO3 calls",
+           "counter_unit": "KiB per dispatch", "levels": {}}
+    per = {}
+    for k, bpp in (("a", 58), ("b", 4)):
+        fc, fl, fn = _batch_levels(os.path.join(fetch_dir, "run_counter_collection.csv"), k)
+        wc, wl, wn = _batch_levels(os.path.join(write_dir, "run_counter_collection.csv"), k)
+        fs, ws = COPY_BYTES / (1024.0 * fc), COPY_BYTES / (1024.0 * wc)
+        out[f"calibration_{k}"] = {"fetch_kib": round(fc, 1), "write_kib": round(wc, 1),
+                                   "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
+        for l in sorted(fl):
+            fb, wb = fl[l] * 1024.0 * fs, wl[l] * 1024.0 * ws
+            alg = bpp * BATCH_LEVEL_PX[l]
+            per[(k, l)] = fb + wb
+            out["levels"].setdefault(str(l), {})[k] = {
+                "dispatches": fn[l], "fetch_kib": round(fl[l], 1), "write_kib": round(wl[l], 1),
+                "fetch_bytes": int(fb), "write_bytes": int(wb), "traffic_bytes": int(fb + wb),
+                "algorithmic_bytes": alg, "ratio": round((fb + wb) / alg, 3)}
+    for l in range(3):
+        if ("a", l) in per and ("b", l) in per:
+            alg = 62 * BATCH_LEVEL_PX[l]
+            out["levels"][str(l)]["joint"] = {"traffic_bytes": int(per[("a", l)] + per[("b", l)]),
+                                              "algorithmic_bytes": alg,
+                                              "ratio": round((per[("a", l)] + per[("b", l)]) / alg, 3)}
+    iters = (10, 5, 4)
+    out["a_traffic_bytes_per_launch_10_5_4"] = int(sum(iters[l] * per[("a", l)] for l in range(3)) / sum(iters))
+    out["joint_ratio_10_5_4"] = round(sum(iters[l] * (per[("a", l)] + per[("b", l)]) for l in range(3)) /
+                                      sum(iters[l] * 62 * BATCH_LEVEL_PX[l] for l in range(3)), 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -227,6 +291,10 @@ def main() -> int:
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
     t.add_argument("--out", default=None)
+    bt = sub.add_parser("batch-traffic")
+    bt.add_argument("fetch_dir")
+    bt.add_argument("write_dir")
+    bt.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.cmd == "stats":
         print(cmd_stats(a.dir, a.top, a.command))
@@ -237,7 +305,8 @@ def main() -> int:
     elif a.cmd == "kernel-traffic":
         print(json.dumps(cmd_kernel_traffic(a.fetch_dir, a.write_dir, a.kernel, a.algorithmic), indent=1))
     else:
-        res = cmd_traffic(a.fetch_dir, a.write_dir)
+        res = cmd_traffic(a.fetch_dir, a.write_dir) if a.cmd == "traffic" else \
+            cmd_batch_traffic(a.fetch_dir, a.write_dir)
         text = json.dumps(res, indent=1)
         if a.out:
             with open(a.out, "w") as fh:
