@@ -183,7 +183,7 @@ class Forms:
             st(L.odom_track_policy(h, None, 1.0, 1.0, by(self.pol_cv), 0, ICP_WEIGHT))
         elif form in ("learned", "learned_full"):
             st(L.odom_init_warmstart_input(h, self.x.data_ptr()))
-            st(ws._lib().cnn_forward(self.net.handle, self.x.data_ptr(), 1, self.pred.data_ptr()))
+            st(ws._lib().cnn_infer(self.net.handle, self.x.data_ptr(), 1, self.pred.data_ptr()))
             pol = self.pol_learned if form == "learned" else self.pol_learned_full
             st(L.odom_track_policy(h, self.pred.data_ptr(), ws.T_SCALE, ws.R_SCALE, by(pol), 0, ICP_WEIGHT))
         else:
@@ -378,14 +378,42 @@ def c1_gpu(args) -> dict:
             "frames_per_s": round(n / el, 2), "ms_per_frame": round(1e3 * el / n, 4), "frames": n}
 
 
+def network_timing(reps: int = 50) -> dict:
+    """Microseconds per call of the warm-start network: the fused one-launch
+    inference (cnn_infer) and the training forward (cnn_forward, one GEMM
+    launch per layer plus im2col / split-K passes), at batch 1 and 8, timed
+    with HIP events over `reps` back-to-back calls."""
+    import torch
+    from slam_amd import warmstart as ws
+    net = ws.WarmStartNet(max_batch=8)
+    net.load()
+    out = {}
+    for n in (1, 8):
+        x = torch.rand((n, 60, 80, 4), device="cuda").to(torch.bfloat16).contiguous()
+        y = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+        for name, fn in (("infer", ws._lib().cnn_infer), ("forward", ws._lib().cnn_forward)):
+            for _ in range(3):
+                fn(net.handle, x.data_ptr(), n, y.data_ptr())
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                fn(net.handle, x.data_ptr(), n, y.data_ptr())
+            e1.record()
+            e1.synchronize()
+            out[f"{name}_batch{n}"] = round(1e3 * e0.elapsed_time(e1) / reps, 2)
+    net.close()
+    return out
+
+
 FORMS = ("reference", "reference_conv", "plain_policy", "learned", "learned_full")
 FORM_DOC = {
     "reference": "previous-pose start, SO3 pre-alignment, fixed 10/5/4 (getIncrementalTransformation)",
     "reference_conv": "previous-pose start, SO3, 10/5/4 with the convergence test",
     "plain_policy": "constant-velocity start, no SO3, 10/5/4 with the convergence test (the learned form "
                     "without the network)",
-    "learned": "network start (bf16 MFMA, batch 1) judged against the constant-velocity start after the "
-               "level-2 refinement of both, convergence test",
+    "learned": "network start (bf16 MFMA, batch 1, one launch) judged against the constant-velocity start "
+               "after the level-2 refinement of both, convergence test",
     "learned_full": "as learned, both starts refined through every level before the judge",
 }
 
@@ -407,7 +435,9 @@ def learned_vs_plain(args, seq_c2: DeviceSequence, L, odo, ref64) -> dict | None
     net.set_stream(odo.stream)
     forms = Forms(L, odo, net)
     nsteps, warm = args.steps, args.warmup
-    res = {"forms": FORM_DOC, "convergence_test": {"dt_m": CONV_T, "dw_rad": CONV_R}, "judge_tol": 0.01}
+    res = {"forms": FORM_DOC, "convergence_test": {"dt_m": CONV_T, "dw_rad": CONV_R}, "judge_tol": 0.01,
+           "network_us": network_timing()}
+    log(f"learned: network {res['network_us']}")
 
     def public(r):
         return {k: v for k, v in r.items() if k not in ("poses", "_kept", "_preds")}

@@ -536,6 +536,157 @@ int splits_for(int K, int MN)
 
 }  // namespace
 
+// ============================================================= fused inference
+// The whole forward pass of one sample in one workgroup: the activations stay
+// in LDS (at most 76.8 KB + 38.4 KB, the first layer's output and input), every
+// convolution is an implicit-GEMM on MFMA (the im2col operand is gathered
+// from the LDS activations as it is consumed, 4 consecutive channels of one
+// input pixel per lane), the weights are read from the bf16 copy, and the
+// epilogue (bias, ReLU, bf16) writes the next layer's input.  The rounding
+// points are those of cnn_forward (bf16 activations, fp32 accumulation), and a
+// sample's result does not depend on the other samples of the launch.  One
+// launch for the whole network; no activations are kept.
+constexpr int kInferThreads = 1024;
+constexpr int kInferBufA = 30 * 40 * 32;     // elements: the largest output (layer 1)
+constexpr int kInferBufB = 60 * 80 * 4;      // the input; also layer 2's output (15 x 20 x 64)
+
+struct InferParams {
+    const bf16* W[kConvs];                   // [Kp][Cout] bf16
+    const float* b[kConvs];
+    int Kp[kConvs];
+    const bf16* fc1W;                        // [128][256]
+    const float* fc1b;
+    const bf16* fc2W;                        // [256][8]
+    const float* fc2b;
+};
+
+// One convolution layer (3x3, stride 2, pad 1) from X (LDS, [HI][WI][CIN])
+// into Y (LDS, [HO][WO][COUT]).  Each wave takes (column tile, up to four row
+// tiles) work items.  The weight fragments of kGroup consecutive 16-deep k
+// steps are loaded together before their MFMAs, so an item waits for one
+// memory latency per group rather than per step; each fragment serves all the
+// item's row tiles.
+constexpr int kGroup = 8;
+
+template <int CIN, int COUT, int HI, int WI, int HO, int WO>
+__device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __restrict__ Y, const bf16* __restrict__ W,
+                                           const float* __restrict__ bias)
+{
+    constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16, CH = 4;
+    constexpr int KREAL = 9 * CIN, KP = ((KREAL + BK - 1) / BK) * BK, KS = KP / 16;
+    static_assert(COUT % 16 == 0 && CIN % 4 == 0 && KP % 16 == 0, "tile shapes");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int WAVES = kInferThreads / 64;
+    constexpr int CHUNKS = (MT + CH - 1) / CH;
+    const int rq = 4 * (lane >> 4);           // this lane's k offset within a 16-deep step
+    for (int item = wave; item < NT * CHUNKS; item += WAVES) {
+        const int nt = item % NT, ch = item / NT;
+        const int n = nt * 16 + (lane & 15);
+        // the rows of this lane's A fragments (one per row tile of the chunk)
+        int hob[CH], wob[CH];
+        bool mok[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int m = (ch * CH + i) * 16 + (lane & 15);
+            mok[i] = (ch * CH + i) < MT && m < M;
+            hob[i] = 2 * (m / WO) - 1;
+            wob[i] = 2 * (m % WO) - 1;
+        }
+        v4f acc[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int g0 = 0; g0 < KS; g0 += kGroup) {
+            v4s b[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int kq = (g0 + g) * 16 + rq;
+                if (g0 + g < KS) {
+                    bf16 t4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) t4[q] = W[(size_t)(kq + q) * COUT + n];
+                    b[g] = v4s{(short)t4[0], (short)t4[1], (short)t4[2], (short)t4[3]};
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (g0 + g >= KS) break;
+                const int kq = (g0 + g) * 16 + rq;
+                const int c = kq % CIN, tap = kq / CIN;
+                const int kh = tap / 3, kw = tap - 3 * kh;
+#pragma unroll
+                for (int i = 0; i < CH; ++i) {
+                    v4s a = v4s{0, 0, 0, 0};
+                    const int hi = hob[i] + kh, wi = wob[i] + kw;
+                    if (mok[i] && kq < KREAL && hi >= 0 && hi < HI && wi >= 0 && wi < WI)
+                        a = *(const v4s*)&X[(hi * WI + wi) * CIN + c];
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[g], acc[i], 0, 0, 0);
+                }
+            }
+        }
+        const float bn = bias[n];
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = (ch * CH + i) * 16 + rq + r;
+                if ((ch * CH + i) >= MT || m >= M) continue;
+                float v = acc[i][r] + bn;
+                v = fmaxf(v, 0.0f);
+                Y[m * COUT + n] = f2bf(v);
+            }
+    }
+}
+
+__global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restrict__ x, InferParams P,
+                                                             float* __restrict__ out)
+{
+    __shared__ __align__(16) bf16 bufA[kInferBufA];
+    __shared__ __align__(16) bf16 bufB[kInferBufB];
+    __shared__ float s_h1[kFc1Out];
+    __shared__ float s_gap[kFc1In];
+    const int t = threadIdx.x;
+    // the input, 16 bytes per thread and step
+    const uint4* src = (const uint4*)(x + (size_t)blockIdx.x * kInferBufB);
+    for (int i = t; i < kInferBufB / 8; i += kInferThreads) ((uint4*)bufB)[i] = src[i];
+    __syncthreads();
+    infer_conv<4, 32, 60, 80, 30, 40>(bufB, bufA, P.W[0], P.b[0]);
+    __syncthreads();
+    infer_conv<32, 64, 30, 40, 15, 20>(bufA, bufB, P.W[1], P.b[1]);
+    __syncthreads();
+    infer_conv<64, 96, 15, 20, 8, 10>(bufB, bufA, P.W[2], P.b[2]);
+    __syncthreads();
+    infer_conv<96, 128, 8, 10, 4, 5>(bufA, bufB, P.W[3], P.b[3]);
+    __syncthreads();
+    infer_conv<128, 128, 4, 5, 2, 3>(bufB, bufA, P.W[4], P.b[4]);
+    __syncthreads();
+    infer_conv<128, 128, 2, 3, 1, 2>(bufA, bufB, P.W[5], P.b[5]);
+    __syncthreads();
+    // global average pooling over the 1 x 2 output (k_gap_fwd's arithmetic)
+    if (t < kFc1In) {
+        const float sgap = bf2f(bufB[t]) + bf2f(bufB[kFc1In + t]);
+        s_gap[t] = bf2f(f2bf(sgap / 2.0f));
+    }
+    __syncthreads();
+    // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands
+    if (t < kFc1Out) {
+        float acc = 0.0f;
+        for (int k = 0; k < kFc1In; ++k) acc += s_gap[k] * bf2f(P.fc1W[(size_t)k * kFc1Out + t]);
+        float v = fmaxf(acc + P.fc1b[t], 0.0f);
+        s_h1[t] = bf2f(f2bf(v));
+    }
+    __syncthreads();
+    if (t < 64 * kOut) {
+        // one wave per output: lanes split the 256-term dot product, then a
+        // fixed-order wave sum
+        const int j = t >> 6, lane = t & 63;
+        float acc = 0.0f;
+        for (int k = lane; k < kFc1Out; k += 64) acc += s_h1[k] * bf2f(P.fc2W[(size_t)k * kOutPad + j]);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) out[(size_t)blockIdx.x * kOut + j] = acc + P.fc2b[j];
+    }
+}
+
 extern "C" {
 
 int cnn_create(int max_batch, cnn_net* out)
@@ -723,6 +874,26 @@ int cnn_forward(cnn_net c, const void* x, int N, float* out)
         return rc;
     if (out) CK(hipMemcpy2DAsync(out, kOut * 4, c->out, kOutPad * 4, kOut * 4, N, hipMemcpyDeviceToDevice, c->stream));
     return launch_check("forward");
+}
+
+// Inference of N samples in one launch (k_cnn_infer): out (device, fp32
+// [N][6]).  Keeps no activations.
+int cnn_infer(cnn_net c, const void* x, int N, float* out)
+{
+    if (!c || !x || !out || N < 1 || N > c->max_batch) return CNN_ERR_ARG;
+    if ((uintptr_t)x % 16 != 0) return CNN_ERR_ARG;
+    InferParams P;
+    for (int l = 0; l < kConvs; ++l) {
+        P.W[l] = c->pb + c->convW[l].off;
+        P.b[l] = c->p + c->convB[l].off;
+        P.Kp[l] = c->Kp[l];
+    }
+    P.fc1W = c->pb + c->fc1W.off;
+    P.fc1b = c->p + c->fc1B.off;
+    P.fc2W = c->pb + c->fc2W.off;
+    P.fc2b = c->p + c->fc2B.off;
+    hipLaunchKernelGGL(k_cnn_infer, dim3(N), dim3(kInferThreads), 0, c->stream, (const bf16*)x, P, out);
+    return launch_check("infer");
 }
 
 // Gradients of the weighted squared error against target (device fp32

@@ -265,7 +265,7 @@ int efloop_get_state(efloop_handle c, int* tick, float* last_weighting, float* l
 // batch's stream.  Every sequence's launches are those of the same sequence
 // tracked alone (a batch of one): the batch gives each pair the block count
 // of a single-pair call and the handles' launch shapes, and the network's
-// forward pass is the same for a sample in any batch.
+// inference (one workgroup per sample) is the same for a sample in any batch.
 struct efloop_batch_s {
     std::vector<efloop_ctx*> loops;
     odom_batch batch = nullptr;
@@ -381,7 +381,7 @@ int efloop_batch_process(efloop_batch b, const odom_frame_view* rgb, const odom_
         const float* pred = nullptr;
         if (b->net) {
             rc = odom_batch_warmstart_input(b->batch, b->x);
-            if (!rc && cnn_forward(b->net, b->x, n, b->pred) != CNN_OK) rc = ODOM_ERR_HIP;
+            if (!rc && cnn_infer(b->net, b->x, n, b->pred) != CNN_OK) rc = ODOM_ERR_HIP;
             pred = b->pred;
         }
         if (!rc) rc = odom_track_batch_policy(b->batch, pred, b->t_scale, b->r_scale, &b->pol, 0,

@@ -34,7 +34,8 @@ _V = ctypes.c_void_p
 
 # Every symbol include/slam_cnn.h declares.
 EXPORTS = ["cnn_create", "cnn_destroy", "cnn_set_stream", "cnn_num_params", "cnn_param_layout", "cnn_set_params",
-           "cnn_get_params", "cnn_get_grads", "cnn_prepare_input", "cnn_forward", "cnn_backward", "cnn_adam_step"]
+           "cnn_get_params", "cnn_get_grads", "cnn_prepare_input", "cnn_forward", "cnn_backward", "cnn_adam_step",
+           "cnn_infer"]
 
 
 def _lib():
@@ -52,6 +53,7 @@ def _lib():
         L.cnn_get_grads.argtypes = [_V, _V, ctypes.c_size_t]
         L.cnn_prepare_input.argtypes = [_V, _V, _V, _V, _V, ctypes.c_int, ctypes.c_size_t, _V]
         L.cnn_forward.argtypes = [_V, _V, ctypes.c_int, _V]
+        L.cnn_infer.argtypes = [_V, _V, ctypes.c_int, _V]
         L.cnn_backward.argtypes = [_V, ctypes.c_int, _V, _V, _V]
         L.cnn_adam_step.argtypes = [_V, ctypes.c_float, ctypes.c_float]
         L._cnn_typed = True
@@ -139,6 +141,15 @@ class WarmStartNet:
         if out is None:
             out = torch.empty((n, 6), dtype=torch.float32, device=x.device)
         _check("cnn_forward", _lib().cnn_forward(self._h, x.data_ptr(), n, out.data_ptr()))
+        return out
+
+    def infer(self, x, out=None):
+        """One-launch inference (cnn_infer): no activations kept."""
+        import torch
+        n = x.shape[0]
+        if out is None:
+            out = torch.empty((n, 6), dtype=torch.float32, device=x.device)
+        _check("cnn_infer", _lib().cnn_infer(self._h, x.data_ptr(), n, out.data_ptr()))
         return out
 
     def backward(self, n: int, target, weights=(1.0,) * 6):

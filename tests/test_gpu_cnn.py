@@ -116,3 +116,34 @@ def test_forward_is_the_same_in_any_inference_batch():
     three = net.forward(x[2:5].contiguous()).cpu().numpy()
     assert np.array_equal(three, full[2:5])
     net.close()
+
+
+def test_fused_inference_matches_forward_and_oracle(setup):
+    """cnn_infer (one launch, activations in LDS, implicit-GEMM convolutions)
+    computes the network of cnn_forward with the same rounding points: the
+    two agree to far below the bf16 tolerance, and the fused result meets the
+    forward's bound against the rounded oracle."""
+    import torch
+    net, co, p, x, xt = setup["net"], setup["co"], setup["p"], setup["x"], setup["xt"]
+    fwd = net.forward(xt).cpu().numpy().astype(np.float64)
+    inf = net.infer(xt).cpu().numpy().astype(np.float64)
+    assert rel(inf, fwd) < 2e-3, rel(inf, fwd)
+    ref, _ = co.forward(p, x, rnd=co.bf16)
+    assert rel(inf, ref) < 1e-2, rel(inf, ref)
+    exact, _ = co.forward(p, x)
+    assert rel(inf, exact) < 4e-2 and cos(inf, exact) > 0.998
+
+
+def test_fused_inference_is_per_sample():
+    """Every sample of a launch runs in its own workgroup: batch 8 equals 8
+    launches of one, bit for bit."""
+    import torch
+    from slam_amd.warmstart import WarmStartNet
+    net = WarmStartNet(max_batch=8)
+    net.init_params(seed=11)
+    rng = np.random.default_rng(12)
+    x = torch.from_numpy(rng.uniform(0, 1, size=(8, 60, 80, 4)).astype(np.float32)).to(torch.bfloat16).cuda()
+    full = net.infer(x.contiguous()).cpu().numpy()
+    for i in range(8):
+        assert np.array_equal(net.infer(x[i:i + 1].contiguous()).cpu().numpy()[0], full[i]), i
+    net.close()
