@@ -170,3 +170,47 @@ def test_batch_rejects_bad_sets(pairs):
     with pytest.raises(OdomError) as e:
         b.track()
     assert e.value.code == -3
+
+
+def test_batch_of_64_pairs_against_the_fp64_oracle(orc):
+    """The benchmarked batch: 64 pairs (16 at 320x240, 32 at 640x480, 16 at
+    848x480), each a seeded room and a motion of up to 3 cm / 2 deg, tracked
+    by one batched ICP+RGB+SO3 call (10/5/4) with the library's batch launch
+    shapes, replayed as a hipGraph.  Every pair is compared with the all-FP64
+    oracle (1 mm / 0.1 deg) and with the ground truth; one pair of each size
+    with the same pair tracked alone (block split only: 2e-5 m / 2e-3 deg)."""
+    layout = ((synth.CAM_320, 16), (synth.CAM_640, 32), (synth.CAM_848, 16))
+    cams, frames = [], []
+    for cam, count in layout:
+        for _ in range(count):
+            seed = 500 + len(cams)
+            T0, T1, f0, f1 = make_pair(cam, seed, max_trans=0.03, max_rot_deg=2.0)
+            cams.append(cam)
+            frames.append((T0, T1, f0, f1, orc.bilateral(f1.depth, MAXD)))
+    odos = [mk(c) for c in cams]
+    for o, (T0, T1, f0, f1, filt) in zip(odos, frames):
+        feed(o, f0, f1, T0, filt)
+        o.set_pose(T0[:3, 3], T0[:3, :3])
+    b = Batch(odos)
+    b.set_graph(True)
+    b.track(False, 10.0, True, False, True)
+    b.synchronize()
+    got = poses_of(odos)
+    worst = [0.0, 0.0]
+    for j, (c, (T0, T1, f0, f1, filt)) in enumerate(zip(cams, frames)):
+        o = orc.Odometry(c.width, c.height, c.cx, c.cy, c.fx, c.fy, fp64=True)
+        feed(o, f0, f1, T0, filt)
+        t, R = o.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], False, 10.0, True, False, True)
+        dt, dr = synth.pose_error(got[j], pose_of(t, R))
+        worst = [max(worst[0], dt), max(worst[1], dr)]
+        assert dt <= 1e-3 and dr <= 0.1, (j, c.width, dt, dr)
+        st = odos[j].stats()
+        assert st["gnIterations"] == 19 and not st["degenerate"], (j, st)
+        assert synth.pose_error(got[j], T1)[0] < 0.01, j
+    print(f"64 pairs against the FP64 oracle: worst {1e3 * worst[0]:.4f} mm, {worst[1]:.5f} deg", flush=True)
+    for j in (0, 16, 48):
+        T0, T1, f0, f1, filt = frames[j]
+        alone, _ = track_alone(dict(cam=cams[j], T0=T0, f0=f0, f1=f1, filt=filt), "joint")
+        dt, dr = synth.pose_error(got[j], alone)
+        assert dt <= 2e-5 and dr <= 2e-3, (j, dt, dr)
+    b.close()
