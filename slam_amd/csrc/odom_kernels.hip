@@ -1238,29 +1238,46 @@ struct PartLoad {
     static constexpr int RG = NT / 8;           // record groups
     static constexpr int IT = kMaxBlk / RG;     // records per thread and pass
     float4 v[2][IT];
+    int n[2];                                   // records of each pass (0: not summed)
 };
 
+// Both passes' loads back to back and without branches, so that one wait
+// covers them: records past a pass's count re-read record 0, and a pass that
+// is off reads its (allocated) buffer; neither is summed.
 template <int NT>
 __device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int na, const float* __restrict__ pb,
                                                int nb, PartLoad<NT>& L)
 {
     constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT;
     const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
+    L.n[0] = na;
+    L.n[1] = nb;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         const float* part = a ? pb : pa;
-        const int n = a ? nb : na;
-        if (part) {
+        const int n = L.n[a];
 #pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const int r = rg + i * RG;
-                L.v[a][i] = *gp((const float4*)(part + (size_t)(r < n ? r : 0) * kPartStride + 4 * c));
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < IT; ++i) L.v[a][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < IT; ++i) {
+            const int r = rg + i * RG;
+            L.v[a][i] = *gp((const float4*)(part + (size_t)(r < n ? r : 0) * kPartStride + 4 * c));
         }
     }
+}
+
+// Marks the point from which the partials may be waited for: the empty asm
+// statements keep the compiler from consuming them (and so waiting for the
+// whole load queue) before the loads issued after them.
+template <int NT>
+__device__ __forceinline__ void partials_fence(PartLoad<NT>& L)
+{
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int i = 0; i < PartLoad<NT>::IT; ++i) {
+            float x = L.v[a][i].x, y = L.v[a][i].y, z = L.v[a][i].z, w = L.v[a][i].w;
+            asm volatile("" : "+v"(x), "+v"(y), "+v"(z), "+v"(w));
+            L.v[a][i] = make_float4(x, y, z, w);
+        }
 }
 
 // FP64 totals of the loaded partials in a fixed order: per thread over its
@@ -1271,15 +1288,15 @@ __device__ __forceinline__ void partials_issue(const float* __restrict__ pa, int
 // the system and statistics it stores), so one block barrier suffices.
 // tot[0..28] pass A, tot[29..57] pass B.
 template <int NT>
-__device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, int na, int nb, double* tot, double* tot1,
-                                             double (*red)[64], double (*fin)[64])
+__device__ __forceinline__ void partials_sum(const PartLoad<NT>& L, double* tot, double* tot1, double (*red)[64],
+                                             double (*fin)[64])
 {
     constexpr int RG = PartLoad<NT>::RG, IT = PartLoad<NT>::IT, G = NT / 64, SL = RG / G;
     static_assert(SL * 8 == 64, "a wave's lanes store exactly its slice of record groups");
     const int c = threadIdx.x & 7, rg = threadIdx.x >> 3;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-        const int n = a ? nb : na;
+        const int n = L.n[a];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
@@ -1344,46 +1361,80 @@ __device__ __forceinline__ void enter_iteration(IterState& S, const LevelDesc& L
     S.rgb_err_cmp = (iter == 0) ? FLT_MAX : S.lastRGBError;
 }
 
-// Finish iteration k-1 and leave the state entering iteration k in W.S: one
-// fused FP64 reduction of its ICP partials (pass A) and photometric partials
-// (pass B), the ICP statistics, the photometric bookkeeping when there is no
-// pass B, and the solve.  Executed by every thread of the block; every block
-// computes the same state.
+// The prologue of an iteration: finish iteration k-1 and leave the state
+// entering iteration k in W.S -- one fused FP64 reduction of its ICP partials
+// (pass A) and photometric partials (pass B), the ICP statistics, the
+// photometric bookkeeping when there is no pass B, the solve and, given the
+// level of iteration k, its warp constants.  Every block computes the same
+// state.  Split in two so that a kernel can issue its own loads in between:
+// prologue_issue loads the state record (one word per thread) and the
+// partials; prologue_finish waits for them only after those loads.
 template <int NT>
-__device__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level, const PrevPose& pp,
-                                Prolog<NT>& W)
+struct PrologIn {
+    unsigned int sw;                 // this thread's word of the state record
+    PartLoad<NT> PL;
+};
+
+template <int NT>
+__device__ __forceinline__ void prologue_issue(const PairDesc& D, const TrackParams& P, int k, int prev_level,
+                                               PrologIn<NT>& I)
 {
-    TrackState* st = D.st;
+    constexpr int NW = (int)(sizeof(IterState) / 4);
+    static_assert(NW <= NT, "one word of the state record per thread");
+    // k == 0 starts from slot 1 (written by the call's set-up) and sums no
+    // partials; its loads still go to allocated buffers, without branches
+    const int slot = (k == 0) ? 1 : ((k - 1) & 1);
+    const int plv = (k > 0) ? prev_level : 0;
+    const GLOBAL_AS unsigned int* src = gp((const unsigned int*)&D.st->it[slot]);
+    I.sw = src[(int)threadIdx.x < NW ? threadIdx.x : 0];
+    partials_issue<NT>(D.part_a[slot], (k > 0 && P.icp) ? D.nblk_a[plv] : 0, D.part_b[slot],
+                       (k > 0 && P.rgb) ? D.nblk_b[plv] : 0, I.PL);
+}
+
+// next: the level of iteration k (its warp constants are set), or nullptr.
+template <int NT>
+__device__ void prologue_finish(const PairDesc& D, const TrackParams& P, int k, int prev_level, const PrevPose& pp,
+                                Prolog<NT>& W, PrologIn<NT>& I, const LevelDesc* next, int level, int iter)
+{
+    constexpr int NW = (int)(sizeof(IterState) / 4);
+    asm volatile("" : "+v"(I.sw));
+    partials_fence<NT>(I.PL);
+    if ((int)threadIdx.x < NW) ((unsigned int*)&W.S)[threadIdx.x] = I.sw;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (k == 0) {
-        copy_in<NT>(&W.S, &st->it[1]);
+        if (next && threadIdx.x == 0) enter_iteration(W.S, *next, level, iter);
         __syncthreads();
         return;
     }
-    const int pb = (k - 1) & 1;
-    const int na = D.nblk_a[prev_level], nb = D.nblk_b[prev_level];
-    PartLoad<NT> PL;
-    copy_in<NT>(&W.S, &st->it[pb]);
-    partials_issue<NT>(P.icp ? D.part_a[pb] : nullptr, na, P.rgb ? D.part_b[pb] : nullptr, nb, PL);
-    __syncthreads();
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
-    // wave 0 solves; wave 1 meanwhile stores the system, the ICP statistics
-    // and the photometric bookkeeping (disjoint fields of the state)
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (run) {
-        partials_sum<NT>(PL, na, nb, W.tot, W.tot1, W.red, W.fin2);
-        if (wave == 0) {
-            solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t, prev_level);
-        } else if (wave == 1) {
-            store_system_stats(W.S, P, W.tot1, W.tot1 + kSE3, lane);
-            if (lane == 0 && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
+    if (run) partials_sum<NT>(I.PL, W.tot, W.tot1, W.red, W.fin2);
+    // wave 0 solves and then, in its first lane, does the bookkeeping and the
+    // next iteration's constants; wave 1 meanwhile stores the system and the
+    // ICP statistics (disjoint fields of the state)
+    if (wave == 0) {
+        if (run) solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t, prev_level);
+        if (lane == 0) {
+            if (run && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
+            if (W.S.broke) W.S.break_level = prev_level;
+            W.S.pending = 0;
+            W.S.broke = 0;
+            if (next) enter_iteration(W.S, *next, level, iter);
         }
-    }
-    if (wave == 1 && lane == 0) {
-        if (W.S.broke) W.S.break_level = prev_level;
-        W.S.pending = 0;
-        W.S.broke = 0;
+    } else if (wave == 1 && run) {
+        store_system_stats(W.S, P, W.tot1, W.tot1 + kSE3, lane);
     }
     __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level,
+                                                const PrevPose& pp, Prolog<NT>& W, const LevelDesc* next, int level,
+                                                int iter)
+{
+    PrologIn<NT> I;
+    prologue_issue<NT>(D, P, k, prev_level, I);
+    prologue_finish<NT>(D, P, k, prev_level, pp, W, I, next, level, iter);
 }
 
 // ======================================================== step kernel A
@@ -1403,19 +1454,23 @@ template <int NT, int PPT>
 __device__ __forceinline__ void load_tile_a(const LevelDesc& L, const TrackParams& P, int base, int N, size_t P3,
                                             TileA<NT, PPT>& T)
 {
+    // Without branches, so that the prologue's waits (issued after these
+    // loads) can count them exactly: a pixel past the end reads the last
+    // pixel, a pass that is off reads valid memory (the vertex map, present
+    // in every pair), and both are masked after the loads.
+    const float* zsrc = P.rgb ? L.zsrc : L.vcur;
+    const uint8_t* inext = P.rgb ? L.inext : (const uint8_t*)L.vcur;
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = base + q * NT;
-        T.zs[q] = QNAN;
-        T.in[q] = 0;
-        T.src[q].v[0] = QNAN;
-        if (idx < N) {
-            if (P.icp) icp_load(L, idx, P3, T.src[q]);
-            if (P.rgb) {
-                T.zs[q] = gp(L.zsrc)[idx];
-                T.in[q] = gp(L.inext)[idx];
-            }
-        }
+        const bool in_range = idx < N;
+        const int i = in_range ? idx : N - 1;
+        icp_load(L, i, P3, T.src[q]);
+        const float z = gp(zsrc)[i];
+        const uint8_t in = gp(inext)[i];
+        if (!(in_range && P.icp)) T.src[q].v[0] = QNAN;
+        T.zs[q] = (in_range && P.rgb) ? z : QNAN;
+        T.in[q] = (in_range && P.rgb) ? in : (uint8_t)0;
     }
 }
 
@@ -1497,16 +1552,17 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     const size_t P3 = (size_t)N;
     const int ntiles = (N + NT * PPT - 1) / (NT * PPT);
 
-    // pose-independent loads of the first tile overlap the prologue
-    TileA<NT, PPT> T;
-    load_tile_a<NT, PPT>(L, P, lb * NT * PPT + threadIdx.x, N, P3, T);
+    // the prologue's loads first, then the pose-independent loads of the
+    // first tile, which stay in flight while the prologue sums and solves
+    PrologIn<NT> PI;
+    if constexpr (FUSED) prologue_issue<NT>(D, P, k, prev_level, PI);
     PrevPose pp;
     load_prev(st, pp);
+    TileA<NT, PPT> T;
+    load_tile_a<NT, PPT>(L, P, lb * NT * PPT + threadIdx.x, N, P3, T);
 
     if constexpr (FUSED) {
-        finish_previous<NT>(D, P, k, prev_level, pp, W);
-        if (threadIdx.x == 0) enter_iteration(W.S, L, level, iter);
-        __syncthreads();
+        prologue_finish<NT>(D, P, k, prev_level, pp, W, PI, &D.lv[level], level, iter);
         if (lb == 0) copy_out<NT>(&st->it[k & 1], &W.S);
     } else {
         copy_in<NT>(&W.S, &st->it[k & 1]);
@@ -1560,9 +1616,7 @@ __global__ __launch_bounds__(NT) void k_solve(const PairDesc* __restrict__ pairs
     TrackState* st = D.st;
     PrevPose pp;
     load_prev(st, pp);
-    finish_previous<NT>(D, P, k, prev_level, pp, W);
-    if (threadIdx.x == 0) enter_iteration(W.S, D.lv[level], level, iter);
-    __syncthreads();
+    finish_previous<NT>(D, P, k, prev_level, pp, W, &D.lv[level], level, iter);
     copy_out<NT>(&st->it[k & 1], &W.S);
 }
 
@@ -1582,15 +1636,16 @@ __device__ __forceinline__ void load_tile_b(const LevelDesc& L, int base, int N,
 {
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
+        // without branches (see load_tile_a): past the end, the last pixel, masked
         const int idx = base + q * NT;
-        T.zs[q] = QNAN;
-        T.in[q] = 0;
-        T.gr[q] = make_short2(0, 0);
-        if (idx < N) {
-            T.zs[q] = gp(L.zsrc)[idx];
-            T.in[q] = gp(L.inext)[idx];
-            T.gr[q] = gp(L.dxy)[idx];
-        }
+        const bool in_range = idx < N;
+        const int i = in_range ? idx : N - 1;
+        const float z = gp(L.zsrc)[i];
+        const uint8_t in = gp(L.inext)[i];
+        const short2 g = gp(L.dxy)[i];
+        T.zs[q] = in_range ? z : QNAN;
+        T.in[q] = in_range ? in : (uint8_t)0;
+        T.gr[q] = in_range ? g : make_short2(0, 0);
     }
 }
 
@@ -1612,16 +1667,11 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
     const GLOBAL_AS IterState& Sg = *gp(&st->it[mode == 0 ? pb : 0]);
     static_assert(NT >= kMaxBlk, "one count partial per thread");
 
-    // Every load of the prologue is issued before its first wait: the first
-    // tile, pass A's count partials and the state fields of the iteration.
-    TileB<NT, PPT> T;
-    load_tile_b<NT, PPT>(L, lb * NT * PPT + threadIdx.x, N, T);
-    unsigned int c0 = 0, c1 = 0;
-    if (mode == 0 && (int)threadIdx.x < D.nblk_a[level]) {
-        const uint2 c = gp((const uint2*)D.part_ai[pb])[threadIdx.x];
-        c0 = c.x;
-        c1 = c.y;
-    }
+    // Every load of the prologue is issued before its first wait, without
+    // branches: pass A's count partials, the state fields of the iteration,
+    // then the first tile (which stays in flight while the counts are summed).
+    const int na = D.nblk_a[level];
+    const uint2 cp = gp((const uint2*)D.part_ai[pb])[(int)threadIdx.x < na ? threadIdx.x : 0];
     const int skip = Sg.skip;
     const float err_cmp = Sg.rgb_err_cmp;
     const float sigma_g = Sg.sigmaVal;
@@ -1629,6 +1679,11 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
     for (int q = 0; q < 9; ++q) rc.K[q] = uni(Sg.krkinv[q]);
     for (int q = 0; q < 3; ++q) rc.kt[q] = uni(Sg.kt[q]);
     rc.maxDepthDelta = P.maxDepthDelta;
+    TileB<NT, PPT> T;
+    load_tile_b<NT, PPT>(L, lb * NT * PPT + threadIdx.x, N, T);
+    unsigned int c0 = cp.x, c1 = cp.y;
+    asm volatile("" : "+v"(c0), "+v"(c1));   // consumed only after the tile loads are issued
+    if (!(mode == 0 && (int)threadIdx.x < na)) c0 = c1 = 0;
     if (mode == 0 && skip) return;
 
     float sigma;
@@ -2013,7 +2068,7 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     float* const plog = gp(st)->pose_log;
     const int plog_cap = gp(st)->pose_log_cap;
     const int frame = gp(st)->frame_count;
-    finish_previous<256>(D, P, K, last_level, pp, W);
+    finish_previous<256>(D, P, K, last_level, pp, W, nullptr, 0, 0);
     if (threadIdx.x != 0) return;
     GLOBAL_AS TrackState* g = gp(st);
     const IterState& S = W.S;
