@@ -50,6 +50,30 @@ __device__ __forceinline__ GLOBAL_AS T* gp(T* p)
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// Staging of an N-texel tile (row length ld, origin (ox, oy) in a w x h
+// image) into LDS with all of a thread's loads in flight together: the
+// coordinates are clamped into the image, so every address is valid, and the
+// stores follow the last load.  An entry whose texel lies outside the image
+// holds 0 (ZERO_OUT) or its clamped neighbour.
+template <int N, int NT, bool ZERO_OUT, class T, class Load>
+__device__ __forceinline__ void stage_tile(T* tile, int ld, int ox, int oy, int w, int h, Load load)
+{
+    constexpr int R = (N + NT - 1) / NT;
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = min((int)threadIdx.x + r * NT, N - 1);
+        const int gx = ox + i % ld, gy = oy + i / ld;
+        v[r] = load(min(max(gx, 0), w - 1), min(max(gy, 0), h - 1));
+        if (ZERO_OUT && (gx < 0 || gx >= w || gy < 0 || gy >= h)) v[r] = T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = (int)threadIdx.x + r * NT;
+        if (i < N) tile[i] = v[r];
+    }
+}
+
 float bexpf_host(float x) { return bexpf(x); }
 
 // =============================================================== a1 bilateral
@@ -91,10 +115,8 @@ __global__ __launch_bounds__(256) void k_bilateral(const uint16_t* __restrict__ 
     __shared__ float sEC[kBilatRange + 1];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kBT, y0 = blockIdx.y * kBTY;
-    for (int i = tid; i < kBLY * kBL; i += 256) {
-        const int gx = x0 - 6 + i % kBL, gy = y0 - 6 + i / kBL;
-        tile[i] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? src[(size_t)gy * pitch + gx] : (uint16_t)0;
-    }
+    stage_tile<kBLY * kBL, 256, true>(tile, kBL, x0 - 6, y0 - 6, w, h,
+                                      [&](int gx, int gy) { return gp(src)[(size_t)gy * pitch + gx]; });
     for (int i = tid; i <= kBilatRange; i += 256)
         sEC[i] = (i < kBilatRange) ? bexpf(-((float)(i * i) * 0.000555556f)) : 0.0f;
     __syncthreads();
@@ -514,6 +536,34 @@ __device__ __forceinline__ T pyr_l0(const PyrSrc& q, int w0, int x, int y)
     return gp((const T*)q.src0)[i];
 }
 
+// Level-0 texels of a tile in LDS (texels outside the image hold their
+// clamped neighbour; the windows clamp their coordinates, so never read them).
+// The source kind is uniform, so each kind stages with its own straight-line
+// loads.
+template <int N, class T>
+__device__ __forceinline__ void pyr_stage(const PyrSrc& q, T* tile, int ld, int ox, int oy, int w0, int h0)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (q.gen == 1) {
+            stage_tile<N, 256, false>(tile, ld, ox, oy, w0, h0, [&](int x, int y) {
+                const float z = gp(q.vz)[(size_t)y * w0 + x];
+                return (T)((z > q.cutoff || z <= 0) ? QNAN : z);
+            });
+            return;
+        }
+    } else if constexpr (sizeof(T) == 1) {
+        if (q.gen == 2) {
+            stage_tile<N, 256, false>(tile, ld, ox, oy, w0, h0, [&](int x, int y) {
+                const GLOBAL_AS uint8_t* px = gp(q.rgb) + (size_t)y * q.pitch + (size_t)x * q.ch;
+                return (T)(uint8_t)(int)((float)px[0] * 0.114f + (float)px[1] * 0.299f + (float)px[2] * 0.587f);
+            });
+            return;
+        }
+    }
+    stage_tile<N, 256, false>(tile, ld, ox, oy, w0, h0,
+                              [&](int x, int y) { return gp((const T*)q.src0)[(size_t)y * w0 + x]; });
+}
+
 // One output of level `lvl` from its source level `src` (sw x sh texels)
 // held in an LDS tile with origin (ox, oy) and row length `ld`.
 template <int KIND, class T>
@@ -550,10 +600,7 @@ __device__ void pyr_block(const PyrGrid& g, int q, int t, int lb, unsigned char*
         const int X0 = tx * kPT, Y0 = ty * kPT;
         const int x = X0 + (tid & (kPT - 1)), y = Y0 + tid / kPT;
         const int ox = 2 * X0 - 2, oy = 2 * Y0 - 2;
-        for (int i = tid; i < kT1 * kT1; i += 256) {
-            const int gx = ox + i % kT1, gy = oy + i / kT1;
-            if (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) t1[i] = pyr_l0<T>(P, w0, gx, gy);
-        }
+        pyr_stage<kT1 * kT1, T>(P, t1, kT1, ox, oy, w0, h0);
         __syncthreads();
         if (x < w1 && y < h1) gp((T*)P.dst[1])[(size_t)y * w1 + x] = pyr_at<KIND>(t1, kT1, ox, oy, w0, h0, x, y);
         return;
@@ -562,10 +609,7 @@ __device__ void pyr_block(const PyrGrid& g, int q, int t, int lb, unsigned char*
     T* t0 = t1 + kT21 * kT21;                    // level-0 texels
     const int o1x = 2 * X0 - 2, o1y = 2 * Y0 - 2;
     const int o0x = 2 * o1x - 2, o0y = 2 * o1y - 2;
-    for (int i = tid; i < kT20 * kT20; i += 256) {
-        const int gx = o0x + i % kT20, gy = o0y + i / kT20;
-        if (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) t0[i] = pyr_l0<T>(P, w0, gx, gy);
-    }
+    pyr_stage<kT20 * kT20, T>(P, t0, kT20, o0x, o0y, w0, h0);
     __syncthreads();
     for (int i = tid; i < kT21 * kT21; i += 256) {
         const int x1 = o1x + i % kT21, y1 = o1y + i / kT21;
@@ -1024,49 +1068,11 @@ __device__ __forceinline__ void acc_row7(float (&acc)[kSE3], const float* row)
     acc[28] += 1.0f;
 }
 
-// ===================================================== single-thread math
-// Per-iteration warp constants from resultRt (RGBDOdometryef.cpp:422-432):
-// Rt = resultRt^-1 taken as the rigid inverse [R^T | -R^T t].
-__device__ void iter_constants(IterState& S, const LevelDesc& L)
+// =================================================== solve and constants
+// The ICP statistics of an iteration (the residual[] of the ICP step and its
+// errors, RGBDOdometryef.cpp:509-510), by one wave while another solves.
+__device__ void store_icp_stats(IterState& S, const TrackParams& P, const double* icp_sums, int lane)
 {
-    const double* Rt = S.resultRt;
-    const double* K = L.K;
-    const double* Ki = L.Ki;
-    double Rinv[9], tinv[3], KR[9], KRKi[9];
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) Rinv[a * 3 + b] = Rt[b * 4 + a];
-    for (int a = 0; a < 3; ++a) tinv[a] = -(Rinv[a * 3] * Rt[3] + Rinv[a * 3 + 1] * Rt[7] + Rinv[a * 3 + 2] * Rt[11]);
-    mm3d(K, Rinv, KR);
-    mm3d(KR, Ki, KRKi);
-    for (int a = 0; a < 9; ++a) S.krkinv[a] = (float)KRKi[a];
-    for (int a = 0; a < 3; ++a) S.kt[a] = (float)(K[a * 3] * tinv[0] + K[a * 3 + 1] * tinv[1] + K[a * 3 + 2] * tinv[2]);
-}
-
-// RGBDOdometryef.cpp:533-575: combine, solve, computeUpdateSE3, pose update.
-// Evaluated redundantly by every lane of one wave from registers (no lane
-// exchanges: the factorisation is a chain of dependent FP64 operations that
-// cross-lane shuffles would only lengthen); lane 0 stores the results.
-// rgb_sums: FP64 totals of the photometric pass (unused without it).
-// The system of a solve (lastA, lastb: the entries solve_update_uniform
-// assembles, one per lane) and the ICP statistics of the iteration
-// (residual[] of the ICP step and its errors, RGBDOdometryef.cpp:509-510), by
-// one wave while another solves.
-__device__ void store_system_stats(IterState& S, const TrackParams& P, const double* icp_sums, const double* rgb_sums,
-                                   int lane)
-{
-    if (lane < 42) {
-        const int i = lane < 36 ? lane / 6 : lane - 36, j = lane < 36 ? lane % 6 : 6;
-        const int r = i < j ? i : j, c = i < j ? j : i;
-        const int s = 7 * r - r * (r - 1) / 2 + (c - r);   // packed upper triangle of [A | b]
-        const double ai = P.icp ? icp_sums[s] : 0.0;
-        const double ar = P.rgb ? rgb_sums[s] : 0.0;
-        const double wgt = P.icpWeight;
-        double v;
-        if (P.icp && P.rgb) v = (c == 6) ? ar + wgt * ai : ar + wgt * wgt * ai;
-        else v = P.icp ? ai : ar;
-        if (lane < 36) S.lastA[lane] = v;
-        else S.lastb[lane - 36] = v;
-    }
     if (P.icp && lane < kSE3) S.icp_sums[lane] = icp_sums[lane];
     if (lane == 0) {
         if (P.icp) {
@@ -1078,72 +1084,97 @@ __device__ void store_system_stats(IterState& S, const TrackParams& P, const dou
     }
 }
 
-// level: the level of the iteration being finished; with the convergence
-// policy (P.convT > 0) an update below both thresholds stops that level.
-__device__ void solve_update_uniform(IterState& S, const TrackParams& P, const double* icp_sums,
-                                     const double* rgb_sums, const float* Rprev, const float* tprev, int level)
+// Select of one of three uniform values by a lane-varying index (registers
+// cannot be indexed per lane).
+template <class T>
+__device__ __forceinline__ T pick3(int i, T a, T b, T c)
 {
-    const bool lead = (threadIdx.x & 63) == 0;
+    return i == 0 ? a : (i == 1 ? b : c);
+}
+
+// RGBDOdometryef.cpp:533-575 (combine, solve, computeUpdateSE3, pose update)
+// by one wave: the 42 entries of the system one per lane (stored as lastA /
+// lastb and read back), the factorisation and the increment on every lane
+// (a chain of dependent FP64 operations, from registers), then the new
+// resultRt entries, Rcurr and tcurr one per lane.  Each entry is formed by
+// the same expression on whichever lane computes it, so the lanes only remove
+// the serial chain of independent entries.  rgb_sums: FP64 totals of the
+// photometric pass (unused without it).  level: the level of the iteration
+// being finished; with the convergence policy (P.convT > 0) an update below
+// both thresholds stops that level.
+__device__ __forceinline__ void solve_update_wave(IterState& S, const TrackParams& P, const double* icp_sums,
+                                                  const double* rgb_sums, const float* Rprev, const float* tprev,
+                                                  int level)
+{
+    const int lane = threadIdx.x & 63;
+    const bool lead = lane == 0;
+    // the system (RGBDOdometryef.cpp:533-545), one entry per lane
+    if (lane < 42) {
+        const int i = lane < 36 ? lane / 6 : lane - 36, j = lane < 36 ? lane % 6 : 6;
+        const int r = i < j ? i : j, c = i < j ? j : i;
+        const int sidx = 7 * r - r * (r - 1) / 2 + (c - r);
+        const double ai = P.icp ? icp_sums[sidx] : 0.0;
+        const double ar = P.rgb ? rgb_sums[sidx] : 0.0;
+        const double wgt = P.icpWeight;
+        double v;
+        if (P.icp && P.rgb) v = (c == 6) ? ar + wgt * ai : ar + wgt * wgt * ai;
+        else v = P.icp ? ai : ar;
+        if (lane < 36) S.lastA[lane] = v;
+        else S.lastb[lane - 36] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
     double A[36], b[6], x[6];
-    const double wgt = P.icpWeight;
-    int s = 0;
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
+    for (int q = 0; q < 36; ++q) A[q] = S.lastA[q];
 #pragma unroll
-        for (int j = i; j < 7; ++j, ++s) {
-            const double ai = P.icp ? icp_sums[s] : 0.0;
-            const double ar = P.rgb ? rgb_sums[s] : 0.0;
-            double v;
-            if (P.icp && P.rgb) v = (j == 6) ? ar + wgt * ai : ar + wgt * wgt * ai;
-            else v = P.icp ? ai : ar;
-            if (j == 6) {
-                b[i] = v;
-            } else {
-                A[i * 6 + j] = v;
-                A[j * 6 + i] = v;
-            }
-        }
+    for (int q = 0; q < 6; ++q) b[q] = S.lastb[q];
     int zeros = 0;
     if (!ldlt_spd<6>(A, b, x)) zeros = ldlt_solve<6>(A, b, x);
     double rot[9];
     rodrigues(x + 3, rot);
-    double old[16], nrt[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) old[q] = S.resultRt[q];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (r == 3) {
-                nrt[r * 4 + c] = old[12 + c];
-            } else {
-                nrt[r * 4 + c] = rot[r * 3] * old[c] + rot[r * 3 + 1] * old[4 + c] + rot[r * 3 + 2] * old[8 + c] +
-                                 x[r] * old[12 + c];
-            }
-        }
-    float Rf[9], tf[3], Ri[9], ti[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = (float)nrt[r * 4 + c];
-        tf[r] = (float)nrt[r * 4 + 3];
+    // resultRt = [rot | x_t] * resultRt, entry (r, c) on lane 4r + c
+    {
+        const int r = (lane >> 2) < 3 ? (lane >> 2) : 2, c = lane & 3;
+        const double o0 = S.resultRt[c], o1 = S.resultRt[4 + c], o2 = S.resultRt[8 + c], o3 = S.resultRt[12 + c];
+        const double r0 = pick3(r, rot[0], rot[3], rot[6]);
+        const double r1 = pick3(r, rot[1], rot[4], rot[7]);
+        const double r2 = pick3(r, rot[2], rot[5], rot[8]);
+        const double xr = pick3(r, x[0], x[1], x[2]);
+        const double v = r0 * o0 + r1 * o1 + r2 * o2 + xr * o3;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 12) S.resultRt[lane] = v;
     }
+    __builtin_amdgcn_wave_barrier();
+    // Rcurr = Rprev * Rf^T (lanes 0-8), tcurr = Rprev * (-Rf^T tf) + tprev (lanes 9-11),
+    // with Rf, tf the float rotation and translation of the new resultRt
+    if (lane < 12) {
+        float Rf[9], tf[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+        for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Ri[r * 3 + c] = Rf[c * 3 + r];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r * 3] * tf[0] + Ri[r * 3 + 1] * tf[1] + Ri[r * 3 + 2] * tf[2]);
-    if (lead) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) S.resultRt[q] = nrt[q];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                S.Rcurr[r * 3 + c] = Rprev[r * 3] * Ri[c] + Rprev[r * 3 + 1] * Ri[3 + c] + Rprev[r * 3 + 2] * Ri[6 + c];
-            S.tcurr[r] = Rprev[r * 3] * ti[0] + Rprev[r * 3 + 1] * ti[1] + Rprev[r * 3 + 2] * ti[2] + tprev[r];
+            for (int c = 0; c < 3; ++c) Rf[a * 3 + c] = (float)S.resultRt[a * 4 + c];
+            tf[a] = (float)S.resultRt[a * 4 + 3];
         }
+        // Ri = Rf^T, ti = -Ri tf
+        float Ri[9], ti[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Ri[a * 3 + c] = Rf[c * 3 + a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ti[a] = -(Ri[a * 3] * tf[0] + Ri[a * 3 + 1] * tf[1] + Ri[a * 3 + 2] * tf[2]);
+        const int a = lane < 9 ? lane / 3 : lane - 9, c = lane < 9 ? lane % 3 : 0;
+        const float p0 = pick3(a, Rprev[0], Rprev[3], Rprev[6]);
+        const float p1 = pick3(a, Rprev[1], Rprev[4], Rprev[7]);
+        const float p2 = pick3(a, Rprev[2], Rprev[5], Rprev[8]);
+        if (lane < 9) {
+            S.Rcurr[lane] = p0 * pick3(c, Ri[0], Ri[1], Ri[2]) + p1 * pick3(c, Ri[3], Ri[4], Ri[5]) +
+                            p2 * pick3(c, Ri[6], Ri[7], Ri[8]);
+        } else {
+            S.tcurr[a] = p0 * ti[0] + p1 * ti[1] + p2 * ti[2] + pick3(a, tprev[0], tprev[1], tprev[2]);
+        }
+    }
+    if (lead) {
         S.degenerate = (zeros == 6);
         S.iters_done++;
         if (P.convT > 0.0f) {
@@ -1151,6 +1182,38 @@ __device__ void solve_update_uniform(IterState& S, const TrackParams& P, const d
             const double nr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
             if (nt < (double)P.convT && nr < (double)P.convR) S.break_level = level;
         }
+    }
+}
+
+// Per-iteration warp constants from resultRt (RGBDOdometryef.cpp:422-432),
+// Rt = resultRt^-1 taken as the rigid inverse [R^T | -R^T t], by one wave:
+// krkinv entry e = (K Rinv Ki)[e] on lane e (lanes 0-8) and kt[a] = (K tinv)[a]
+// on lane 9 + a; each lane forms the row of K Rinv it needs with the
+// expressions of mm3d.  K, Ki: the level's intrinsics and their inverse.
+__device__ __forceinline__ void iter_constants_wave(IterState& S, const double* K, const double* Ki)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane >= 12) return;
+    const double* Rt = S.resultRt;
+    double Rinv[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Rinv[a * 3 + b] = Rt[b * 4 + a];
+    const int a = lane < 9 ? lane / 3 : lane - 9, c = lane < 9 ? lane % 3 : 0;
+    const double k0 = pick3(a, K[0], K[3], K[6]), k1 = pick3(a, K[1], K[4], K[7]), k2 = pick3(a, K[2], K[5], K[8]);
+    if (lane < 9) {
+        double KR[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) KR[j] = k0 * Rinv[j] + k1 * Rinv[3 + j] + k2 * Rinv[6 + j];
+        const double v = KR[0] * pick3(c, Ki[0], Ki[1], Ki[2]) + KR[1] * pick3(c, Ki[3], Ki[4], Ki[5]) +
+                         KR[2] * pick3(c, Ki[6], Ki[7], Ki[8]);
+        S.krkinv[lane] = (float)v;
+    } else {
+        double tinv[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) tinv[m] = -(Rinv[m * 3] * Rt[3] + Rinv[m * 3 + 1] * Rt[7] + Rinv[m * 3 + 2] * Rt[11]);
+        S.kt[a] = (float)(k0 * tinv[0] + k1 * tinv[1] + k2 * tinv[2]);
     }
 }
 
@@ -1351,16 +1414,6 @@ struct StateScratch {
     unsigned int redi[NT / 64][2];
 };
 
-// State entering iteration `iter` of `level`: skipped when the rgbOnly test
-// stopped this level, else the warp constants of the current estimate
-// (RGBDOdometryef.cpp:418-432) and the lastRGBError the test compares to.
-__device__ __forceinline__ void enter_iteration(IterState& S, const LevelDesc& L, int level, int iter)
-{
-    S.skip = (S.break_level == level);
-    if (!S.skip) iter_constants(S, L);
-    S.rgb_err_cmp = (iter == 0) ? FLT_MAX : S.lastRGBError;
-}
-
 // The prologue of an iteration: finish iteration k-1 and leave the state
 // entering iteration k in W.S -- one fused FP64 reduction of its ICP partials
 // (pass A) and photometric partials (pass B), the ICP statistics, the
@@ -1391,10 +1444,35 @@ __device__ __forceinline__ void prologue_issue(const PairDesc& D, const TrackPar
                        (k > 0 && P.rgb) ? D.nblk_b[plv] : 0, I.PL);
 }
 
-// next: the level of iteration k (its warp constants are set), or nullptr.
+// Orders one wave's LDS accesses across its lanes (a lane reads what another
+// lane of the wave stored before this point).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// State entering iteration `iter` of `level` from the first wave: skipped
+// when the rgbOnly test stopped this level, else the warp constants of the
+// current estimate (RGBDOdometryef.cpp:418-432), and the lastRGBError the test
+// compares to.  K, Ki: the level's intrinsics and their inverse.
+__device__ __forceinline__ void enter_iteration_wave(IterState& S, const double* K, const double* Ki, int level,
+                                                     int iter)
+{
+    if ((threadIdx.x & 63) == 0) {
+        S.skip = (S.break_level == level);
+        S.rgb_err_cmp = (iter == 0) ? FLT_MAX : S.lastRGBError;
+    }
+    wave_lds_sync();
+    if (!S.skip) iter_constants_wave(S, K, Ki);
+}
+
+// K, Ki: intrinsics of the level of iteration k (its warp constants are set),
+// or nullptr.
 template <int NT>
-__device__ void prologue_finish(const PairDesc& D, const TrackParams& P, int k, int prev_level, const PrevPose& pp,
-                                Prolog<NT>& W, PrologIn<NT>& I, const LevelDesc* next, int level, int iter)
+__device__ __forceinline__ void prologue_finish(const PairDesc& D, const TrackParams& P, int k, int prev_level,
+                                                const PrevPose& pp, Prolog<NT>& W, PrologIn<NT>& I, const double* K,
+                                                const double* Ki, int level, int iter)
 {
     constexpr int NW = (int)(sizeof(IterState) / 4);
     asm volatile("" : "+v"(I.sw));
@@ -1403,38 +1481,39 @@ __device__ void prologue_finish(const PairDesc& D, const TrackParams& P, int k, 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (k == 0) {
-        if (next && threadIdx.x == 0) enter_iteration(W.S, *next, level, iter);
+        if (K && wave == 0) enter_iteration_wave(W.S, K, Ki, level, iter);
         __syncthreads();
         return;
     }
     const bool run = P.rgb ? (W.S.pending != 0) : (W.S.skip == 0);
     if (run) partials_sum<NT>(I.PL, W.tot, W.tot1, W.red, W.fin2);
-    // wave 0 solves and then, in its first lane, does the bookkeeping and the
-    // next iteration's constants; wave 1 meanwhile stores the system and the
-    // ICP statistics (disjoint fields of the state)
+    // wave 0 solves, does the bookkeeping and sets the next iteration's
+    // constants; wave 1 meanwhile stores the ICP statistics (disjoint fields)
     if (wave == 0) {
-        if (run) solve_update_uniform(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t, prev_level);
+        if (run) solve_update_wave(W.S, P, W.tot, W.tot + kSE3, pp.R, pp.t, prev_level);
+        wave_lds_sync();
         if (lane == 0) {
             if (run && !P.rgb) assoc_rgb(W.S, P, W.S.rgb_err_cmp, 0, 0);
             if (W.S.broke) W.S.break_level = prev_level;
             W.S.pending = 0;
             W.S.broke = 0;
-            if (next) enter_iteration(W.S, *next, level, iter);
         }
+        wave_lds_sync();
+        if (K) enter_iteration_wave(W.S, K, Ki, level, iter);
     } else if (wave == 1 && run) {
-        store_system_stats(W.S, P, W.tot1, W.tot1 + kSE3, lane);
+        store_icp_stats(W.S, P, W.tot1, lane);
     }
     __syncthreads();
 }
 
 template <int NT>
 __device__ __forceinline__ void finish_previous(const PairDesc& D, const TrackParams& P, int k, int prev_level,
-                                                const PrevPose& pp, Prolog<NT>& W, const LevelDesc* next, int level,
-                                                int iter)
+                                                const PrevPose& pp, Prolog<NT>& W, const double* K, const double* Ki,
+                                                int level, int iter)
 {
     PrologIn<NT> I;
     prologue_issue<NT>(D, P, k, prev_level, I);
-    prologue_finish<NT>(D, P, k, prev_level, pp, W, I, next, level, iter);
+    prologue_finish<NT>(D, P, k, prev_level, pp, W, I, K, Ki, level, iter);
 }
 
 // ======================================================== step kernel A
@@ -1562,7 +1641,7 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
     load_tile_a<NT, PPT>(L, P, lb * NT * PPT + threadIdx.x, N, P3, T);
 
     if constexpr (FUSED) {
-        prologue_finish<NT>(D, P, k, prev_level, pp, W, PI, &D.lv[level], level, iter);
+        prologue_finish<NT>(D, P, k, prev_level, pp, W, PI, L.K, L.Ki, level, iter);
         if (lb == 0) copy_out<NT>(&st->it[k & 1], &W.S);
     } else {
         copy_in<NT>(&W.S, &st->it[k & 1]);
@@ -1616,7 +1695,7 @@ __global__ __launch_bounds__(NT) void k_solve(const PairDesc* __restrict__ pairs
     TrackState* st = D.st;
     PrevPose pp;
     load_prev(st, pp);
-    finish_previous<NT>(D, P, k, prev_level, pp, W, &D.lv[level], level, iter);
+    finish_previous<NT>(D, P, k, prev_level, pp, W, D.lv[level].K, D.lv[level].Ki, level, iter);
     copy_out<NT>(&st->it[k & 1], &W.S);
 }
 
@@ -2068,7 +2147,7 @@ __global__ __launch_bounds__(256) void k_track_end(const PairDesc* __restrict__ 
     float* const plog = gp(st)->pose_log;
     const int plog_cap = gp(st)->pose_log_cap;
     const int frame = gp(st)->frame_count;
-    finish_previous<256>(D, P, K, last_level, pp, W, nullptr, 0, 0);
+    finish_previous<256>(D, P, K, last_level, pp, W, nullptr, nullptr, 0, 0);
     if (threadIdx.x != 0) return;
     GLOBAL_AS TrackState* g = gp(st);
     const IterState& S = W.S;
