@@ -29,6 +29,12 @@ FMT_RGBA32F = 4
 MEM_HOST = 0
 MEM_DEVICE = 1
 
+# Loop-closure acceptance thresholds of the reference's configs
+# (src/configs/ef_file.cfg:9-14): icpErrThresh, covThresh, icpCountThresh.
+ICP_ERR_THRESH = 5e-5
+COV_THRESH = 1e-5
+ICP_COUNT_THRESH = 40000
+
 
 class FrameView(ctypes.Structure):
     _fields_ = [("ptr", ctypes.c_void_p), ("pitch_bytes", ctypes.c_int), ("width", ctypes.c_int),
@@ -386,6 +392,34 @@ class RGBDOdometry:
         _check("odom_loop_check", lib().odom_loop_check(self._h, float(covThresh), float(icpCountThresh),
                                                         float(icpErrThresh), ctypes.byref(ok)))
         return bool(ok.value)
+
+    def modelToModel(self, inactiveVertices, inactiveNormals, inactiveImage, activeVertices, activeNormals,
+                     activeImage, pose, depthCutoff: float, pyramid: bool = True, fastOdom: bool = False,
+                     covThresh: float = COV_THRESH, icpCountThresh: float = ICP_COUNT_THRESH,
+                     icpErrThresh: float = ICP_ERR_THRESH):
+        """The frame loop's local loop-closure check on this handle (its
+        ``modelToModel`` odometry), elastic_fusion_file.cpp:449-498: the
+        active model prediction is registered to the inactive one from
+        ``pose`` (4x4) with ICP and RGB and SO3 off, and accepted when every
+        diagonal entry of the covariance (inverse of the last 6x6 system) is
+        at most ``covThresh``, more than ``icpCountThresh`` pixels were ICP
+        inliers and the ICP error is below ``icpErrThresh`` (odom_loop_check).
+
+        The vertex/normal maps are (h, w, 4) float32 in the camera frame of
+        ``pose``, the images (h, w, 3 or 4) uint8, host arrays or device
+        tensors.  Returns (accepted, estPose) with estPose the registered 4x4
+        float32 estimate."""
+        pose = np.asarray(pose, dtype=np.float32)
+        # initICP* before initRGB*: the RGB inits read the vertex maps
+        self.initICPModel(inactiveVertices, inactiveNormals, depthCutoff, pose)
+        self.initRGBModel(inactiveImage)
+        self.initICPMaps(activeVertices, activeNormals, depthCutoff)
+        self.initRGB(activeImage)
+        t, R = self.getIncrementalTransformation(pose[:3, 3], pose[:3, :3], False, 10.0, pyramid, fastOdom, False)
+        est = np.eye(4, dtype=np.float32)
+        est[:3, :3] = np.asarray(R, np.float32).reshape(3, 3)
+        est[:3, 3] = np.asarray(t, np.float32).reshape(3)
+        return self.loopCheck(covThresh, icpCountThresh, icpErrThresh), est
 
     def stats(self) -> dict:
         s = Stats()

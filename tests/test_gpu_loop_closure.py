@@ -6,8 +6,8 @@ vertex) that must be rejected without an error."""
 import numpy as np
 import pytest
 
-from slam_amd import loop_closure, synth
-from slam_amd.odometry import RGBDOdometry
+from slam_amd import synth
+from slam_amd.odometry import COV_THRESH, ICP_COUNT_THRESH, ICP_ERR_THRESH, RGBDOdometry
 
 from helpers import CUTOFF, make_pair, pose_of
 
@@ -63,25 +63,23 @@ def test_loop_check_matches_oracle(orc, cam, seed):
 
 @pytest.mark.parametrize("cam,seed", [(synth.CAM_320, 33), (synth.CAM_640, 34)], ids=["320x240", "640x480"])
 def test_model_to_model_default_thresholds(orc, cam, seed):
-    """The wrapper with the reference configs' thresholds decides as the
-    oracle does (the case is skipped where a measured value sits within 5% of
+    """RGBDOdometry.modelToModel with the reference configs' thresholds
+    decides as the oracle does (the case is skipped where a measured value sits within 5% of
     a threshold, where FP32 summation order could flip it)."""
     T0, T1, f0, f1 = make_pair(cam, seed)
-    res = loop_closure.model_to_model(mk(cam), f0.vert4, f0.norm4, f0.rgba, f1.vert4, f1.norm4, f1.rgba,
-                                      T0, CUTOFF)
+    accepted, est = mk(cam).modelToModel(f0.vert4, f0.norm4, f0.rgba, f1.vert4, f1.norm4, f1.rgba, T0, CUTOFF)
     ref = orc.Odometry(cam.width, cam.height, cam.cx, cam.cy, cam.fx, cam.fy)
     Tr = register(ref, f0, f1, T0)
-    dt, dr = synth.pose_error(res.pose.astype(np.float64), Tr)
+    dt, dr = synth.pose_error(est.astype(np.float64), Tr)
     assert dt <= 1e-3 and dr <= 0.1, (dt, dr)
     s = ref.stats()
     cov = float(np.diag(ref.getCovariance()).max())
-    near = [abs(v / th - 1) < 0.05 for v, th in ((cov, loop_closure.COV_THRESH),
-                                                 (s["lastICPCount"], loop_closure.ICP_COUNT_THRESH),
-                                                 (s["lastICPError"], loop_closure.ICP_ERR_THRESH))]
+    near = [abs(v / th - 1) < 0.05 for v, th in ((cov, COV_THRESH), (s["lastICPCount"], ICP_COUNT_THRESH),
+                                                 (s["lastICPError"], ICP_ERR_THRESH))]
     if any(near):
         pytest.skip(f"measured values within 5% of a threshold: cov {cov} stats {s}")
-    want = ref.loop_accept(loop_closure.COV_THRESH, loop_closure.ICP_COUNT_THRESH, loop_closure.ICP_ERR_THRESH)
-    assert res.accepted == want, (cov, s)
+    want = ref.loop_accept(COV_THRESH, ICP_COUNT_THRESH, ICP_ERR_THRESH)
+    assert accepted == want, (cov, s)
 
 
 def test_degenerate_registration_is_rejected(orc):
