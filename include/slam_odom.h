@@ -128,6 +128,16 @@ int odom_synchronize(odom_handle h);
 /* Replay each distinct tracking call (options, schedule and image parity)
  * from a hipGraph captured on its first use; 0 launches kernel by kernel. */
 int odom_set_graph(odom_handle h, int enable);
+/* No counterpart in the reference.  The SO3 pre-alignment loop
+ * (RGBDOdometryef.cpp:294-380) runs as one launch whose workgroups meet at a
+ * grid barrier after every iteration (when the pairs' workgroups fit on the
+ * device at once, else one launch per iteration).  spin_limit: polls before a
+ * barrier wait gives up (default 1 << 20, about a second); a call in which
+ * one gave up makes odom_get_pose and odom_get_stats return ODOM_ERR_HIP
+ * until odom_init_first_rgb; 0 gives up at the first wait (the failure path,
+ * for tests).  spin_limit < 0 selects one launch per iteration.  Both forms give the same results, bit for bit.  A batch takes
+ * its first handle's setting when it is created. */
+int odom_set_so3_loop(odom_handle h, int spin_limit);
 /* Record every call's resulting pose ([R|t], 12 floats row-major) into a
  * device ring of `capacity` entries, slot = call index % capacity; the call
  * index restarts at 0.  capacity 0 turns the log off. */

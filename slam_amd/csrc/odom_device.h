@@ -18,6 +18,9 @@ constexpr int kSO3 = 11;   // 9 products of [J|r] (3x3+3), r^2, count
 constexpr int kPartStride = 32;  // floats per block partial record
 constexpr int kMaxBlk = 256;     // block partials per pair, level and pass: the step
                                  // grids stride over their tiles with at most this many blocks
+// Polls of the one-launch SO3 loop's barrier waits before one gives up (about
+// a second); the default of a handle (odom_set_so3_loop).
+constexpr int kSO3SpinDefault = 1 << 20;
 
 // Maps of one pyramid level of one pair.  Vertex/normal maps are planar
 // (x plane, y plane, z plane of w*h floats each); a pixel is invalid iff its x
@@ -121,6 +124,11 @@ struct TrackState {
     double lastRt[16];
     int alt_iters;                   // iterations of the judged-away hypothesis
     int judge_level;                 // level after which the last judge ran (-1 none)
+    // the SO3 loop in one launch: arrivals at its grid barrier and exits of
+    // its workgroups (both back to 0 when the launch ends), and a sticky flag
+    // set when a barrier wait gave up
+    unsigned int so3_arrive, so3_exit;
+    int so3_fail;
 };
 
 // Uniform options of one track call.

@@ -127,6 +127,7 @@ struct odom_ctx {
 
     PairDesc h_pair[2]{};
     StepShape shape_a[kLevels], shape_b[kLevels];
+    int so3_spin = kSO3SpinDefault;  // polls before an SO3-loop barrier wait gives up; < 0: one launch per iteration
     TrackState h_st{};
     TrackState* h_pin = nullptr;     // pinned landing buffer of the state readback
     bool have_state = false;
@@ -400,7 +401,7 @@ void track_launches(odom_ctx* c, const TrackParams& P, hipEvent_t* ev)
         launch_rgb_prep(a, c->stream);
     }
     PairDesc* dp = c->d_pair[c->parity];
-    if (P.so3) launch_so3(dp, 1, &P, c->stream);  // begin rides on the last SO3 launch
+    if (P.so3) launch_so3(dp, 1, &P, c->so3_spin, c->stream);  // begin rides on the SO3 loop's end
     else launch_track_begin(dp, 1, P, c->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
@@ -733,6 +734,8 @@ int odom_init_first_rgb(odom_handle c, const odom_frame_view* rgb)
     double eye[16];
     for (int k = 0; k < 16; ++k) eye[k] = (k % 5 == 0) ? 1.0 : 0.0;
     if (int rc = write_field(c, c->st->lastRt, eye, 16)) return rc;
+    const int zero = 0;
+    if (int rc = write_field(c, &c->st->so3_fail, &zero, 1)) return rc;
     return enqueue_populate(c, rgb, nullptr, dst);
 }
 
@@ -752,6 +755,7 @@ int odom_get_pose(odom_handle c, float trans[3], float rot[9])
     if (!c || !trans || !rot) return ODOM_ERR_ARG;
     int rc = read_state(c);
     if (rc) return rc;
+    if (c->h_st.so3_fail) return ODOM_ERR_HIP;
     for (int a = 0; a < 3; ++a) {
         for (int b = 0; b < 3; ++b) rot[a * 3 + b] = c->h_st.pose_out[a * 4 + b];
         trans[a] = c->h_st.pose_out[a * 4 + 3];
@@ -817,6 +821,16 @@ int odom_track(odom_handle c, float trans[3], float rot[9], int rgbOnly, float i
     return odom_get_pose(c, trans, rot);
 }
 
+int odom_set_so3_loop(odom_handle c, int spin_limit)
+{
+    if (!c) return ODOM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->so3_spin = spin_limit;
+    drop_graphs(c);
+    drop_self_batch(c);
+    return ODOM_OK;
+}
+
 int odom_set_graph(odom_handle c, int enable)
 {
     if (!c) return ODOM_ERR_ARG;
@@ -866,6 +880,7 @@ int odom_get_stats(odom_handle c, odom_stats* s)
     if (!c || !s) return ODOM_ERR_ARG;
     const int rc = read_state(c);
     if (rc) return rc;
+    if (c->h_st.so3_fail) return ODOM_ERR_HIP;
     return slam::odom_stats_last_read(c, s);
 }
 
@@ -1277,6 +1292,7 @@ struct odom_batch_s {
     PairDesc* h_stage = nullptr;     // pinned upload buffer (kMaxHyp entries per handle)
     bool use_graph = false;
     bool split_solve = true;         // k_solve + unfused pass A (else the fused prologue in every block)
+    int so3_spin = kSO3SpinDefault;  // the SO3 loop setting of the batch's first handle
     struct G { std::vector<int> key; hipGraphExec_t exec; };
     std::vector<G> graphs;
     hipError_t last_hip = hipSuccess;
@@ -1442,7 +1458,7 @@ void batch_launches(odom_batch_s* b, const CallSpec& cs, PairDesc* const* dp, hi
     int hyp = cs.hyp;
     PairDesc* cur = dp[hyp - 1];
     if (cs.policy_start) launch_seed_policy(cur, n * hyp, hyp, cs.pred, cs.t_scale, cs.r_scale, b->stream);
-    if (P.so3) launch_so3(cur, n * hyp, &P, b->stream);  // begin rides on the last SO3 launch
+    if (P.so3) launch_so3(cur, n * hyp, &P, b->so3_spin, b->stream);  // begin rides on the SO3 loop's end
     else launch_track_begin(cur, n * hyp, P, b->stream);
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
@@ -1656,6 +1672,7 @@ int batch_new(const odom_handle* handles, int n, bool own_stream, odom_batch_s**
     odom_batch_s* b = new odom_batch_s();
     b->h.assign(handles, handles + n);
     b->own_stream = own_stream;
+    b->so3_spin = handles[0]->so3_spin;
     if (own_stream) {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {

@@ -94,7 +94,9 @@ struct StepLaunch {
 };
 // SO3 pre-alignment; max_px_l2: largest level-2 pixel count over the pairs
 // The SO3 loop; with begin, its last launch also does launch_track_begin's work.
-void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, hipStream_t s);
+// spin_limit < 0: one launch per iteration; else the one-launch loop (when
+// the pairs fit), whose barrier waits give up after spin_limit polls.
+void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s);
 void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
 // warm start: seed from the network prediction (6 floats per pair, device),
 // optionally kept only when the level-2 ICP association is not worse than plain

@@ -2650,9 +2650,166 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
     block_partial<kSO3, kSO3Threads>(acc, D.part_a[it & 1] + (size_t)lb * kPartStride, s_red);
 }
 
-void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, hipStream_t s)
+// The SO3 loop in one launch: the kSO3Blocks workgroups of a pair run the
+// iterations of k_so3_iter back to back and meet at a grid barrier after each
+// iteration's block partials (an agent-scope release add on the pair's arrival
+// counter, then acquire loads until every workgroup has arrived).  Each
+// workgroup then sums the partials and evaluates the iteration itself, as
+// k_so3_iter's prologue does, so the state, the sums and their order are
+// those of the launch-per-iteration form.  Every wait is bounded by
+// spin_limit polls; a wait that gives up sets the sticky so3_fail flag and
+// its workgroup leaves the loop, so every workgroup exits.  The last
+// workgroup to exit sets the counters back to 0 for the next call.  Only
+// launched when the pairs' workgroups fit on the device at once.
+__global__ __launch_bounds__(kSO3Threads) void k_so3_loop(const PairDesc* __restrict__ pairs, int begin,
+                                                          TrackParams P, int spin_limit)
+{
+    __shared__ So3Slot S;
+    __shared__ double s_tot[kSO3];
+    __shared__ float s_red[kSO3Threads / 64][kSO3];
+    __shared__ float sH[9], sKi[9], sKR[9];
+    __shared__ int s_fail;
+    const int p = blockIdx.x / kSO3Blocks, lb = blockIdx.x - p * kSO3Blocks;
+    const PairDesc& D = pairs[p];
+    TrackState* st = D.st;
+    const LevelDesc& L = D.lv[2];
+    const int w = L.w, h = L.h, n = w * h;
+    constexpr int stride = kSO3Blocks * kSO3Threads;
+    const int first = lb * kSO3Threads + threadIdx.x;
+    const GLOBAL_AS uint8_t* last = gp(D.so3_last);
+    const GLOBAL_AS uint8_t* next = gp(D.so3_next);
+    unsigned int* arrive = &st->so3_arrive;
+
+    // the rotation-independent part of this thread's first pixels, once
+    So3Src src[kSO3Pre];
+#pragma unroll
+    for (int j = 0; j < kSO3Pre; ++j) {
+        const int idx = first + j * stride;
+        if (idx < n) {
+            const int y = idx / w, x = idx - y * w;
+            so3_src(last, w, h, x, y, src[j]);
+        }
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 9; ++k) {
+            S.resultR[k] = S.lastResultR[k] = (k % 4 == 0) ? 1.0 : 0.0;
+            S.R_lr[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+        }
+        S.lastError = S.lastCount = FLT_MAX / 2;
+        S.err = S.cnt = 0.0f;
+        S.iters = 0;
+        S.done = 0;
+        s_fail = 0;
+    }
+    __syncthreads();
+    for (int it = 0; it < 10; ++it) {
+        if (threadIdx.x == 0) {
+            double KR[9], Hd[9];
+            mm3d(L.K, S.resultR, KR);
+            mm3d(KR, L.Ki, Hd);
+            for (int k = 0; k < 9; ++k) {
+                sH[k] = (float)Hd[k];
+                sKi[k] = (float)L.Ki[k];
+                sKR[k] = (float)KR[k];
+            }
+        }
+        __syncthreads();
+        float Hr[9], Kir[9], KRr[9];
+        for (int k = 0; k < 9; ++k) {
+            Hr[k] = uni(sH[k]);
+            Kir[k] = uni(sKi[k]);
+            KRr[k] = uni(sKR[k]);
+        }
+        float acc[kSO3];
+#pragma unroll
+        for (int k = 0; k < kSO3; ++k) acc[k] = 0.0f;
+        int wx[kSO3Pre], wy[kSO3Pre];
+        bool ok[kSO3Pre];
+#pragma unroll
+        for (int j = 0; j < kSO3Pre; ++j) {
+            const int idx = first + j * stride;
+            const int y = idx / w, x = idx - y * w;
+            ok[j] = idx < n && so3_warp(Hr, w, h, x, y, wx[j], wy[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kSO3Pre; ++j)
+            if (ok[j]) {
+                const int idx = first + j * stride;
+                const int y = idx / w, x = idx - y * w;
+                float row[4];
+                so3_row(next, w, wx[j], wy[j], src[j], x, y, Kir, KRr, row);
+                acc_row4(acc, row);
+            }
+        for (int idx = first + kSO3Pre * stride; idx < n; idx += stride) {
+            const int y = idx / w, x = idx - y * w;
+            float row[4];
+            if (so3_pixel(last, next, w, h, x, y, Hr, Kir, KRr, row)) acc_row4(acc, row);
+        }
+        const float* part = D.part_a[it & 1];
+        block_partial<kSO3, kSO3Threads>(acc, (float*)part + (size_t)lb * kPartStride, s_red);
+        // grid barrier of the pair's workgroups
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned int target = (unsigned int)(it + 1) * kSO3Blocks;
+            __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int polls = 0;
+            // spin_limit 0 gives up at once (the failure path, forced from the host)
+            while (spin_limit == 0 ||
+                   __hip_atomic_load(arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++polls > spin_limit) {
+                    s_fail = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        if (s_fail) break;
+        // the iteration's totals (FP64, rows in order) and its evaluation
+        if (threadIdx.x < kSO3) {
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < kSO3Blocks; ++r) t += (double)gp(part)[r * kPartStride + threadIdx.x];
+            s_tot[threadIdx.x] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) so3_evaluate(S, s_tot);
+        __syncthreads();
+        if (S.done) break;
+    }
+    if (threadIdx.x == 0) {
+        GLOBAL_AS TrackState* g = gp(st);
+        if (s_fail) {
+            g->so3_fail = 1;
+        } else if (lb == 0) {
+            for (int k = 0; k < 9; ++k) g->so3R[k] = S.resultR[k];
+            g->so3_iters = S.iters;
+            g->lastSO3Error = S.err;
+            g->lastSO3Count = S.cnt;
+            if (begin) track_begin_pair(D, P, S.resultR);
+        }
+        // the last workgroup out resets the counters
+        if (__hip_atomic_fetch_add(&st->so3_exit, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+            kSO3Blocks - 1) {
+            __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->so3_exit, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Workgroups of the one-launch SO3 loop that are sure to be resident at once
+// on an otherwise idle device (one per CU of a 256-CU part, each 512 threads
+// and ~2 KB of LDS); larger batches take the launch-per-iteration form.
+constexpr int kSO3LoopMaxBlocks = 256;
+
+void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s)
 {
     const TrackParams P = begin ? *begin : TrackParams{};
+    if (spin_limit >= 0 && npairs * kSO3Blocks <= kSO3LoopMaxBlocks) {
+        hipLaunchKernelGGL(k_so3_loop, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, begin ? 1 : 0, P,
+                           spin_limit);
+        return;
+    }
     for (int it = 0; it <= 10; ++it)
         hipLaunchKernelGGL(k_so3_iter, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, it,
                            (it == 10 && begin) ? 1 : 0, P);

@@ -91,6 +91,7 @@ EXPORTS = [
     "odom_get_stream", "odom_init_warmstart_input", "odom_track_seeded", "odom_set_graph", "odom_profile_steps",
     "odom_loop_check", "odom_icp_step_noalloc", "odom_default_policy", "odom_track_policy",
     "odom_batch_get_stream", "odom_batch_warmstart_input", "odom_track_batch_policy", "odom_stream_copy",
+    "odom_set_so3_loop",
 ]
 
 _V = ctypes.c_void_p
@@ -116,6 +117,7 @@ def lib() -> ctypes.CDLL:
     L.odom_track.argtypes = [_V, _V, _V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.odom_track_async.argtypes = [_V, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.odom_set_pose.argtypes = [_V, _V, _V]
+    L.odom_set_so3_loop.argtypes = [_V, ctypes.c_int]
     L.odom_get_pose.argtypes = [_V, _V, _V]
     L.odom_synchronize.argtypes = [_V]
     L.odom_get_covariance.argtypes = [_V, _V]
@@ -392,6 +394,11 @@ class RGBDOdometry:
         _check("odom_loop_check", lib().odom_loop_check(self._h, float(covThresh), float(icpCountThresh),
                                                         float(icpErrThresh), ctypes.byref(ok)))
         return bool(ok.value)
+
+    def set_so3_loop(self, spin_limit: int = 1 << 20) -> None:
+        """The SO3 loop as one launch with a bounded grid barrier (spin_limit
+        polls per wait), or one launch per iteration (spin_limit < 0)."""
+        _check("odom_set_so3_loop", lib().odom_set_so3_loop(self._h, int(spin_limit)))
 
     def modelToModel(self, inactiveVertices, inactiveNormals, inactiveImage, activeVertices, activeNormals,
                      activeImage, pose, depthCutoff: float, pyramid: bool = True, fastOdom: bool = False,
