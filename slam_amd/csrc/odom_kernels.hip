@@ -497,18 +497,21 @@ __device__ __forceinline__ uint16_t pyr_u16_at(Get get, int sw, int sh, int x, i
 
 // Levels 0 (when generated), 1 and 2 of up to two pyramids in one launch,
 // from LDS tiles.  A level-1 block stages the level-0 texels of its 16x16
-// outputs' windows (36x36); a level-2 block stages 43x43 level-0 texels,
-// computes the 20x20 level-1 values its 8x8 outputs read (the same
-// computation the level-1 blocks store), then its outputs.  Level 0 is read
+// outputs' windows (36x36); a level-2 block stages 43x27 level-0 texels,
+// computes the 20x12 level-1 values its 8x4 outputs read (the same
+// computation the level-1 blocks store; one value per thread, so the block
+// runs two dependent Gaussian sums, not three), then its outputs.  Level 0 is read
 // from memory or generated (verticesToDepth, or the intensity conversion of
 // populateRGBDData, RGBDOdometryef.cpp:212, 225), in which case level-0
 // blocks also store it.  The taps, their clamped coordinates and the
 // summation order are those of the per-level functions above.
 constexpr int kPT = 16;                 // level-1 output tile edge
 constexpr int kT1 = 2 * kPT + 4;        // its level-0 texels
-constexpr int kPT2 = 8;                 // level-2 output tile edge
-constexpr int kT21 = 2 * kPT2 + 4;      // its level-1 values
-constexpr int kT20 = 2 * kT21 + 3;      // its level-0 texels
+constexpr int kPT2X = 8, kPT2Y = 4;     // level-2 output tile (its level-1 values, below, fit
+                                        // one pass of the 256 threads)
+constexpr int kT21X = 2 * kPT2X + 4, kT21Y = 2 * kPT2Y + 4;   // its level-1 values
+constexpr int kT20X = 2 * kT21X + 3, kT20Y = 2 * kT21Y + 3;   // its level-0 texels
+static_assert(kT21X * kT21Y <= 256, "one level-1 value per thread");
 
 struct PyrGrid {
     PyrJob j;
@@ -605,25 +608,26 @@ __device__ void pyr_block(const PyrGrid& g, int q, int t, int lb, unsigned char*
         if (x < w1 && y < h1) gp((T*)P.dst[1])[(size_t)y * w1 + x] = pyr_at<KIND>(t1, kT1, ox, oy, w0, h0, x, y);
         return;
     }
-    const int X0 = tx * kPT2, Y0 = ty * kPT2;
-    T* t0 = t1 + kT21 * kT21;                    // level-0 texels
+    const int X0 = tx * kPT2X, Y0 = ty * kPT2Y;
+    T* t0 = t1 + kT21X * kT21Y;                  // level-0 texels
     const int o1x = 2 * X0 - 2, o1y = 2 * Y0 - 2;
     const int o0x = 2 * o1x - 2, o0y = 2 * o1y - 2;
-    pyr_stage<kT20 * kT20, T>(P, t0, kT20, o0x, o0y, w0, h0);
+    pyr_stage<kT20X * kT20Y, T>(P, t0, kT20X, o0x, o0y, w0, h0);
     __syncthreads();
-    for (int i = tid; i < kT21 * kT21; i += 256) {
-        const int x1 = o1x + i % kT21, y1 = o1y + i / kT21;
-        if (x1 >= 0 && x1 < w1 && y1 >= 0 && y1 < h1) t1[i] = pyr_at<KIND>(t0, kT20, o0x, o0y, w0, h0, x1, y1);
+    if (tid < kT21X * kT21Y) {
+        const int x1 = o1x + tid % kT21X, y1 = o1y + tid / kT21X;
+        if (x1 >= 0 && x1 < w1 && y1 >= 0 && y1 < h1) t1[tid] = pyr_at<KIND>(t0, kT20X, o0x, o0y, w0, h0, x1, y1);
     }
     __syncthreads();
-    const int x = X0 + (tid & (kPT2 - 1)), y = Y0 + tid / kPT2;
-    if (tid < kPT2 * kPT2 && x < w2 && y < h2)
-        gp((T*)P.dst[2])[(size_t)y * w2 + x] = pyr_at<KIND>(t1, kT21, o1x, o1y, w1, h1, x, y);
+    const int x = X0 + (tid & (kPT2X - 1)), y = Y0 + tid / kPT2X;
+    if (tid < kPT2X * kPT2Y && x < w2 && y < h2)
+        gp((T*)P.dst[2])[(size_t)y * w2 + x] = pyr_at<KIND>(t1, kT21X, o1x, o1y, w1, h1, x, y);
 }
 
 __global__ __launch_bounds__(256) void k_pyramid(PyrGrid g)
 {
-    __shared__ __align__(16) unsigned char smem[(kT1 * kT1 > kT21 * kT21 + kT20 * kT20 ? kT1 * kT1 : kT21 * kT21 + kT20 * kT20) * 4];
+    constexpr int kL2 = kT21X * kT21Y + kT20X * kT20Y;
+    __shared__ __align__(16) unsigned char smem[(kT1 * kT1 > kL2 ? kT1 * kT1 : kL2) * 4];
     int sgi = 0;
     while (sgi + 1 < g.nseg && (int)blockIdx.x >= g.b0[sgi + 1]) ++sgi;
     const int q = g.seg_q[sgi], t = g.seg_t[sgi], lb = blockIdx.x - g.b0[sgi];
@@ -639,8 +643,8 @@ void launch_pyramid(const PyrJob& j, hipStream_t s)
     g.j = j;
     const int w1 = j.w0 / 2, h1 = j.h0 / 2, w2 = w1 / 2, h2 = h1 / 2;
     g.tiles_x[1] = cdiv(w1, kPT);
-    g.tiles_x[2] = cdiv(w2, kPT2);
-    const int n2 = g.tiles_x[2] * cdiv(h2, kPT2), n1 = g.tiles_x[1] * cdiv(h1, kPT);
+    g.tiles_x[2] = cdiv(w2, kPT2X);
+    const int n2 = g.tiles_x[2] * cdiv(h2, kPT2Y), n1 = g.tiles_x[1] * cdiv(h1, kPT);
     int nb = 0;
     // level-2 tiles first (the longest blocks), then level 1, then level-0 generation
     for (int t = 2; t >= 0; --t)
