@@ -43,7 +43,7 @@ int cnn_destroy(cnn_net c);
 int cnn_set_stream(cnn_net c, void *hip_stream);
 /* flat fp32 parameter vector: per convolution W [Kp][Cout] then b [Cout]
  * (Kp = 9 Cin rounded up to 32, rows k = (kh*3 + kw)*Cin + c), then fc1 W
- * [128][256], b [256], fc2 W [256][8], b [8] (columns 6, 7 unused).
+ * [128][3840], b [3840], fc2 W [3840][8], b [8] (columns 6, 7 unused).
  * cnn_param_layout returns the number of tensors (16) and their offsets and
  * sizes in that order. */
 int cnn_num_params(cnn_net c, size_t *n);

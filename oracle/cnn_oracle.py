@@ -5,9 +5,9 @@ Only ``tests/`` may import it.
 
 Same architecture and flat parameter layout as the library: six 3x3 / stride 2
 / pad 1 convolutions with ReLU (4-32-64-96-128-128-128 channels, NHWC), global
-average pooling, FC 128-256 (ReLU), FC 256-6.  Per convolution W is [Kp][Cout]
+average pooling, FC 128-3840 (ReLU), FC 3840-6.  Per convolution W is [Kp][Cout]
 with rows k = (kh * 3 + kw) * Cin + c (rows >= 9 Cin are padding), then b;
-fc1 W [128][256], b [256]; fc2 W [256][8], b [8] (columns 6, 7 unused).
+fc1 W [128][3840], b [3840]; fc2 W [3840][8], b [8] (columns 6, 7 unused).
 Loss: sum_j w_j (out_j - target_j)^2 per sample, gradients of its mean.
 """
 from __future__ import annotations
@@ -16,7 +16,7 @@ import numpy as np
 
 SPEC = [(4, 32, 60, 80, 30, 40), (32, 64, 30, 40, 15, 20), (64, 96, 15, 20, 8, 10),
         (96, 128, 8, 10, 4, 5), (128, 128, 4, 5, 2, 3), (128, 128, 2, 3, 1, 2)]
-FC1_IN, FC1_OUT, OUT, OUT_PAD = 128, 256, 6, 8
+FC1_IN, FC1_OUT, OUT, OUT_PAD = 128, 3840, 6, 8
 
 
 def kp(cin: int) -> int:

@@ -4,8 +4,8 @@
 // (model depth, model intensity, current depth, current intensity).
 // Six 3x3 stride-2 convolutions (pad 1) with ReLU, 4 -> 32 -> 64 -> 96 ->
 // 128 -> 128 -> 128 channels (60x80 -> 30x40 -> 15x20 -> 8x10 -> 4x5 -> 2x3
-// -> 1x2), global average pooling, FC 128 -> 256 (ReLU) -> 6: a scaled se(3)
-// increment.  No counterpart in the reference (its only network is the
+// -> 1x2), global average pooling, FC 128 -> 3840 (ReLU) -> 6: a scaled se(3)
+// increment (1.0 M parameters).  No counterpart in the reference (its only network is the
 // unrelated YOLACT segmenter); the tracker uses the prediction as the initial
 // increment of a Gauss-Newton solve.
 //
@@ -388,7 +388,7 @@ struct ConvSpec {
 constexpr int kConvs = 6;
 const ConvSpec kSpec[kConvs] = {{4, 32, 60, 80, 30, 40},  {32, 64, 30, 40, 15, 20},  {64, 96, 15, 20, 8, 10},
                                 {96, 128, 8, 10, 4, 5},   {128, 128, 4, 5, 2, 3},  {128, 128, 2, 3, 1, 2}};
-constexpr int kFc1In = 128, kFc1Out = 256, kOut = 6, kOutPad = 8;
+constexpr int kFc1In = 128, kFc1Out = 3840, kOut = 6, kOutPad = 8;
 
 struct Param {
     size_t off, n;
@@ -401,7 +401,7 @@ using namespace cnn;
 struct cnn_net_s {
     int max_batch = 0;
     hipStream_t stream = nullptr;    // null: the legacy default stream
-    // parameters (flat): per conv W [Kp][Cout] + b [Cout]; fc1 W [128][256] + b; fc2 W [256][8] + b [8]
+    // parameters (flat): per conv W [Kp][Cout] + b [Cout]; fc1 W [128][3840] + b; fc2 W [3840][8] + b [8]
     Param convW[kConvs], convB[kConvs], fc1W, fc1B, fc2W, fc2B;
     int Kp[kConvs];
     size_t nparam = 0;
@@ -554,7 +554,7 @@ struct InferParams {
     const bf16* W[kConvs];                   // [Kp][Cout] bf16
     const float* b[kConvs];
     int Kp[kConvs];
-    const bf16* fc1W;                        // [128][256]
+    const bf16* fc1W;                        // [128][3840]
     const float* fc1b;
     const bf16* fc2W;                        // [256][8]
     const float* fc2b;
@@ -667,17 +667,31 @@ __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restr
         s_gap[t] = bf2f(f2bf(sgap / 2.0f));
     }
     __syncthreads();
-    // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands
-    if (t < kFc1Out) {
-        float acc = 0.0f;
-        for (int k = 0; k < kFc1In; ++k) acc += s_gap[k] * bf2f(P.fc1W[(size_t)k * kFc1Out + t]);
-        float v = fmaxf(acc + P.fc1b[t], 0.0f);
-        s_h1[t] = bf2f(f2bf(v));
+    // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands;
+    // each thread forms its kFc1Out / kInferThreads hidden units side by side
+    {
+        constexpr int H = (kFc1Out + kInferThreads - 1) / kInferThreads;
+        float acc[H];
+#pragma unroll
+        for (int q = 0; q < H; ++q) acc[q] = 0.0f;
+        for (int k = 0; k < kFc1In; ++k) {
+            const float g = s_gap[k];
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const int o = t + q * kInferThreads;
+                if (o < kFc1Out) acc[q] += g * bf2f(P.fc1W[(size_t)k * kFc1Out + o]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const int o = t + q * kInferThreads;
+            if (o < kFc1Out) s_h1[o] = bf2f(f2bf(fmaxf(acc[q] + P.fc1b[o], 0.0f)));
+        }
     }
     __syncthreads();
     if (t < 64 * kOut) {
-        // one wave per output: lanes split the 256-term dot product, then a
-        // fixed-order wave sum
+        // one wave per output: lanes split the kFc1Out-term dot product, then
+        // a fixed-order wave sum
         const int j = t >> 6, lane = t & 63;
         float acc = 0.0f;
         for (int k = lane; k < kFc1Out; k += 64) acc += s_h1[k] * bf2f(P.fc2W[(size_t)k * kOutPad + j]);

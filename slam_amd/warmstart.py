@@ -193,8 +193,9 @@ def he_init(layout, nparam: int, seed: int = 0) -> np.ndarray:
     off, n = layout[12]
     flat[off:off + n] = rng.normal(0.0, math.sqrt(2.0 / 128), size=n)
     off, n = layout[14]
-    W2 = np.zeros((256, 8), np.float32)
-    W2[:, :6] = rng.normal(0.0, math.sqrt(1.0 / 256), size=(256, 6))
+    hidden = n // 8
+    W2 = np.zeros((hidden, 8), np.float32)
+    W2[:, :6] = rng.normal(0.0, math.sqrt(1.0 / hidden), size=(hidden, 6))
     flat[off:off + n] = W2.ravel()
     return flat
 
