@@ -644,6 +644,7 @@ __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restr
     __shared__ __align__(16) bf16 bufB[kInferBufB];
     __shared__ float s_h1[kFc1Out];
     __shared__ float s_gap[kFc1In];
+    __shared__ float s_fc2[kInferThreads / 64][kOut];
     const int t = threadIdx.x;
     // the input, 16 bytes per thread and step
     const uint4* src = (const uint4*)(x + (size_t)blockIdx.x * kInferBufB);
@@ -667,37 +668,74 @@ __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restr
         s_gap[t] = bf2f(f2bf(sgap / 2.0f));
     }
     __syncthreads();
-    // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands;
-    // each thread forms its kFc1Out / kInferThreads hidden units side by side
-    {
-        constexpr int H = (kFc1Out + kInferThreads - 1) / kInferThreads;
-        float acc[H];
+    // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands.
+    // fc1 is a matrix-vector product over 983 KB of weights: each thread
+    // forms 4 consecutive hidden units from 8-byte weight loads, 32 rows of
+    // them in flight at a time; every unit sums its 128 terms in row order.
+    static_assert(kFc1Out % 4 == 0 && kFc1Out / 4 <= kInferThreads && kFc1In % 32 == 0, "fc1 blocking");
+    if (t < kFc1Out / 4) {
+        constexpr int G = kFc1Out / 4;
+        const uint2* Wv = (const uint2*)P.fc1W;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < kFc1In; k0 += 32) {
+            uint2 wv[32];
 #pragma unroll
-        for (int q = 0; q < H; ++q) acc[q] = 0.0f;
-        for (int k = 0; k < kFc1In; ++k) {
-            const float g = s_gap[k];
+            for (int kk = 0; kk < 32; ++kk) wv[kk] = Wv[(size_t)(k0 + kk) * G + t];
 #pragma unroll
-            for (int q = 0; q < H; ++q) {
-                const int o = t + q * kInferThreads;
-                if (o < kFc1Out) acc[q] += g * bf2f(P.fc1W[(size_t)k * kFc1Out + o]);
+            for (int kk = 0; kk < 32; ++kk) {
+                const float g = s_gap[k0 + kk];
+                acc[0] += g * bf2f((bf16)(wv[kk].x & 0xffffu));
+                acc[1] += g * bf2f((bf16)(wv[kk].x >> 16));
+                acc[2] += g * bf2f((bf16)(wv[kk].y & 0xffffu));
+                acc[3] += g * bf2f((bf16)(wv[kk].y >> 16));
             }
         }
 #pragma unroll
-        for (int q = 0; q < H; ++q) {
-            const int o = t + q * kInferThreads;
-            if (o < kFc1Out) s_h1[o] = bf2f(f2bf(fmaxf(acc[q] + P.fc1b[o], 0.0f)));
-        }
+        for (int q = 0; q < 4; ++q) s_h1[4 * t + q] = bf2f(f2bf(fmaxf(acc[q] + P.fc1b[4 * t + q], 0.0f)));
     }
     __syncthreads();
-    if (t < 64 * kOut) {
-        // one wave per output: lanes split the kFc1Out-term dot product, then
-        // a fixed-order wave sum
-        const int j = t >> 6, lane = t & 63;
-        float acc = 0.0f;
-        for (int k = lane; k < kFc1Out; k += 64) acc += s_h1[k] * bf2f(P.fc2W[(size_t)k * kOutPad + j]);
+    // fc2: every wave sums its slice of the hidden units for all six outputs
+    // (one 16-byte weight row per lane and step, all of a lane's rows in
+    // flight together), then the waves' sums in wave order
+    {
+        constexpr int NW = kInferThreads / 64, RPW = kFc1Out / NW, STEPS = (RPW + 63) / 64;
+        static_assert(kFc1Out % NW == 0 && kOutPad == 8 && kOut <= 8, "fc2 blocking");
+        const int wv = t >> 6, lane = t & 63;
+        const uint4* W8 = (const uint4*)P.fc2W;   // row k: the 8 (padded) outputs' bf16 weights
+        uint4 w[STEPS];
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (lane == 0) out[(size_t)blockIdx.x * kOut + j] = acc + P.fc2b[j];
+        for (int q = 0; q < STEPS; ++q) {
+            const int r = lane + 64 * q;
+            w[q] = W8[wv * RPW + (r < RPW ? r : 0)];
+        }
+        float acc[kOut];
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) acc[j] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < STEPS; ++q) {
+            const int r = lane + 64 * q;
+            if (r < RPW) {
+                const float h = s_h1[wv * RPW + r];
+                const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+#pragma unroll
+                for (int j = 0; j < kOut; ++j)
+                    acc[j] += h * bf2f((bf16)((j & 1) ? (wq[j >> 1] >> 16) : (wq[j >> 1] & 0xffffu)));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < kOut; ++j) s_fc2[wv][j] = acc[j];
+        __syncthreads();
+        if (t < kOut) {
+            float o = 0.0f;
+            for (int q = 0; q < NW; ++q) o += s_fc2[q][t];
+            out[(size_t)blockIdx.x * kOut + t] = o + P.fc2b[t];
+        }
     }
 }
 
