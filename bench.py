@@ -1042,8 +1042,8 @@ def main() -> int:
         dataio.write_pose_log(args.pose_log, list(range(1, args.steps + 1)), est)
 
     # roofline of the step kernels: per level, the last iteration of a
-    # completed call relaunched back to back between one HIP event pair on the
-    # library's stream (odom_profile_steps); the call itself with an event
+    # completed call relaunched back to back, one graph replay between one HIP
+    # event pair on the library's stream (odom_profile_steps); the call itself with an event
     # pair around every launch gives the end-to-end call time
     iters = [10, 5, 4]
     npx = [(cam.width >> l) * (cam.height >> l) for l in range(3)]

@@ -192,7 +192,8 @@ int odom_profile_track(odom_handle h, int rgb_only, float icp_weight, int pyrami
 /* Mean microseconds of the two step kernels per level as they run inside a
  * tracking call (ICP/association pass us_a, photometric pass us_b): after a
  * completed call with the same options, each level's last iteration is
- * relaunched reps times back to back between one event pair. */
+ * relaunched reps times back to back, as one graph replay between one event
+ * pair (so the host's launch cost leaves no gaps between the kernels). */
 int odom_profile_steps(odom_handle h, int rgb_only, float icp_weight, int reps, float us_a[3], float us_b[3]);
 /* Times the supported shapes on the handle's current inputs and keeps the
  * fastest per kernel and level. */

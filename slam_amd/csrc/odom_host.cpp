@@ -1172,19 +1172,39 @@ int odom_profile_steps(odom_handle c, int rgbOnly, float icpWeight, int reps, fl
         us_a[level] = us_b[level] = 0.0f;
         if (k <= 0) continue;
         const int iter = sched[k] % 64, prev = sched[k - 1] / 64;
-        for (int pass = 0; pass < (P.rgb ? 2 : 1); ++pass) {
-            for (int r = 0; r < reps + 2; ++r) {
-                if (r == 2) HIPCHK(c, hipEventRecord(e0, c->stream));
-                if (pass == 0) launch_step_a(step_launch(c, 0, level), level, iter, k, prev, P, 0, nullptr, c->stream);
-                else launch_step_b(step_launch(c, 1, level), level, iter, k, P, 0, c->stream);
+        for (int pass = 0; pass < (P.rgb ? 2 : 1) && rc == ODOM_OK; ++pass) {
+            // the relaunches are captured as one graph and replayed twice (the
+            // first replay warms up), so that the host's per-launch cost, a
+            // tracing profiler's included, leaves no idle time between them
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                for (int r = 0; r < reps; ++r) {
+                    if (pass == 0) launch_step_a(step_launch(c, 0, level), level, iter, k, prev, P, 0, nullptr, c->stream);
+                    else launch_step_b(step_launch(c, 1, level), level, iter, k, P, 0, c->stream);
+                }
+                e = hipStreamEndCapture(c->stream, &graph);
             }
-            HIPCHK(c, hipEventRecord(e1, c->stream));
-            HIPCHK(c, hipEventSynchronize(e1));
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.0f;
-            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (e != hipSuccess) {
+                c->last_hip = e;
+                fprintf(stderr, "slam_odom: odom_profile_steps: %s\n", hipGetErrorString(e));
+                rc = ODOM_ERR_HIP;
+                break;
+            }
             (pass == 0 ? us_a : us_b)[level] = ms * 1000.0f / reps;
         }
-        rc = check_launch(c);
+        if (rc == ODOM_OK) rc = check_launch(c);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -74,8 +74,9 @@ def cmd_frame(d: str, mode: str = "plain") -> str:
         if "k_batch" in names or b - a > 400:
             continue
         seeded = "k_seed" in names
-        so3 = "k_so3_iter" in names
-        if mode == "any" or (mode == "plain" and so3 and not seeded) or (mode == "learned" and seeded):
+        so3 = "k_so3_iter" in names or "k_so3_loop" in names
+        batched = "k_solve" in names and not seeded     # the batch solves in launches of its own
+        if mode == "any" or (mode == "plain" and so3 and not seeded and not batched) or (mode == "learned" and seeded):
             frames.append((a, b))
     i0, i1 = frames[len(frames) // 2]
     t0 = int(tr[i0]["End_Timestamp"])
@@ -89,6 +90,26 @@ def cmd_frame(d: str, mode: str = "plain") -> str:
     for k, (n, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
         out.append(f"| `{k}` | {n} | {us:.1f} | {us / n:.2f} |")
     out.append(f"\nOne tracking call (frame) end to end: {total:.1f} us, {i1 - i0} launches.")
+    return "\n".join(out)
+
+
+def cmd_runs(d: str, kernel: str, min_len: int = 40) -> str:
+    """Runs of at least min_len back-to-back launches of one step kernel (the
+    relaunches of odom_profile_steps): their mean kernel duration, the figure
+    the benchmark's event pair measures, and the mean spacing of their starts."""
+    tr = list(csv.DictReader(open(os.path.join(d, "run_kernel_trace.csv"))))
+    tr.sort(key=lambda r: int(r["Start_Timestamp"]))
+    out = ["| kernel | launches | avg us | start-to-start us |", "|---|---:|---:|---:|"]
+    i = 0
+    while i < len(tr):
+        j = i
+        while j < len(tr) and tr[j]["Kernel_Name"] == tr[i]["Kernel_Name"]:
+            j += 1
+        if j - i >= min_len and kernel in tr[i]["Kernel_Name"]:
+            dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in tr[i:j]]
+            gap = (int(tr[j - 1]["Start_Timestamp"]) - int(tr[i]["Start_Timestamp"])) / 1e3 / (j - i - 1)
+            out.append(f"| `{short(tr[i]['Kernel_Name'])}` | {j - i} | {sum(dur) / len(dur):.2f} | {gap:.2f} |")
+        i = j
     return "\n".join(out)
 
 
@@ -279,6 +300,10 @@ def main() -> int:
     f.add_argument("dir")
     f.add_argument("--mode", choices=("plain", "learned", "any"), default="plain",
                    help="plain: calls with the SO3 pass; learned: seeded calls")
+    rn = sub.add_parser("runs")
+    rn.add_argument("dir")
+    rn.add_argument("--kernel", default="k_step_")
+    rn.add_argument("--min", type=int, default=40)
     lp = sub.add_parser("loop")
     lp.add_argument("dir")
     lp.add_argument("--frames", type=int, nargs=2, default=(10, 200))
@@ -300,6 +325,8 @@ def main() -> int:
         print(cmd_stats(a.dir, a.top, a.command))
     elif a.cmd == "frame":
         print(cmd_frame(a.dir, a.mode))
+    elif a.cmd == "runs":
+        print(cmd_runs(a.dir, a.kernel, a.min))
     elif a.cmd == "loop":
         print(cmd_loop(a.dir, *a.frames))
     elif a.cmd == "kernel-traffic":
