@@ -63,7 +63,7 @@ const ShapeRow kShapeTable[] = {
 };
 // The batches' row: the 64-pair workload of uneven sizes (16 at 320x240, 32
 // at 640x480, 16 at 848x480) with 4 blocks per compute unit.
-const ShapeRow kBatchShapes = {0, 0, {{512, 2}, {256, 3}, {256, 1}}, {{512, 2}, {512, 2}, {512, 3}}};
+const ShapeRow kBatchShapes = {0, 0, {{512, 2}, {512, 2}, {256, 1}}, {{512, 3}, {256, 3}, {512, 3}}};
 
 StepShape table_shape(int w, int h, int kernel, int level)
 {

@@ -54,9 +54,9 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 // image) into LDS with all of a thread's loads in flight together: the
 // coordinates are clamped into the image, so every address is valid, and the
 // stores follow the last load.  An entry whose texel lies outside the image
-// holds 0 (ZERO_OUT) or its clamped neighbour.
-template <int N, int NT, bool ZERO_OUT, class T, class Load>
-__device__ __forceinline__ void stage_tile(T* tile, int ld, int ox, int oy, int w, int h, Load load)
+// holds `fill` (FILL_OUT) or its clamped neighbour.
+template <int N, int NT, bool FILL_OUT, class T, class Load>
+__device__ __forceinline__ void stage_tile(T* tile, int ld, int ox, int oy, int w, int h, Load load, T fill = T(0))
 {
     constexpr int R = (N + NT - 1) / NT;
     T v[R];
@@ -65,7 +65,7 @@ __device__ __forceinline__ void stage_tile(T* tile, int ld, int ox, int oy, int 
         const int i = min((int)threadIdx.x + r * NT, N - 1);
         const int gx = ox + i % ld, gy = oy + i / ld;
         v[r] = load(min(max(gx, 0), w - 1), min(max(gy, 0), h - 1));
-        if (ZERO_OUT && (gx < 0 || gx >= w || gy < 0 || gy >= h)) v[r] = T(0);
+        if (FILL_OUT && (gx < 0 || gx >= w || gy < 0 || gy >= h)) v[r] = fill;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -82,6 +82,11 @@ float bexpf_host(float x) { return bexpf(x); }
 // the product of a 13x13 spatial table and a range table over |value - tmp|
 // (factors of exp(-(space2*0.024691358 + color2*0.000555556))), both built
 // per block with bexpf.  Range factors for |value - tmp| >= 380 are 0.
+// Texels outside the image are staged as 0xFFFF, at least 380 from every
+// filtered value when the cut-off is below 65155, so they weigh exactly 0 and
+// every tile runs the unclipped 13x13 loop: a zero term leaves both sums'
+// bits unchanged, which equals the reference's clipped window.  Above that
+// cut-off, border tiles clip the window explicitly.
 constexpr int kBilatRange = 380;
 constexpr int kBT = 32;                 // output tile width
 constexpr int kBTY = 8;                 // output tile height (one pixel per thread)
@@ -111,36 +116,44 @@ __global__ __launch_bounds__(256) void k_bilateral(const uint16_t* __restrict__ 
                                                    uint16_t* __restrict__ dst, int w, int h,
                                                    unsigned int maxv)
 {
-    __shared__ uint16_t tile[kBLY * kBL];
+    // texels as floats (exact for 16-bit values): the tap's difference, its
+    // magnitude and the sum's factor then need no conversions of their own
+    __shared__ float tile[kBLY * kBL];
     __shared__ float sEC[kBilatRange + 1];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kBT, y0 = blockIdx.y * kBTY;
+    const bool unclipped = maxv < 65535u - kBilatRange ||
+                           (x0 >= 6 && y0 >= 6 && x0 + kBT + 6 <= w && y0 + kBTY + 6 <= h);
     stage_tile<kBLY * kBL, 256, true>(tile, kBL, x0 - 6, y0 - 6, w, h,
-                                      [&](int gx, int gy) { return gp(src)[(size_t)gy * pitch + gx]; });
+                                      [&](int gx, int gy) { return (float)gp(src)[(size_t)gy * pitch + gx]; },
+                                      65535.0f);
     for (int i = tid; i <= kBilatRange; i += 256)
         sEC[i] = (i < kBilatRange) ? bexpf(-((float)(i * i) * 0.000555556f)) : 0.0f;
     __syncthreads();
     const int lx = tid & 31, ly = tid >> 5;
     const int x = x0 + lx, y = y0 + ly;
     if (x >= w || y >= h) return;
-    const int value = tile[(ly + 6) * kBL + lx + 6];
+    const float vf = tile[(ly + 6) * kBL + lx + 6];
+    const int value = (int)vf;
     if ((unsigned int)value > maxv || value < 300) {
         dst[(size_t)y * w + x] = 0;
         return;
     }
+    // |value - tmp| clamped to the table, exactly as in integers: both are
+    // integers below 2^17, so the float difference is exact
+    auto range = [&](float t) { return sEC[(int)fminf(fabsf(vf - t), (float)kBilatRange)]; };
     float sum1 = 0.0f, sum2 = 0.0f;
-    if (x0 >= 6 && y0 >= 6 && x0 + kBT + 6 <= w && y0 + kBTY + 6 <= h) {
-        // interior tile: no window clipping, 13 columns unrolled per row
+    if (unclipped) {
+        // no window clipping, 13 columns unrolled per row
 #pragma unroll 1
         for (int dy = 0; dy < 13; ++dy) {
             const int trow = (ly + dy) * kBL + lx;
             const float* es = c_bilat_es + dy * 13;
 #pragma unroll
             for (int dx = 0; dx < 13; ++dx) {
-                const int t = tile[trow + dx];
-                const int a = min(abs(value - t), kBilatRange);
-                const float wgt = es[dx] * sEC[a];
-                sum1 = sum1 + (float)t * wgt;
+                const float t = tile[trow + dx];
+                const float wgt = es[dx] * range(t);
+                sum1 = sum1 + t * wgt;
                 sum2 = sum2 + wgt;
             }
         }
@@ -151,10 +164,9 @@ __global__ __launch_bounds__(256) void k_bilateral(const uint16_t* __restrict__ 
             const int trow = (cy - y0 + 6) * kBL - x0 + 6;
             const int erow = (cy - y + 6) * 13 + 6 - x;
             for (int cx = cx0; cx < cx1; ++cx) {
-                const int t = tile[trow + cx];
-                const int a = min(abs(value - t), kBilatRange);
-                const float wgt = c_bilat_es[erow + cx] * sEC[a];
-                sum1 = sum1 + (float)t * wgt;
+                const float t = tile[trow + cx];
+                const float wgt = c_bilat_es[erow + cx] * range(t);
+                sum1 = sum1 + t * wgt;
                 sum2 = sum2 + wgt;
             }
         }
@@ -1557,58 +1569,78 @@ __device__ __forceinline__ void load_tile_a(const LevelDesc& L, const TrackParam
     }
 }
 
+// The pose-dependent gathers of a tile: the ICP projections and their
+// gathers, then the photometric warps (computed while those gathers fly) and
+// theirs, all without branches (an index of -1 reads pixel 0, and a pass that
+// is off reads the vertex map; both are masked in rows_tile_a).
 template <int NT, int PPT>
-__device__ __forceinline__ void work_tile_a(const LevelDesc& L, const TrackParams& P, const IcpC& ic, const RgbC& rc,
-                                            int base, int N, size_t P3, const TileA<NT, PPT>& T,
+struct GatherA {
+    int j[PPT], jm[PPT];
+    float vg[PPT][3], vcp[PPT][3], tz[PPT], d0[PPT];
+    Px6 dst[PPT];
+    uint8_t il[PPT];
+};
+
+template <int NT, int PPT>
+__device__ __forceinline__ void gather_tile_a(const LevelDesc& L, const TrackParams& P, const IcpC& ic,
+                                              const RgbC& rc, int base, int N, size_t P3, const TileA<NT, PPT>& T,
+                                              GatherA<NT, PPT>& G)
+{
+    const float* vprev = P.icp ? L.vprev : L.vcur;
+    const float* nprev = P.icp ? L.nprev : L.vcur;
+    const float* zlast = P.rgb ? L.zlast : L.vcur;
+    const uint8_t* ilast = P.rgb ? L.ilast : (const uint8_t*)L.vcur;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) G.j[q] = icp_project(L, ic, T.src[q], G.vg[q], G.vcp[q]);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int j = G.j[q] >= 0 ? G.j[q] : 0;
+        G.dst[q].v[0] = gp(vprev)[j];
+        G.dst[q].v[1] = gp(vprev)[j + P3];
+        G.dst[q].v[2] = gp(vprev)[j + 2 * P3];
+        G.dst[q].n[0] = gp(nprev)[j];
+        G.dst[q].n[1] = gp(nprev)[j + P3];
+        G.dst[q].n[2] = gp(nprev)[j + 2 * P3];
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = base + q * NT;
+        const int y = idx / L.w, x = idx - y * L.w;
+        G.jm[q] = (idx < N) ? rgb_warp(L, rc, x, y, T.zs[q], G.tz[q]) : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int j = G.jm[q] >= 0 ? G.jm[q] : 0;
+        G.d0[q] = gp(zlast)[j];
+        G.il[q] = gp(ilast)[j];
+    }
+}
+
+template <int NT, int PPT>
+__device__ __forceinline__ void rows_tile_a(const LevelDesc& L, const IcpC& ic, const RgbC& rc, int base, int N,
+                                            const TileA<NT, PPT>& T, const GatherA<NT, PPT>& G,
                                             float (&acc)[kSE3], unsigned int& cnt, unsigned int& sig,
                                             uint8_t* __restrict__ dbg_mask)
 {
     unsigned int m[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) m[q] = 0;
-    if (P.icp) {
-        float vg[PPT][3], vcp[PPT][3];
-        int j[PPT];
-        Px6 dst[PPT];
 #pragma unroll
-        for (int q = 0; q < PPT; ++q) j[q] = icp_project(L, ic, T.src[q], vg[q], vcp[q]);
-#pragma unroll
-        for (int q = 0; q < PPT; ++q)
-            if (j[q] >= 0) icp_gather(L, j[q], P3, dst[q]);
-#pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            float row[7];
-            if (j[q] >= 0 && icp_row(ic, T.src[q], vg[q], vcp[q], dst[q], row)) {
-                acc_row7(acc, row);
-                m[q] = 1u;
-            }
+    for (int q = 0; q < PPT; ++q) {
+        float row[7];
+        if (G.j[q] >= 0 && icp_row(ic, T.src[q], G.vg[q], G.vcp[q], G.dst[q], row)) {
+            acc_row7(acc, row);
+            m[q] = 1u;
         }
     }
-    if (P.rgb) {
-        int jm[PPT];
-        float tz[PPT], d0[PPT];
-        uint8_t il[PPT];
 #pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            const int idx = base + q * NT;
-            const int y = idx / L.w, x = idx - y * L.w;
-            jm[q] = (idx < N) ? rgb_warp(L, rc, x, y, T.zs[q], tz[q]) : -1;
+    for (int q = 0; q < PPT; ++q)
+        if (G.jm[q] >= 0 && rgb_test(rc, G.tz[q], G.d0[q], G.il[q])) {
+            const float diff = (float)T.in[q] - (float)G.il[q];
+            cnt += 1u;
+            sig += (unsigned int)(int)(diff * diff);
+            m[q] |= 2u;
         }
-#pragma unroll
-        for (int q = 0; q < PPT; ++q)
-            if (jm[q] >= 0) {
-                d0[q] = gp(L.zlast)[jm[q]];
-                il[q] = gp(L.ilast)[jm[q]];
-            }
-#pragma unroll
-        for (int q = 0; q < PPT; ++q)
-            if (jm[q] >= 0 && rgb_test(rc, tz[q], d0[q], il[q])) {
-                const float diff = (float)T.in[q] - (float)il[q];
-                cnt += 1u;
-                sig += (unsigned int)(int)(diff * diff);
-                m[q] |= 2u;
-            }
-    }
     if (dbg_mask)
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -1673,10 +1705,26 @@ __global__ __launch_bounds__(NT) void k_step_a(const PairDesc* __restrict__ pair
 #pragma unroll
     for (int q = 0; q < kSE3; ++q) acc[q] = 0.0f;
     unsigned int cnt = 0, sig = 0;
-    for (int tile = lb; tile < ntiles; tile += nblk) {
-        const int base = tile * NT * PPT + threadIdx.x;
-        if (tile != lb) load_tile_a<NT, PPT>(L, P, base, N, P3, T);
-        work_tile_a<NT, PPT>(L, P, ic, rc, base, N, P3, T, acc, cnt, sig, dbg_mask);
+    GatherA<NT, PPT> G;
+    if (nblk >= ntiles) {
+        // at most one tile per block (single pairs)
+        if (lb < ntiles) {
+            const int base = lb * NT * PPT + threadIdx.x;
+            gather_tile_a<NT, PPT>(L, P, ic, rc, base, N, P3, T, G);
+            rows_tile_a<NT, PPT>(L, ic, rc, base, N, T, G, acc, cnt, sig, dbg_mask);
+        }
+    } else {
+        // several tiles per block (batches): the next tile's loads are
+        // issued behind this tile's gathers and fly while its rows are formed
+        for (int tile = lb; tile < ntiles; tile += nblk) {
+            const int base = tile * NT * PPT + threadIdx.x;
+            gather_tile_a<NT, PPT>(L, P, ic, rc, base, N, P3, T, G);
+            asm volatile("" ::: "memory");   // the gathers stay ahead of the next tile's loads
+            TileA<NT, PPT> Tn;
+            load_tile_a<NT, PPT>(L, P, base + nblk * NT * PPT, N, P3, Tn);
+            rows_tile_a<NT, PPT>(L, ic, rc, base, N, T, G, acc, cnt, sig, dbg_mask);
+            T = Tn;
+        }
     }
     const int pb = k & 1;
     block_partial<kSE3, NT>(acc, D.part_a[pb] + (size_t)lb * kPartStride, s_red);
@@ -1811,24 +1859,18 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
     float acc[kSE3];
 #pragma unroll
     for (int q = 0; q < kSE3; ++q) acc[q] = 0.0f;
-    for (int tile = lb; tile < ntiles; tile += nblk) {
-        const int base = tile * NT * PPT + threadIdx.x;
-        if (tile != lb) load_tile_b<NT, PPT>(L, base, N, T);
-        int jm[PPT];
-        float tz[PPT], d0[PPT];
-        uint8_t il[PPT];
+    int jm[PPT];
+    float tz[PPT], d0[PPT];
+    uint8_t il[PPT];
+    auto warp_tile = [&](int base) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int idx = base + q * NT;
             const int y = idx / L.w, x = idx - y * L.w;
             jm[q] = (idx < N) ? rgb_warp(L, rc, x, y, T.zs[q], tz[q]) : -1;
         }
-#pragma unroll
-        for (int q = 0; q < PPT; ++q)
-            if (jm[q] >= 0) {
-                d0[q] = gp(L.zlast)[jm[q]];
-                il[q] = gp(L.ilast)[jm[q]];
-            }
+    };
+    auto rows_tile = [&]() {
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
             if (jm[q] >= 0 && rgb_test(rc, tz[q], d0[q], il[q])) {
@@ -1837,6 +1879,38 @@ __global__ __launch_bounds__(NT) void k_step_b(const PairDesc* __restrict__ pair
                 rgb_row(L, sigma, P.sobelScale, T.gr[q], diff, d0[q], jm[q], row);
                 acc_row7(acc, row);
             }
+    };
+    if (nblk >= ntiles) {
+        // at most one tile per block (single pairs)
+        if (lb < ntiles) {
+            warp_tile(lb * NT * PPT + threadIdx.x);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q)
+                if (jm[q] >= 0) {
+                    d0[q] = gp(L.zlast)[jm[q]];
+                    il[q] = gp(L.ilast)[jm[q]];
+                }
+            rows_tile();
+        }
+    } else {
+        // several tiles per block (batches): the next tile's loads are
+        // issued behind this tile's gathers, without branches (past the end
+        // they read the last pixel), and fly while this tile's rows are formed
+        for (int tile = lb; tile < ntiles; tile += nblk) {
+            const int base = tile * NT * PPT + threadIdx.x;
+            warp_tile(base);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int j = jm[q] >= 0 ? jm[q] : 0;
+                d0[q] = gp(L.zlast)[j];
+                il[q] = gp(L.ilast)[j];
+            }
+            asm volatile("" ::: "memory");   // the gathers stay ahead of the next tile's loads
+            TileB<NT, PPT> Tn;
+            load_tile_b<NT, PPT>(L, base + nblk * NT * PPT, N, Tn);
+            rows_tile();
+            T = Tn;
+        }
     }
     block_partial<kSE3, NT>(acc, D.part_b[pb] + (size_t)lb * kPartStride, s_red);
 }

@@ -62,6 +62,25 @@ def test_bilateral_bit_exact(pair, orc):
     assert np.array_equal(pair["gpu"].bilateral(raw, 3.0), orc.bilateral(raw, 3.0))
 
 
+@pytest.mark.parametrize("maxd", [60.0, 65.1545, 70.0])
+def test_bilateral_border_tiles_bit_exact(orc, maxd):
+    """Border tiles at a ragged size.  Below a 65.155 m cut-off they stage
+    out-of-image texels as 0xFFFF, which is more than 380 from every filtered
+    value and so weighs exactly 0; above it they clip the window.  Either way
+    the result is the reference's clipped window bit for bit: uniform u16
+    noise, noise next to the cut-off, and a flat image at the largest value a
+    65.1545 m cut-off keeps (381 from the fill)."""
+    gpu = mk(CAM_ODD)
+    rng = np.random.default_rng(21)
+    shape = (CAM_ODD.height, CAM_ODD.width)
+    for raw in (rng.integers(0, 65536, size=shape, dtype=np.uint16),
+                rng.integers(64700, 65536, size=shape, dtype=np.uint16),
+                np.full(shape, 65154, np.uint16)):
+        ref = orc.bilateral(raw, maxd)
+        assert np.array_equal(gpu.bilateral(raw, maxd), ref), maxd
+    assert ref.min() == (65154 if maxd > 65.154 else 0)
+
+
 def test_depth_pyramid_and_maps_bit_exact(pair):
     gpu, ref = pair["gpu"], pair["ref"]
     for l in range(3):
