@@ -412,9 +412,9 @@ void launch_model_maps(const ModelLevels& a, hipStream_t s)
 // reversed weight index (ty-cy-1)*5+(tx-cx-1) are kept.
 // The 5x5 kernel {1 4 6 4 1} x {1 4 6 4 1} of utils.cu:453-457, entry
 // (r, c) computed as an exact product instead of a per-lane table lookup.
-__device__ __forceinline__ float gauss5(int k)
+__device__ __forceinline__ int gauss5i(int k)
 {
-    return k == 2 ? 6.0f : ((k & 1) ? 4.0f : 1.0f);
+    return k == 2 ? 6 : ((k & 1) ? 4 : 1);
 }
 
 // ===================================================== pyramid launches
@@ -441,6 +441,8 @@ __device__ __forceinline__ float gauss_f32_at(Get get, int sw, int sh, int x, in
     const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
     float v[25];
     window5(get, sw, sh, x, y, v);
+    // the reference adds each float weight to an int count, truncating; the
+    // weights are integers, so an integer sum is the same count
     float sum = 0.0f;
     int count = 0;
 #pragma unroll
@@ -450,9 +452,9 @@ __device__ __forceinline__ float gauss_f32_at(Get get, int sw, int sh, int x, in
             const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
             const float val = v[dy * 5 + dx];
             if (cy >= 0 && cy < ty && cx >= 0 && cx < tx && !isnan(val)) {
-                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + val * gk;
-                count = (int)((float)count + gk);
+                const int gk = gauss5i(ty - cy - 1) * gauss5i(tx - cx - 1);
+                sum = sum + val * (float)gk;
+                count += gk;
             }
         }
     return sum / (float)count;
@@ -464,21 +466,23 @@ __device__ __forceinline__ uint8_t gauss_u8_at(Get get, int sw, int sh, int x, i
     const int tx = min(2 * x + 3, sw - 1), ty = min(2 * y + 3, sh - 1);
     uint8_t v[25];
     window5(get, sw, sh, x, y, v);
-    float sum = 0.0f;
-    int count = 0;
+    // every term val * gk is an integer below 2^14 and their sum stays below
+    // 2^24, so the reference's float sum is exact in any order: integer sums
+    // give the same bits
+    int sum = 0, count = 0;
 #pragma unroll
     for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
             const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
-            const uint8_t val = v[dy * 5 + dx];
+            const int val = v[dy * 5 + dx];
             if (cy >= 0 && cy < ty && cx >= 0 && cx < tx && val > 0) {
-                const float gk = gauss5(ty - cy - 1) * gauss5(tx - cx - 1);
-                sum = sum + (float)val * gk;
-                count = (int)((float)count + gk);
+                const int gk = gauss5i(ty - cy - 1) * gauss5i(tx - cx - 1);
+                sum += val * gk;
+                count += gk;
             }
         }
-    const float q = sum / (float)count;
+    const float q = (float)sum / (float)count;
     return (count == 0) ? (uint8_t)0 : (uint8_t)q;
 }
 
@@ -488,23 +492,27 @@ __device__ __forceinline__ uint8_t gauss_u8_at(Get get, int sw, int sh, int x, i
 template <class Get>
 __device__ __forceinline__ uint16_t pyr_u16_at(Get get, int sw, int sh, int x, int y)
 {
-    const float wts[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    // In 1/256 units the weights are the integers {1 4 6 4 1} x {1 4 6 4 1}:
+    // each float term val * w * w is exact, and the sums stay below 2^24
+    // units, so the reference's float sums are exact in any order and equal
+    // the integer sums; scaling both by 2^-8 leaves their quotient unchanged.
     int v[25];
     window5(get, sw, sh, x, y, v);
     const int center = v[12];
-    float sum = 0.0f, wall = 0.0f;
+    int sum = 0, wall = 0;
 #pragma unroll
     for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
             const int cy = 2 * y - 2 + dy, cx = 2 * x - 2 + dx;
             const int val = v[dy * 5 + dx];
-            if (cy >= 0 && cy < sh && cx >= 0 && cx < sw && (float)abs(val - center) < 90.0f) {
-                sum = sum + (float)val * wts[dx] * wts[dy];
-                wall = wall + wts[dx] * wts[dy];
+            if (cy >= 0 && cy < sh && cx >= 0 && cx < sw && abs(val - center) < 90) {
+                const int wk = gauss5i(dx) * gauss5i(dy);
+                sum += val * wk;
+                wall += wk;
             }
         }
-    return (uint16_t)(int)(sum / wall);
+    return (uint16_t)(int)((float)sum / (float)wall);
 }
 
 // Levels 0 (when generated), 1 and 2 of up to two pyramids in one launch,

@@ -13,13 +13,15 @@
 //
 // Here that is two launches and no rasterizer:
 //   k_splat_raster   one thread per surfel: the vertex stage, then every
-//                    pixel centre of the sprite square (exact bounds); a
-//                    fragment that passes the disk test offers
+//                    pixel centre of the sprite square (exact bounds) that
+//                    the disk's window-space extent can reach (disk_bounds);
+//                    a fragment that passes the disk test offers
 //                    ((24-bit depth) << 32 | surfel index) to the pixel with a
 //                    64-bit atomic minimum, which is GL_LESS in draw order: the
 //                    nearest depth wins and, among equal depths, the lowest
-//                    index (the first drawn).  The ray through each pixel
-//                    centre is read from a per-handle table (k_rays);
+//                    index (the first drawn); key_min skips the atomics that
+//                    cannot win.  The ray through each pixel centre is read
+//                    from a per-handle table (k_rays);
 //   k_splat_resolve  one thread per pixel: reads the winning key, clears it
 //                    for the next prediction, and recomputes that surfel's
 //                    fragment at this pixel to write the four outputs.
@@ -165,6 +167,66 @@ __device__ __forceinline__ bool fragment(const Surf& s, float num, float maxDept
 
 __device__ __forceinline__ float plane_num(const Surf& s) { return (s.hx * s.nx + s.hy * s.ny) + s.hz * s.nz; }
 
+// A bound on the pixel centres whose fragment can pass: the window-space
+// extent of the surfel's disk (centre h, normal n, radius rad).  The disk is
+// h + rad (cos p u + sin p v) for an orthonormal pair u, v of its plane;
+// x = fx Cx / Cz + cx is extremal where A sin p + B cos p + K = 0 (A, B, K
+// below, the same for y), which has the two solutions taken here.  A fragment
+// passes only when its ray meets the disk up to the fragment test's FP32
+// rounding, about 1e-4 pixel in window space, and this FP32 evaluation (with
+// hardware reciprocals and square roots) is within about 3e-4 pixel of the
+// exact extent, so the bound widened by kDiskMargin holds every passing
+// centre.  False (no bound: the sprite square alone is used) for views
+// within about 3 degrees of the plane and for radii above 2 % of the distance
+// (rays that meet such disks may graze the plane, where the test's rounding
+// grows), for disks that reach z <= 1 % of the centre's depth (their
+// projections are not bounded ellipses), and where the extremes are poorly
+// conditioned (D < R2 / 2; with the radius bound D is above 0.999 R2).
+constexpr float kDiskMargin = 0.05f;
+
+__device__ bool disk_bounds(const Surf& s, const SplatCam& K, float& x0, float& x1, float& y0, float& y1)
+{
+    const float h[3] = {s.hx, s.hy, s.hz};
+    const float inl = __builtin_amdgcn_rsqf(s.nx * s.nx + s.ny * s.ny + s.nz * s.nz);
+    const float n[3] = {s.nx * inl, s.ny * inl, s.nz * inl};
+    const float r = s.rad;
+    const float hl = __builtin_amdgcn_sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    if (!(fabsf(h[0] * n[0] + h[1] * n[1] + h[2] * n[2]) >= 0.05f * hl) || !(r <= 0.02f * hl)) return false;
+    if (!(h[2] - r * __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - n[2] * n[2])) > 0.01f * h[2])) return false;
+    // u = n x e (e the axis least aligned with n), v = n x u
+    const bool ax = fabsf(n[0]) < 0.9f;
+    float u[3] = {ax ? 0.0f : -n[2], ax ? n[2] : 0.0f, ax ? -n[1] : n[0]};
+    const float iul = __builtin_amdgcn_rsqf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    u[0] *= iul, u[1] *= iul, u[2] *= iul;
+    const float v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+    float lo[2], hi[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const float A = h[c] * u[2] - h[2] * u[c];
+        const float B = h[2] * v[c] - h[c] * v[2];
+        const float Kc = r * (v[c] * u[2] - u[c] * v[2]);
+        const float R2 = A * A + B * B;
+        const float D = R2 - Kc * Kc;
+        if (!(D > 0.5f * R2)) return false;
+        const float sd = __builtin_amdgcn_sqrtf(D), iR2 = __builtin_amdgcn_rcpf(R2);
+        const float f = c == 0 ? K.fx : K.fy, ctr = c == 0 ? K.cx : K.cy;
+        float e[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float sg = q == 0 ? 1.0f : -1.0f;
+            const float sn = (-Kc * A + sg * B * sd) * iR2, cs = (-Kc * B - sg * A * sd) * iR2;
+            const float Cc = h[c] + r * (cs * u[c] + sn * v[c]);
+            const float Cz = h[2] + r * (cs * u[2] + sn * v[2]);
+            e[q] = f * Cc * __builtin_amdgcn_rcpf(Cz) + ctr;
+        }
+        if (!(fabsf(e[0]) < 1e6f && fabsf(e[1]) < 1e6f)) return false;   // also NaN
+        lo[c] = fminf(e[0], e[1]) - kDiskMargin;
+        hi[c] = fmaxf(e[0], e[1]) + kDiskMargin;
+    }
+    x0 = lo[0], x1 = hi[0], y0 = lo[1], y1 = hi[1];
+    return true;
+}
+
 // The pixel centres c = i + 0.5 of the sprite: lo <= c < hi.  Window
 // coordinates stay below 2^13 and the size at most 256, so lo - 0.5 is exact
 // and the ceilings give the bounds; the checks only guard that reasoning.
@@ -182,6 +244,17 @@ __device__ __forceinline__ int last_centre(float hi)
     if ((float)(i + 1) + 0.5f < hi) i += 1;
     else if ((float)i + 0.5f >= hi) i -= 1;
     return i;
+}
+
+// GL_LESS as a 64-bit minimum.  Keys only decrease while a raster runs, so
+// when a coherent (device-scope) read of the key, which returns one of its
+// values in update order, is already at or below v, the minimum would change
+// nothing and the atomic is skipped.  Most fragments lose to one drawn
+// earlier, so most of the atomics, which device scope performs beyond the
+// compute die's cache, are saved; the result is the same.
+__device__ __forceinline__ void key_min(unsigned long long* key, unsigned long long v)
+{
+    if (__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > v) atomicMin(key, v);
 }
 
 __global__ __launch_bounds__(256) void k_rays(SplatCam K, float4* rays)
@@ -217,9 +290,17 @@ __global__ __launch_bounds__(256) void k_splat_raster(const float4* __restrict__
         if (surfel_stage(model[b], model[b + 1], model[b + 2], U, s) && sprite(s, U.cam, xw, yw, size)) {
             const float half = size * 0.5f;
             i0 = max(0, first_centre(xw - half));
-            const int i1 = min(U.cam.w - 1, last_centre(xw + half));
+            int i1 = min(U.cam.w - 1, last_centre(xw + half));
             j0 = max(0, first_centre(yw - half));
-            const int j1 = min(U.cam.h - 1, last_centre(yw + half));
+            int j1 = min(U.cam.h - 1, last_centre(yw + half));
+            float bx0, bx1, by0, by1;
+            if (disk_bounds(s, U.cam, bx0, bx1, by0, by1)) {
+                // centres c = i + 0.5 inside [b0, b1] (|b| < 1e6, so the conversions are exact)
+                i0 = max(i0, (int)ceilf(bx0 - 0.5f));
+                i1 = min(i1, (int)floorf(bx1 - 0.5f));
+                j0 = max(j0, (int)ceilf(by0 - 0.5f));
+                j1 = min(j1, (int)floorf(by1 - 0.5f));
+            }
             if (i1 >= i0 && j1 >= j0) {
                 ncols = i1 - i0 + 1;
                 cnt = ncols * (j1 - j0 + 1);
@@ -235,7 +316,7 @@ __global__ __launch_bounds__(256) void k_splat_raster(const float4* __restrict__
         unsigned q;
         if (!fragment(f, fnum, U.maxDepth, rays[p], z, q)) return;
         ++drawn;
-        atomicMin(keys + p, ((unsigned long long)q << 32) | idx);
+        key_min(keys + p, ((unsigned long long)q << 32) | idx);
     };
     if (cnt > 0 && cnt <= kBigSprite) {
         const int j1 = j0 + cnt / ncols - 1, i1 = i0 + ncols - 1;

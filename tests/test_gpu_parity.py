@@ -349,18 +349,37 @@ def test_call_order_is_checked():
 
 
 def test_step_shapes_agree(pair):
-    """Every launch shape of the autotuner gives the same mask and sums."""
+    """Every launch shape of the autotuner gives the same ICP mask and sums,
+    association mask and counts, and photometric sums.  The shapes cover both
+    loop forms of the step kernels: one tile per block, and several (a level
+    0 of 640x480 has 1200 tiles of 256 pixels for at most 256 blocks, which
+    then load each next tile behind the current gathers)."""
     gpu, cam = pair["gpu"], pair["cam"]
+    saved = [gpu.get_step_shape(k, 0) for k in (0, 1)]
     T = pair["T1"]
     Rc, tc, Rpi, tp = _constants(pair["T0"], T, 0, cam)
     base, mb = gpu.icp_step(0, Rc, tc, Rpi, tp)
+    gpu.compute_derivatives()
+    Trel = np.linalg.inv(pair["T0"]) @ T
+    Trel[:3, 3] *= 0.5
+    krkinv, kt = _warp_consts(Trel, cam, 0)
+    cnt0, sig0, mr0 = gpu.rgb_residual(0, krkinv, kt)
+    sigma = float(np.float32(math.sqrt(cnt0)))
+    rgb0 = gpu.rgb_step(0, sigma, krkinv, kt)
+    assert cnt0 > 50
     for t in (256, 512):
         for q in (1, 2, 3, 4):
             gpu.set_step_shape(0, 0, t, q)
+            gpu.set_step_shape(1, 0, t, q)
             out, m = gpu.icp_step(0, Rc, tc, Rpi, tp)
             assert np.array_equal(m, mb)
             assert sums_close(out, base) < SUM_RTOL
-    gpu.set_step_shape(0, 0, 256, 2)
+            cnt, sig, mr = gpu.rgb_residual(0, krkinv, kt)
+            assert (cnt, sig) == (cnt0, sig0) and np.array_equal(mr, mr0), (t, q)
+            out = gpu.rgb_step(0, sigma, krkinv, kt)
+            assert out[28] == rgb0[28] and sums_close(out, rgb0) < SUM_RTOL, (t, q)
+    for k in (0, 1):
+        gpu.set_step_shape(k, 0, *saved[k])
 
 
 def test_device_memory_inputs(orc):
