@@ -395,14 +395,16 @@ void prep_levels(odom_ctx* c, PrepLevels& a)
 // step launch (ev[2*i], ev[2*i+1] for launch i, A and B alternating).
 void track_launches(odom_ctx* c, const TrackParams& P, hipEvent_t* ev)
 {
-    if (P.rgb) {
-        PrepLevels a{};
-        prep_levels(c, a);
-        launch_rgb_prep(a, c->stream);
-    }
+    PrepLevels a{};
+    if (P.rgb) prep_levels(c, a);
     PairDesc* dp = c->d_pair[c->parity];
-    if (P.so3) launch_so3(dp, 1, &P, c->so3_spin, c->stream);  // begin rides on the SO3 loop's end
-    else launch_track_begin(dp, 1, P, c->stream);
+    if (P.so3) {
+        // begin rides on the SO3 loop's end, the derivatives on its spare blocks
+        launch_so3(dp, 1, &P, c->so3_spin, c->stream, P.rgb ? &a : nullptr);
+    } else {
+        if (P.rgb) launch_rgb_prep(a, c->stream);
+        launch_track_begin(dp, 1, P, c->stream);
+    }
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);

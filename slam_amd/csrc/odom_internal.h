@@ -95,8 +95,11 @@ struct StepLaunch {
 // SO3 pre-alignment; max_px_l2: largest level-2 pixel count over the pairs
 // The SO3 loop; with begin, its last launch also does launch_track_begin's work.
 // spin_limit < 0: one launch per iteration; else the one-launch loop (when
-// the pairs fit), whose barrier waits give up after spin_limit polls.
-void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s);
+// the pairs fit), whose barrier waits give up after spin_limit polls.  With
+// prep (one pair), launch_rgb_prep's work is done too: by extra blocks of the
+// one-launch loop, or by a launch of its own before the per-iteration form.
+void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s,
+                const PrepLevels* prep = nullptr);
 void launch_track_begin(const PairDesc* pairs, int npairs, const TrackParams& P, hipStream_t s);
 // warm start: seed from the network prediction (6 floats per pair, device),
 // optionally kept only when the level-2 ICP association is not worse than plain

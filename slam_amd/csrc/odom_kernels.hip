@@ -2746,10 +2746,20 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_iter(const PairDesc* __rest
 // spin_limit polls; a wait that gives up sets the sticky so3_fail flag and
 // its workgroup leaves the loop, so every workgroup exits.  The last
 // workgroup to exit sets the counters back to 0 for the next call.  Only
-// launched when the pairs' workgroups fit on the device at once.
+// launched when the pairs' workgroups fit on the device at once.  Blocks from
+// nloop on compute the derivative images and photometric masks of one pair
+// (k_rgb_prep's work, `prep`, segments `pseg` of kSO3Threads pixels), which
+// the loop does not read; they come after the loop's blocks in dispatch order
+// and take no part in its barrier.
 __global__ __launch_bounds__(kSO3Threads) void k_so3_loop(const PairDesc* __restrict__ pairs, int begin,
-                                                          TrackParams P, int spin_limit)
+                                                          TrackParams P, int spin_limit, int nloop,
+                                                          PrepLevels prep, Seg3 pseg)
 {
+    if ((int)blockIdx.x >= nloop) {
+        const int b = (int)blockIdx.x - nloop, l = seg_level(pseg, b);
+        rgb_prep_px(prep, l, (b - pseg.b0[l]) * kSO3Threads + threadIdx.x);
+        return;
+    }
     __shared__ So3Slot S;
     __shared__ double s_tot[kSO3];
     __shared__ float s_red[kSO3Threads / 64][kSO3];
@@ -2888,14 +2898,25 @@ __global__ __launch_bounds__(kSO3Threads) void k_so3_loop(const PairDesc* __rest
 // and ~2 KB of LDS); larger batches take the launch-per-iteration form.
 constexpr int kSO3LoopMaxBlocks = 256;
 
-void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s)
+void launch_so3(const PairDesc* pairs, int npairs, const TrackParams* begin, int spin_limit, hipStream_t s,
+                const PrepLevels* prep)
 {
     const TrackParams P = begin ? *begin : TrackParams{};
     if (spin_limit >= 0 && npairs * kSO3Blocks <= kSO3LoopMaxBlocks) {
-        hipLaunchKernelGGL(k_so3_loop, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, begin ? 1 : 0, P,
-                           spin_limit);
+        const int nloop = npairs * kSO3Blocks;
+        PrepLevels a{};
+        Seg3 g{};
+        if (prep) {
+            a = *prep;
+            int n[kLevels];
+            for (int l = 0; l < kLevels; ++l) n[l] = a.w[l] * a.h[l];
+            g = make_seg(n, kSO3Threads);
+        }
+        hipLaunchKernelGGL(k_so3_loop, dim3(nloop + (prep ? g.b0[kLevels] : 0)), dim3(kSO3Threads), 0, s, pairs,
+                           begin ? 1 : 0, P, spin_limit, nloop, a, g);
         return;
     }
+    if (prep) launch_rgb_prep(*prep, s);
     for (int it = 0; it <= 10; ++it)
         hipLaunchKernelGGL(k_so3_iter, dim3(npairs * kSO3Blocks), dim3(kSO3Threads), 0, s, pairs, it,
                            (it == 10 && begin) ? 1 : 0, P);
