@@ -407,6 +407,11 @@ struct cnn_net_s {
     size_t nparam = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16* pb = nullptr;
+    // the conv weights in the inference kernel's fragment order (k_pack_infer),
+    // repacked when the weights have changed since the last pack
+    bf16* pw = nullptr;
+    size_t pw_off[kConvs] = {};
+    unsigned long long wver = 1, pver = 0;
     uint8_t* mask = nullptr;
     int step = 0;
     // activations of the last forward (for backward), sized for max_batch
@@ -551,7 +556,7 @@ constexpr int kInferBufA = 30 * 40 * 32;     // elements: the largest output (la
 constexpr int kInferBufB = 60 * 80 * 4;      // the input; also layer 2's output (15 x 20 x 64)
 
 struct InferParams {
-    const bf16* W[kConvs];                   // [Kp][Cout] bf16
+    const bf16* W[kConvs];                   // fragment order (k_pack_infer) bf16
     const float* b[kConvs];
     int Kp[kConvs];
     const bf16* fc1W;                        // [128][3840]
@@ -560,13 +565,35 @@ struct InferParams {
     const float* fc2b;
 };
 
+// The conv weights [Kp][Cout] of every layer in the inference kernel's
+// fragment order (grid.y = layer).
+struct PackArgs {
+    const bf16* src[kConvs];
+    bf16* dst[kConvs];
+    size_t n[kConvs];
+    int nt[kConvs], cout[kConvs];
+};
+
+__global__ __launch_bounds__(256) void k_pack_infer(PackArgs A)
+{
+    const int l = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= A.n[l]) return;
+    const int q = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const size_t r = e >> 8;
+    const int nt = (int)(r % A.nt[l]), ks = (int)(r / A.nt[l]);
+    const int k = ks * 16 + 4 * (lane >> 4) + q, n = nt * 16 + (lane & 15);
+    A.dst[l][e] = A.src[l][(size_t)k * A.cout[l] + n];
+}
+
 // One convolution layer (3x3, stride 2, pad 1) from X (LDS, [HI][WI][CIN])
 // into Y (LDS, [HO][WO][COUT]).  Each wave takes (column tile, up to four row
-// tiles) work items.  The weight fragments of kGroup consecutive 16-deep k
-// steps are loaded together before their MFMAs, so an item waits for one
-// memory latency per group rather than per step; each fragment serves all the
-// item's row tiles.
-constexpr int kGroup = 8;
+// tiles) work items.  The weight fragments of up to kGroup consecutive
+// 16-deep k steps are loaded together before their MFMAs, so an item waits
+// for one memory latency per group rather than per step; each fragment serves
+// all the item's row tiles.  W is in fragment order: step ks, column tile nt,
+// lane L hold W[16 ks + 4 (L >> 4) + q][16 nt + (L & 15)], q = 0..3.
+constexpr int kGroup = 24;
 
 template <int CIN, int COUT, int HI, int WI, int HO, int WO>
 __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __restrict__ Y, const bf16* __restrict__ W,
@@ -574,6 +601,7 @@ __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __r
 {
     constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16, CH = 4;
     constexpr int KREAL = 9 * CIN, KP = ((KREAL + BK - 1) / BK) * BK, KS = KP / 16;
+    constexpr int GR = KS < kGroup ? KS : kGroup;
     static_assert(COUT % 16 == 0 && CIN % 4 == 0 && KP % 16 == 0, "tile shapes");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int WAVES = kInferThreads / 64;
@@ -595,20 +623,15 @@ __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __r
         v4f acc[CH];
 #pragma unroll
         for (int i = 0; i < CH; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
-        for (int g0 = 0; g0 < KS; g0 += kGroup) {
-            v4s b[kGroup];
+        for (int g0 = 0; g0 < KS; g0 += GR) {
+            // one 8-byte fragment per lane and step (the wave reads 512
+            // contiguous bytes), all of the group's steps in flight together
+            v4s b[GR];
 #pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                const int kq = (g0 + g) * 16 + rq;
-                if (g0 + g < KS) {
-                    bf16 t4[4];
+            for (int g = 0; g < GR; ++g)
+                if (g0 + g < KS) b[g] = *(const v4s*)(W + ((size_t)((g0 + g) * NT + nt) * 64 + lane) * 4);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) t4[q] = W[(size_t)(kq + q) * COUT + n];
-                    b[g] = v4s{(short)t4[0], (short)t4[1], (short)t4[2], (short)t4[3]};
-                }
-            }
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
+            for (int g = 0; g < GR; ++g) {
                 if (g0 + g >= KS) break;
                 const int kq = (g0 + g) * 16 + rq;
                 const int c = kq % CIN, tap = kq / CIN;
@@ -774,6 +797,12 @@ int cnn_create(int max_batch, cnn_net* out)
     rc = rc ? rc : dalloc(c, &c->v, off);
     rc = rc ? rc : dalloc(c, &c->pb, off);
     rc = rc ? rc : dalloc(c, &c->mask, off);
+    size_t pw_n = 0;
+    for (int l = 0; l < kConvs; ++l) {
+        c->pw_off[l] = pw_n;
+        pw_n += (size_t)c->Kp[l] * kSpec[l].cout;
+    }
+    rc = rc ? rc : dalloc(c, &c->pw, pw_n);
     for (int l = 0; l < kConvs && !rc; ++l) {
         const size_t M = (size_t)N * kSpec[l].ho * kSpec[l].wo;
         rc = dalloc(c, &c->xcol[l], M * c->Kp[l]);
@@ -865,6 +894,7 @@ int cnn_set_params(cnn_net c, const float* host, size_t n)
     if (!c || !host || n != c->nparam) return CNN_ERR_ARG;
     CK(hipMemcpy(c->p, host, n * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_to_bf16, g1(n), dim3(256), 0, c->stream, c->p, n, c->pb);
+    c->wver++;
     CK(hipMemset(c->m, 0, n * 4));
     CK(hipMemset(c->v, 0, n * 4));
     c->step = 0;
@@ -934,9 +964,23 @@ int cnn_infer(cnn_net c, const void* x, int N, float* out)
 {
     if (!c || !x || !out || N < 1 || N > c->max_batch) return CNN_ERR_ARG;
     if ((uintptr_t)x % 16 != 0) return CNN_ERR_ARG;
+    if (c->pver != c->wver) {
+        PackArgs A;
+        size_t n = 0;
+        for (int l = 0; l < kConvs; ++l) {
+            A.src[l] = c->pb + c->convW[l].off;
+            A.dst[l] = c->pw + c->pw_off[l];
+            A.n[l] = (size_t)c->Kp[l] * kSpec[l].cout;
+            A.nt[l] = kSpec[l].cout / 16;
+            A.cout[l] = kSpec[l].cout;
+            n = std::max(n, A.n[l]);
+        }
+        hipLaunchKernelGGL(k_pack_infer, dim3((unsigned)((n + 255) / 256), kConvs), dim3(256), 0, c->stream, A);
+        c->pver = c->wver;
+    }
     InferParams P;
     for (int l = 0; l < kConvs; ++l) {
-        P.W[l] = c->pb + c->convW[l].off;
+        P.W[l] = c->pw + c->pw_off[l];
         P.b[l] = c->p + c->convB[l].off;
         P.Kp[l] = c->Kp[l];
     }
@@ -1012,6 +1056,7 @@ int cnn_adam_step(cnn_net c, float lr, float wd)
     const float bc1 = 1.0f - powf(b1, (float)c->step), bc2 = 1.0f - powf(b2, (float)c->step);
     hipLaunchKernelGGL(k_adam, g1(c->nparam), dim3(256), 0, c->stream, c->p, c->g, c->m, c->v, c->pb, c->mask,
                        c->nparam, lr, b1, b2, eps, bc1, bc2, wd);
+    c->wver++;
     return launch_check("adam");
 }
 

@@ -63,6 +63,31 @@ def test_one_launch_loop_equals_launch_per_iteration(orc, cam, seed):
         assert synth.pose_error(a[k], poses[k + 1])[0] < 0.01, k
 
 
+def test_loop_blocks_compute_the_derivatives(orc):
+    """The derivative images and photometric source masks that spare blocks
+    of the one-launch loop compute equal the standalone derivative pass's, bit
+    for bit, at every level (a ragged size, so the blocks' last segment is
+    partial)."""
+    cam = synth.Camera(202, 154, 150.0, 150.0, 100.5, 76.5)
+    poses, frames = sequence(cam, 66, n=2)
+    filt = orc.bilateral(frames[1].depth, MAXD)
+    ref, loop = mk(cam), mk(cam)
+    for o in (ref, loop):
+        o.set_pose(poses[0][:3, 3], poses[0][:3, :3])
+        o.initFirstRGB(frames[0].rgb)
+        feed_frame(o, frames, poses, 1, filt)
+    ref.compute_derivatives()
+    want = [(ref.read_buffer("dxy", l), ref.read_buffer("zsrc", l)) for l in range(3)]
+    loop.track_async(False, 10.0, True, False, True)
+    loop.synchronize()
+    # the tracking call swapped the image pair afterwards; the derivative
+    # buffers are not swapped
+    for l in range(3):
+        dxy, zsrc = loop.read_buffer("dxy", l), loop.read_buffer("zsrc", l)
+        assert np.array_equal(dxy, want[l][0]), l
+        assert np.array_equal(zsrc.view(np.uint32), want[l][1].view(np.uint32)), l
+
+
 def test_batched_one_launch_loop_equals_per_pair_calls(orc):
     """Three pairs of different sizes in one batch (48 workgroups of the
     loop, one barrier counter per pair): each pair's pose equals its own
