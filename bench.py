@@ -940,6 +940,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-pair measurement")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched 64-pair measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 320x240 ICP-only measurement (whose level 0 has the grid of a 640x480 level 1)")
     ap.add_argument("--burst-frames", type=int, default=150)
     ap.add_argument("--no-graph", action="store_true", help="launch tracking calls kernel by kernel")
     ap.add_argument("--pose-log", default=None, help="write the timed frames' poses in the frame loop's log format")
@@ -1097,7 +1099,7 @@ def main() -> int:
                     "vs_fp32_oracle_max_mm": error_stats(est[:n], ref32, "x")["x_max_mm"]}
         log(f"accuracy {accuracy}")
 
-    c1 = c1_gpu(args)
+    c1 = None if args.no_c1 else c1_gpu(args)
     log(f"C1 {c1}")
     learned = None if args.no_learned else learned_vs_plain(args, seq, L, odo, ref64)
     del seq

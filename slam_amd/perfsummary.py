@@ -168,10 +168,20 @@ def _level(name: str, grid: int, wg: int) -> int | None:
     return None
 
 
+def _calibration_copies(rows) -> list:
+    """Counter values of the four 256 MiB calibration copies of
+    ``bench.py --calibrate-copy``: the first four device copies of the run in
+    dispatch order (the copy-peak measurement that may follow uses the same
+    blit grid on 1 GiB buffers)."""
+    cp = [r for r in rows if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    if cp and "Dispatch_Id" in cp[0]:
+        cp.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return [float(r["Counter_Value"]) for r in cp[:4]]
+
+
 def _per_level(path: str):
     rows = list(csv.DictReader(open(path)))
-    copy = [float(r["Counter_Value"]) for r in rows
-            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    copy = _calibration_copies(rows)
     lv = defaultdict(list)
     for r in rows:
         if "k_step_a" not in r["Kernel_Name"] or "true>" not in r["Kernel_Name"]:
@@ -206,8 +216,7 @@ def cmd_traffic(fetch_dir: str, write_dir: str) -> dict:
 
 def _kernel_counter(path: str, name: str):
     rows = list(csv.DictReader(open(path)))
-    copy = [float(r["Counter_Value"]) for r in rows
-            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    copy = _calibration_copies(rows)
     vals = [float(r["Counter_Value"]) for r in rows if name in r["Kernel_Name"]]
     if not copy or not vals:
         raise ValueError(f"no copy calibration or no {name} dispatches in {path}")
@@ -239,8 +248,7 @@ def _batch_levels(path: str, kernel: str):
     unfused pass A or pass B), by level: every batched call launches each
     pass 19 times in the schedule's order (4 at level 2, 5 at 1, 10 at 0)."""
     rows = list(csv.DictReader(open(path)))
-    copy = [float(r["Counter_Value"]) for r in rows
-            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072]
+    copy = _calibration_copies(rows)
     if "Dispatch_Id" in rows[0]:
         rows.sort(key=lambda r: int(r["Dispatch_Id"]))
     if kernel == "a":
