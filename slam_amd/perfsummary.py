@@ -5,7 +5,7 @@
     python -m slam_amd.perfsummary frame  DIR
         kernels of one mid-run tracking call (DIR/run_kernel_trace.csv)
     python -m slam_amd.perfsummary loop   DIR [--frames A B]
-        frames A..B of the first frame-loop run in the trace (a frame runs
+        frames A..B of the longest frame-loop run in the trace (a frame runs
         from one bilateral launch to the next and holds a tracking call and
         a splat prediction): kernel time per frame, the frame's span, and
         the idle gaps above 4 us with the launches on either side
@@ -117,14 +117,20 @@ def cmd_loop(d: str, first: int, last: int) -> str:
     tr = list(csv.DictReader(open(os.path.join(d, "run_kernel_trace.csv"))))
     tr.sort(key=lambda r: int(r["Start_Timestamp"]))
     starts = [i for i, r in enumerate(tr) if "k_bilateral" in r["Kernel_Name"]]
-    frames = []
+    # the longest run of consecutive single-sequence loop frames (each one
+    # bilateral filter, tracking call and prediction); the batched loops
+    # interleave their sequences' launches and do not form such runs
+    runs, cur = [], []
     for a, b in zip(starts[:-1], starts[1:]):
         names = " ".join(r["Kernel_Name"] for r in tr[a:b])
         if "k_track_end" in names and "k_splat_raster" in names:
-            frames.append((a, b))
-        elif frames:
-            break                       # the first run of loop frames has ended
-    frames = frames[first:last]
+            cur.append((a, b))
+        elif cur:
+            runs.append(cur)
+            cur = []
+    if cur:
+        runs.append(cur)
+    frames = max(runs, key=len)[first:last] if runs else []
     if not frames:
         raise ValueError("no frame-loop frames in the trace")
     agg = defaultdict(lambda: [0, 0.0])
@@ -145,7 +151,7 @@ def cmd_loop(d: str, first: int, last: int) -> str:
                 e[0] += 1
                 e[1] += idle
     n = len(frames)
-    out = [f"Frames {first}..{first + n - 1} of the first loop run: span {span / n:.1f} us per frame, "
+    out = [f"Frames {first}..{first + n - 1} of the longest run of single-sequence loop frames: span {span / n:.1f} us per frame, "
            f"kernels {busy / n:.1f} us per frame.\n",
            "| kernel | launches per frame | us per frame |", "|---|---:|---:|"]
     for k, (c, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
