@@ -217,17 +217,6 @@ __device__ __forceinline__ void store12(float* s, const float* o)
     p[2] = make_float4(o[8], o[9], o[10], o[11]);
 }
 
-// GL_LESS as a 64-bit minimum.  Keys only decrease while a raster runs, so
-// when a coherent (device-scope) read of the key, which returns one of its
-// values in update order, is already at or below v, the minimum would change
-// nothing and the atomic is skipped.  Most fragments lose to one drawn
-// earlier, so most of the atomics, which device scope performs beyond the
-// compute die's cache, are saved; the result is the same.
-__device__ __forceinline__ void key_min(unsigned long long* key, unsigned long long v)
-{
-    if (__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > v) atomicMin(key, v);
-}
-
 // ----------------------------------------------------------------- scan
 
 // Block-exclusive scan of 2048 flags (8 per thread); per-item prefixes and
@@ -451,7 +440,7 @@ __global__ __launch_bounds__(256) void k_index_raster(IndexArgs A, unsigned long
     if (px < 0 || py < 0) return;
     const unsigned q = index_depth(A, hp[2]);
     if (q >= kDepthClear) return;
-    key_min(keys + cm(A.I, px, py), ((unsigned long long)q << 32) | (unsigned)k);
+    atomicMin(keys + cm(A.I, px, py), ((unsigned long long)q << 32) | (unsigned)k);
 }
 
 __global__ __launch_bounds__(256) void k_index_resolve(IndexArgs A, unsigned long long* keys, uint32_t* index,
