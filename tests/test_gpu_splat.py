@@ -49,6 +49,45 @@ def check_prediction(im, cam, pose, model, **kw):
     return exp
 
 
+def test_prediction_bit_exact_on_hard_geometry():
+    """Surfels placed at random in the view: distances 0.15-4 m, normals
+    uniform on the sphere (so some are seen edge-on or from behind), radii
+    log-uniform 1 mm-15 cm (sprites up to the 256-pixel point size), with
+    duplicates.  The raster's candidate bound and its fall-back to the whole
+    sprite square (grazing views, radii above 2 % of the distance, disks that
+    reach close to the camera plane) must keep every passing fragment: the
+    prediction equals the oracle's bit for bit."""
+    cam = CAM_ODD
+    rng = np.random.default_rng(47)
+    n = 6000
+    z = rng.uniform(0.15, 4.0, n)
+    x = (rng.uniform(0, cam.width, n) - cam.cx) / cam.fx * z
+    y = (rng.uniform(0, cam.height, n) - cam.cy) / cam.fy * z
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    # a fifth of the normals nearly perpendicular to the view ray (edge-on disks)
+    ray = np.stack([x, y, z], 1) / np.linalg.norm(np.stack([x, y, z], 1), axis=1, keepdims=True)
+    k = n // 5
+    side = np.cross(ray[:k], nrm[:k])
+    side /= np.linalg.norm(side, axis=1, keepdims=True)
+    nrm[:k] = side + rng.uniform(-0.05, 0.05, (k, 1)) * ray[:k]
+    nrm[:k] /= np.linalg.norm(nrm[:k], axis=1, keepdims=True)
+    model = np.zeros((n, 12), np.float32)
+    model[:, 0], model[:, 1], model[:, 2] = x, y, z
+    model[:, 3] = rng.uniform(7.0, 12.0, n)
+    model[:, 4] = rng.integers(0, 1 << 24, n).astype(np.float32)
+    model[:, 6] = rng.integers(0, 10, n)
+    model[:, 7] = rng.integers(10, 20, n)
+    model[:, 8:11] = nrm
+    model[:, 11] = np.exp(rng.uniform(np.log(0.001), np.log(0.15), n))
+    model = np.concatenate([model, model[:500]])          # exact duplicates (ties)
+    im = splat.IndexMap(*cam_args(cam), max_host_surfels=model.shape[0])
+    pose = np.eye(4, dtype=np.float32)
+    im.combinedPredict(pose, model, CUTOFF, 6.0, 20, 20, 15)
+    exp = check_prediction(im, cam, pose, model)
+    assert (exp[1][..., 2] > 0).mean() > 0.5
+
+
 @pytest.mark.parametrize("cam,seed", [(synth.CAM_320, 41), (synth.CAM_640, 42), (CAM_ODD, 43)],
                          ids=["320x240", "640x480", "202x154"])
 def test_prediction_bit_exact(cam, seed):
