@@ -230,6 +230,33 @@ def test_track_matches_oracle(orc, mode, cam):
         assert abs(sg["lastICPCount"] - so["lastICPCount"]) <= 0.002 * so["lastICPCount"] + 5
 
 
+def test_full_hd_track_matches_oracle(orc):
+    """1920x1080, ICP+RGB+SO3: the largest size here.  Level 0 has more tiles
+    than a pass has blocks (each block loads its next tile behind the current
+    gathers) and the SO3 level has more pixels than the loop's threads
+    preload.  The pose agrees with the FP32 and FP64 oracles within 1 mm /
+    0.1 deg, and the final ICP and photometric inlier counts with the FP32
+    oracle's within 0.2 %."""
+    cam = synth.Camera(1920, 1080, 1050.0, 1050.0, 959.5, 539.5)
+    T0, T1, f0, f1 = make_pair(cam, 12)
+    filt = orc.bilateral(f1.depth, MAXD)
+    res = {}
+    for name, odo in (("gpu", mk(cam)), ("f32", orc_mk(orc, cam)), ("f64", orc_mk(orc, cam, fp64=True))):
+        feed(odo, f0, f1, T0, filt)
+        t, R = odo.getIncrementalTransformation(T0[:3, 3], T0[:3, :3], **MODES["icp_rgb_so3"])
+        res[name] = (pose_of(t, R), odo.stats())
+    Tg = res["gpu"][0]
+    for ref in ("f32", "f64"):
+        dt, dr = synth.pose_error(Tg, res[ref][0])
+        assert dt <= 1e-3 and dr <= 0.1, (ref, dt, dr)
+    dt, dr = synth.pose_error(Tg, T1)
+    assert dt < 5e-3 and dr < 0.3, (dt, dr)
+    sg, so = res["gpu"][1], res["f32"][1]
+    assert abs(sg["lastRGBCount"] - so["lastRGBCount"]) <= 0.002 * so["lastRGBCount"] + 5
+    assert abs(sg["lastICPCount"] - so["lastICPCount"]) <= 0.002 * so["lastICPCount"] + 5
+    assert sg["lastICPCount"] > 1e5
+
+
 def test_sequence_matches_oracle(orc):
     """Five frames of frame-to-model tracking: state carried across calls
     (lastNextImage swap, vmaps_tmp) stays in step with the oracle."""
