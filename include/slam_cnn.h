@@ -60,13 +60,17 @@ int cnn_prepare_input(cnn_net c, const float *depth_a, const uint8_t *image_a, c
                       const uint8_t *image_b, int n, size_t frame_stride, void *x);
 /* Forward of n samples; out (fp32 [n][6]) may be null.  Keeps activations. */
 int cnn_forward(cnn_net c, const void *x, int n, float *out);
-/* Inference of n samples in one launch (x 16-byte aligned): each sample's
- * whole network runs in one workgroup with its activations in LDS (implicit-
- * GEMM convolutions on MFMA with the bias/ReLU/bf16 epilogue fused), with the
- * rounding points of cnn_forward; a sample's output does not depend on the
- * batch.  out: fp32 [n][6].  Keeps no activations (training uses
- * cnn_forward). */
+/* Inference of n samples (x 16-byte aligned): each sample's convolutions run
+ * in one workgroup with the activations in LDS (implicit-GEMM convolutions on
+ * MFMA with the bias/ReLU/bf16 epilogue fused), with the rounding points of
+ * cnn_forward; the FC layers run in a second launch of 16 workgroups per
+ * sample.  A sample's output does not depend on the batch.  out: fp32
+ * [n][6].  Keeps no activations (training uses cnn_forward). */
 int cnn_infer(cnn_net c, const void *x, int n, float *out);
+/* No counterpart in the reference: where cnn_infer computes the FC layers,
+ * 1 (the default) in 16 workgroups per sample after the convolutions' launch,
+ * 0 in the convolutions' own workgroup (one launch); the same bits. */
+int cnn_set_infer_split(cnn_net c, int split);
 /* Loss sum_j w_j (out_j - target_j)^2 per sample (loss_n, optional) and
  * its parameter gradients (mean over the n samples of the last forward). */
 int cnn_backward(cnn_net c, int n, const float *target, const float *w, float *loss_n);

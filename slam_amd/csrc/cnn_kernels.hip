@@ -412,6 +412,11 @@ struct cnn_net_s {
     bf16* pw = nullptr;
     size_t pw_off[kConvs] = {};
     unsigned long long wver = 1, pver = 0;
+    // the split inference head: pooled features, per-workgroup FC2 partials
+    // and the per-sample arrival tickets (back to 0 after every launch)
+    int infer_split = 1;
+    float *infer_gap = nullptr, *infer_part = nullptr;
+    unsigned int* infer_ticket = nullptr;
     uint8_t* mask = nullptr;
     int step = 0;
     // activations of the last forward (for backward), sized for max_batch
@@ -552,6 +557,7 @@ int splits_for(int K, int MN)
 // sample's result does not depend on the other samples of the launch.  One
 // launch for the whole network; no activations are kept.
 constexpr int kInferThreads = 1024;
+constexpr int kHeadGroups = kInferThreads / 64;   // k_cnn_head's groups per sample: one per head wave
 constexpr int kInferBufA = 30 * 40 * 32;     // elements: the largest output (layer 1)
 constexpr int kInferBufB = 60 * 80 * 4;      // the input; also layer 2's output (15 x 20 x 64)
 
@@ -661,7 +667,7 @@ __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __r
 }
 
 __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restrict__ x, InferParams P,
-                                                             float* __restrict__ out)
+                                                             float* __restrict__ out, float* __restrict__ gap_out)
 {
     __shared__ __align__(16) bf16 bufA[kInferBufA];
     __shared__ __align__(16) bf16 bufB[kInferBufB];
@@ -689,7 +695,9 @@ __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restr
     if (t < kFc1In) {
         const float sgap = bf2f(bufB[t]) + bf2f(bufB[kFc1In + t]);
         s_gap[t] = bf2f(f2bf(sgap / 2.0f));
+        if (gap_out) gap_out[(size_t)blockIdx.x * kFc1In + t] = s_gap[t];
     }
+    if (gap_out) return;   // the FC layers run in k_cnn_head
     __syncthreads();
     // fc1 (ReLU, bf16 output) and fc2, fp32 accumulation of bf16 operands.
     // fc1 is a matrix-vector product over 983 KB of weights: each thread
@@ -803,6 +811,9 @@ int cnn_create(int max_batch, cnn_net* out)
         pw_n += (size_t)c->Kp[l] * kSpec[l].cout;
     }
     rc = rc ? rc : dalloc(c, &c->pw, pw_n);
+    rc = rc ? rc : dalloc(c, &c->infer_gap, (size_t)N * kFc1In);
+    rc = rc ? rc : dalloc(c, &c->infer_part, (size_t)N * kHeadGroups * kOutPad);
+    rc = rc ? rc : dalloc(c, &c->infer_ticket, (size_t)N);
     for (int l = 0; l < kConvs && !rc; ++l) {
         const size_t M = (size_t)N * kSpec[l].ho * kSpec[l].wo;
         rc = dalloc(c, &c->xcol[l], M * c->Kp[l]);
@@ -843,7 +854,8 @@ int cnn_create(int max_batch, cnn_net* out)
     if (hipMemcpy(c->mask, mk.data(), off, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->p, 0, off * 4) != hipSuccess || hipMemset(c->g, 0, off * 4) != hipSuccess ||
         hipMemset(c->m, 0, off * 4) != hipSuccess || hipMemset(c->v, 0, off * 4) != hipSuccess ||
-        hipMemset(c->pb, 0, off * 2) != hipSuccess) {
+        hipMemset(c->pb, 0, off * 2) != hipSuccess ||
+        hipMemset(c->infer_ticket, 0, (size_t)N * sizeof(unsigned int)) != hipSuccess) {
         cnn_destroy(c);
         return CNN_ERR_HIP;
     }
@@ -958,6 +970,91 @@ int cnn_forward(cnn_net c, const void* x, int N, float* out)
     return launch_check("forward");
 }
 
+// The FC layers of the split inference: kHeadGroups workgroups per sample
+// (blockIdx.x = sample * kHeadGroups + group).  Group g forms the 240 hidden
+// units [240 g, 240 g + 240) with all 128 of a unit's weights in flight at
+// once, each unit summing its terms in row order, and their FC2 sums exactly
+// as wave g of k_cnn_infer's head does (so the same bits); the group that
+// arrives last at the sample's ticket adds the groups' sums in group order and
+// the bias, and sets the ticket back to 0.
+__global__ __launch_bounds__(256) void k_cnn_head(const float* __restrict__ gap, InferParams P,
+                                                  float* __restrict__ part, unsigned int* __restrict__ ticket,
+                                                  float* __restrict__ out)
+{
+    constexpr int RPW = kFc1Out / kHeadGroups, STEPS = (RPW + 63) / 64;
+    static_assert(kFc1Out % kHeadGroups == 0 && RPW <= 256 && kOutPad == 8 && kOut <= 8, "head blocking");
+    __shared__ float s_gap[kFc1In];
+    __shared__ float s_h1[RPW];
+    __shared__ unsigned int s_last;
+    const int t = threadIdx.x;
+    const int smp = blockIdx.x / kHeadGroups, grp = blockIdx.x - smp * kHeadGroups;
+    const int unit = grp * RPW + t;
+    // the unit's 128 fc1 weights (coalesced over the units) and the group's
+    // fc2 rows, all loads issued before the first wait
+    bf16 w[kFc1In];
+    if (t < RPW) {
+#pragma unroll
+        for (int k = 0; k < kFc1In; ++k) w[k] = P.fc1W[(size_t)k * kFc1Out + unit];
+    }
+    const int lane = t & 63;
+    uint4 w2[STEPS];
+    const uint4* W8 = (const uint4*)P.fc2W;
+    if (t < 64) {
+#pragma unroll
+        for (int q = 0; q < STEPS; ++q) {
+            const int r = lane + 64 * q;
+            w2[q] = W8[grp * RPW + (r < RPW ? r : 0)];
+        }
+    }
+    if (t < kFc1In) s_gap[t] = gap[(size_t)smp * kFc1In + t];
+    __syncthreads();
+    if (t < RPW) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kFc1In; ++k) acc += s_gap[k] * bf2f(w[k]);
+        s_h1[t] = bf2f(f2bf(fmaxf(acc + P.fc1b[unit], 0.0f)));
+    }
+    __syncthreads();
+    if (t < 64) {
+        float acc[kOut];
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) acc[j] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < STEPS; ++q) {
+            const int r = lane + 64 * q;
+            if (r < RPW) {
+                const float h = s_h1[r];
+                const uint32_t wq[4] = {w2[q].x, w2[q].y, w2[q].z, w2[q].w};
+#pragma unroll
+                for (int j = 0; j < kOut; ++j)
+                    acc[j] += h * bf2f((bf16)((j & 1) ? (wq[j >> 1] >> 16) : (wq[j >> 1] & 0xffffu)));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < kOut; ++j) part[((size_t)smp * kHeadGroups + grp) * kOutPad + j] = acc[j];
+            // the partial sums first, then the arrival (release); the last
+            // arrival reads every group's sums (acquire)
+            s_last = __hip_atomic_fetch_add(ticket + smp, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                     (unsigned)(kHeadGroups - 1);
+        }
+    }
+    __syncthreads();
+    if (s_last && t < kOut) {
+        const float* ps = part + (size_t)smp * kHeadGroups * kOutPad;
+        float o = 0.0f;
+        for (int q = 0; q < kHeadGroups; ++q)
+            o += __hip_atomic_load(ps + (size_t)q * kOutPad + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[(size_t)smp * kOut + t] = o + P.fc2b[t];
+        if (t == 0) __hip_atomic_store(ticket + smp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Inference of N samples in one launch (k_cnn_infer): out (device, fp32
 // [N][6]).  Keeps no activations.
 int cnn_infer(cnn_net c, const void* x, int N, float* out)
@@ -988,8 +1085,25 @@ int cnn_infer(cnn_net c, const void* x, int N, float* out)
     P.fc1b = c->p + c->fc1B.off;
     P.fc2W = c->pb + c->fc2W.off;
     P.fc2b = c->p + c->fc2B.off;
-    hipLaunchKernelGGL(k_cnn_infer, dim3(N), dim3(kInferThreads), 0, c->stream, (const bf16*)x, P, out);
+    if (c->infer_split) {
+        hipLaunchKernelGGL(k_cnn_infer, dim3(N), dim3(kInferThreads), 0, c->stream, (const bf16*)x, P, out,
+                           c->infer_gap);
+        hipLaunchKernelGGL(k_cnn_head, dim3(N * kHeadGroups), dim3(256), 0, c->stream, c->infer_gap, P,
+                           c->infer_part, c->infer_ticket, out);
+    } else {
+        hipLaunchKernelGGL(k_cnn_infer, dim3(N), dim3(kInferThreads), 0, c->stream, (const bf16*)x, P, out,
+                           (float*)nullptr);
+    }
     return launch_check("infer");
+}
+
+// The inference head's layout: 1 (default) the FC layers in k_cnn_head's
+// workgroups, 0 in the convolution workgroup; the same bits either way.
+int cnn_set_infer_split(cnn_net c, int split)
+{
+    if (!c || (split != 0 && split != 1)) return CNN_ERR_ARG;
+    c->infer_split = split;
+    return CNN_OK;
 }
 
 // Gradients of the weighted squared error against target (device fp32

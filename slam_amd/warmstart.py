@@ -35,7 +35,7 @@ _V = ctypes.c_void_p
 # Every symbol include/slam_cnn.h declares.
 EXPORTS = ["cnn_create", "cnn_destroy", "cnn_set_stream", "cnn_num_params", "cnn_param_layout", "cnn_set_params",
            "cnn_get_params", "cnn_get_grads", "cnn_prepare_input", "cnn_forward", "cnn_backward", "cnn_adam_step",
-           "cnn_infer"]
+           "cnn_infer", "cnn_set_infer_split"]
 
 
 def _lib():
@@ -54,6 +54,7 @@ def _lib():
         L.cnn_prepare_input.argtypes = [_V, _V, _V, _V, _V, ctypes.c_int, ctypes.c_size_t, _V]
         L.cnn_forward.argtypes = [_V, _V, ctypes.c_int, _V]
         L.cnn_infer.argtypes = [_V, _V, ctypes.c_int, _V]
+        L.cnn_set_infer_split.argtypes = [_V, ctypes.c_int]
         L.cnn_backward.argtypes = [_V, ctypes.c_int, _V, _V, _V]
         L.cnn_adam_step.argtypes = [_V, ctypes.c_float, ctypes.c_float]
         L._cnn_typed = True
@@ -143,8 +144,15 @@ class WarmStartNet:
         _check("cnn_forward", _lib().cnn_forward(self._h, x.data_ptr(), n, out.data_ptr()))
         return out
 
+    def set_infer_split(self, split: bool) -> None:
+        """Where infer computes the FC layers: in 16 workgroups per sample
+        after the convolutions' launch (True, the default) or in the
+        convolutions' workgroup (False); the same bits."""
+        _check("cnn_set_infer_split", _lib().cnn_set_infer_split(self._h, int(bool(split))))
+
     def infer(self, x, out=None):
-        """One-launch inference (cnn_infer): no activations kept."""
+        """Inference (cnn_infer): the convolutions in one workgroup per sample,
+        then the FC layers; no activations kept."""
         import torch
         n = x.shape[0]
         if out is None:

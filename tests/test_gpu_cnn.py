@@ -150,3 +150,24 @@ def test_fused_inference_is_per_sample():
     for i in range(8):
         assert np.array_equal(net.infer(x[i:i + 1].contiguous()).cpu().numpy()[0], full[i]), i
     net.close()
+
+
+def test_split_head_equals_one_launch_head():
+    """The FC layers in 16 workgroups per sample (the default) give the bits
+    of the FC layers in the convolution workgroup, at batch 1 and 8, and on a
+    second call (the arrival tickets are back at 0)."""
+    import torch
+    from slam_amd.warmstart import WarmStartNet
+    net = WarmStartNet(max_batch=8)
+    net.init_params(seed=13)
+    rng = np.random.default_rng(14)
+    x = torch.from_numpy(rng.uniform(0, 1, size=(8, 60, 80, 4)).astype(np.float32)).to(torch.bfloat16).cuda()
+    x = x.contiguous()
+    net.set_infer_split(False)
+    one = net.infer(x).cpu().numpy()
+    one1 = net.infer(x[:1].contiguous()).cpu().numpy()
+    net.set_infer_split(True)
+    for _ in range(2):
+        assert np.array_equal(net.infer(x).cpu().numpy(), one)
+        assert np.array_equal(net.infer(x[:1].contiguous()).cpu().numpy(), one1)
+    net.close()
