@@ -558,8 +558,15 @@ int splits_for(int K, int MN)
 // launch for the whole network; no activations are kept.
 constexpr int kInferThreads = 1024;
 constexpr int kHeadGroups = kInferThreads / 64;   // k_cnn_head's groups per sample: one per head wave
-constexpr int kInferBufA = 30 * 40 * 32;     // elements: the largest output (layer 1)
-constexpr int kInferBufB = 60 * 80 * 4;      // the input; also layer 2's output (15 x 20 x 64)
+// Activations in LDS are [H][W][C + kActPad] (the input: [H][W][4]): a pixel
+// stride of C + 4 elements puts the 16 pixels of a stride-2 fragment load
+// 16 bytes apart modulo the 128-byte bank period (2-way), where C alone
+// (a multiple of 32 from layer 2 on) would put all of them on one bank.
+constexpr int kActPad = 4;
+constexpr int kInferBufA = 30 * 40 * (32 + kActPad);   // elements: the largest output (layer 1)
+constexpr int kInferBufB = 15 * 20 * (64 + kActPad);   // layer 2's output; also holds the input
+constexpr int kInferIn = 60 * 80 * 4;                  // the input sample
+static_assert(kInferIn <= kInferBufB, "the input fits its buffer");
 
 struct InferParams {
     const bf16* W[kConvs];                   // fragment order (k_pack_infer) bf16
@@ -601,10 +608,11 @@ __global__ __launch_bounds__(256) void k_pack_infer(PackArgs A)
 // lane L hold W[16 ks + 4 (L >> 4) + q][16 nt + (L & 15)], q = 0..3.
 constexpr int kGroup = 24;
 
-template <int CIN, int COUT, int HI, int WI, int HO, int WO>
+template <int CIN, int COUT, int HI, int WI, int HO, int WO, int XS, int YS>
 __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __restrict__ Y, const bf16* __restrict__ W,
                                            const float* __restrict__ bias)
 {
+    // XS, YS: the pixel strides (elements) of X and Y
     constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16, CH = 4;
     constexpr int KREAL = 9 * CIN, KP = ((KREAL + BK - 1) / BK) * BK, KS = KP / 16;
     constexpr int GR = KS < kGroup ? KS : kGroup;
@@ -647,7 +655,7 @@ __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __r
                     v4s a = v4s{0, 0, 0, 0};
                     const int hi = hob[i] + kh, wi = wob[i] + kw;
                     if (mok[i] && kq < KREAL && hi >= 0 && hi < HI && wi >= 0 && wi < WI)
-                        a = *(const v4s*)&X[(hi * WI + wi) * CIN + c];
+                        a = *(const v4s*)&X[(hi * WI + wi) * XS + c];
                     acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[g], acc[i], 0, 0, 0);
                 }
             }
@@ -661,7 +669,7 @@ __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __r
                 if ((ch * CH + i) >= MT || m >= M) continue;
                 float v = acc[i][r] + bn;
                 v = fmaxf(v, 0.0f);
-                Y[m * COUT + n] = f2bf(v);
+                Y[m * YS + n] = f2bf(v);
             }
     }
 }
@@ -676,24 +684,25 @@ __global__ __launch_bounds__(kInferThreads) void k_cnn_infer(const bf16* __restr
     __shared__ float s_fc2[kInferThreads / 64][kOut];
     const int t = threadIdx.x;
     // the input, 16 bytes per thread and step
-    const uint4* src = (const uint4*)(x + (size_t)blockIdx.x * kInferBufB);
-    for (int i = t; i < kInferBufB / 8; i += kInferThreads) ((uint4*)bufB)[i] = src[i];
+    const uint4* src = (const uint4*)(x + (size_t)blockIdx.x * kInferIn);
+    for (int i = t; i < kInferIn / 8; i += kInferThreads) ((uint4*)bufB)[i] = src[i];
     __syncthreads();
-    infer_conv<4, 32, 60, 80, 30, 40>(bufB, bufA, P.W[0], P.b[0]);
+    constexpr int P4 = 4 + 0, P32 = 32 + kActPad, P64 = 64 + kActPad, P96 = 96 + kActPad, P128 = 128 + kActPad;
+    infer_conv<4, 32, 60, 80, 30, 40, P4, P32>(bufB, bufA, P.W[0], P.b[0]);
     __syncthreads();
-    infer_conv<32, 64, 30, 40, 15, 20>(bufA, bufB, P.W[1], P.b[1]);
+    infer_conv<32, 64, 30, 40, 15, 20, P32, P64>(bufA, bufB, P.W[1], P.b[1]);
     __syncthreads();
-    infer_conv<64, 96, 15, 20, 8, 10>(bufB, bufA, P.W[2], P.b[2]);
+    infer_conv<64, 96, 15, 20, 8, 10, P64, P96>(bufB, bufA, P.W[2], P.b[2]);
     __syncthreads();
-    infer_conv<96, 128, 8, 10, 4, 5>(bufA, bufB, P.W[3], P.b[3]);
+    infer_conv<96, 128, 8, 10, 4, 5, P96, P128>(bufA, bufB, P.W[3], P.b[3]);
     __syncthreads();
-    infer_conv<128, 128, 4, 5, 2, 3>(bufB, bufA, P.W[4], P.b[4]);
+    infer_conv<128, 128, 4, 5, 2, 3, P128, P128>(bufB, bufA, P.W[4], P.b[4]);
     __syncthreads();
-    infer_conv<128, 128, 2, 3, 1, 2>(bufA, bufB, P.W[5], P.b[5]);
+    infer_conv<128, 128, 2, 3, 1, 2, P128, P128>(bufA, bufB, P.W[5], P.b[5]);
     __syncthreads();
     // global average pooling over the 1 x 2 output (k_gap_fwd's arithmetic)
     if (t < kFc1In) {
-        const float sgap = bf2f(bufB[t]) + bf2f(bufB[kFc1In + t]);
+        const float sgap = bf2f(bufB[t]) + bf2f(bufB[P128 + t]);
         s_gap[t] = bf2f(f2bf(sgap / 2.0f));
         if (gap_out) gap_out[(size_t)blockIdx.x * kFc1In + t] = s_gap[t];
     }
