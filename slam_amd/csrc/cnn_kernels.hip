@@ -608,69 +608,136 @@ __global__ __launch_bounds__(256) void k_pack_infer(PackArgs A)
 // lane L hold W[16 ks + 4 (L >> 4) + q][16 nt + (L & 15)], q = 0..3.
 constexpr int kGroup = 24;
 
+// The k steps [k0, k1) of one work item (column tile nt, row tiles from
+// ch * CH) into acc.
+template <int CIN, int COUT, int HI, int WI, int HO, int WO, int XS, int CH, int GR, int k0, int k1>
+__device__ __forceinline__ void infer_conv_item(const bf16* __restrict__ X, const bf16* __restrict__ W, int nt,
+                                                int ch, v4f (&acc)[CH])
+{
+    constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16;
+    constexpr int KREAL = 9 * CIN;
+    const int lane = threadIdx.x & 63;
+    const int rq = 4 * (lane >> 4);           // this lane's k offset within a 16-deep step
+    // the rows of this lane's A fragments (one per row tile of the chunk)
+    int hob[CH], wob[CH];
+    bool mok[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const int m = (ch * CH + i) * 16 + (lane & 15);
+        mok[i] = (ch * CH + i) < MT && m < M;
+        hob[i] = 2 * (m / WO) - 1;
+        wob[i] = 2 * (m % WO) - 1;
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int g0 = k0; g0 < k1; g0 += GR) {
+        // one 8-byte fragment per lane and step (the wave reads 512
+        // contiguous bytes), all of the group's steps in flight together
+        v4s b[GR];
+#pragma unroll
+        for (int g = 0; g < GR; ++g)
+            if (g0 + g < k1) b[g] = *(const v4s*)(W + ((size_t)((g0 + g) * NT + nt) * 64 + lane) * 4);
+#pragma unroll
+        for (int g = 0; g < GR; ++g) {
+            if (g0 + g >= k1) break;
+            const int kq = (g0 + g) * 16 + rq;
+            const int c = kq % CIN, tap = kq / CIN;
+            const int kh = tap / 3, kw = tap - 3 * kh;
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                v4s a = v4s{0, 0, 0, 0};
+                const int hi = hob[i] + kh, wi = wob[i] + kw;
+                if (mok[i] && kq < KREAL && hi >= 0 && hi < HI && wi >= 0 && wi < WI)
+                    a = *(const v4s*)&X[(hi * WI + wi) * XS + c];
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[g], acc[i], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// Bias, ReLU and the bf16 store of one work item.
+template <int COUT, int HO, int WO, int YS, int CH>
+__device__ __forceinline__ void infer_conv_store(bf16* __restrict__ Y, const float* __restrict__ bias, int nt, int ch,
+                                                 const v4f (&acc)[CH])
+{
+    constexpr int M = HO * WO, MT = (M + 15) / 16;
+    const int lane = threadIdx.x & 63;
+    const int rq = 4 * (lane >> 4);
+    const int n = nt * 16 + (lane & 15);
+    const float bn = bias[n];
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = (ch * CH + i) * 16 + rq + r;
+            if ((ch * CH + i) >= MT || m >= M) continue;
+            float v = acc[i][r] + bn;
+            v = fmaxf(v, 0.0f);
+            Y[m * YS + n] = f2bf(v);
+        }
+}
+
+// XS, YS: the pixel strides (elements) of X and Y.  CH: row tiles per work
+// item (fewer when the layer has fewer, so no MFMA runs on an all-padding
+// tile).  Layers with at most half as many work items as waves (the deep,
+// small ones) split each item's k steps in two halves on two waves; the
+// second half's sums go through LDS past the layer's output in Y (at
+// kKPartOff elements; Y is larger than the small outputs that split) and are
+// added to the first half's, in that order.
+constexpr int kKPartOff = 4096;
+
 template <int CIN, int COUT, int HI, int WI, int HO, int WO, int XS, int YS>
 __device__ __forceinline__ void infer_conv(const bf16* __restrict__ X, bf16* __restrict__ Y, const bf16* __restrict__ W,
                                            const float* __restrict__ bias)
 {
-    // XS, YS: the pixel strides (elements) of X and Y
-    constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16, CH = 4;
+    constexpr int M = HO * WO, MT = (M + 15) / 16, NT = COUT / 16, CH = MT < 4 ? MT : 4;
     constexpr int KREAL = 9 * CIN, KP = ((KREAL + BK - 1) / BK) * BK, KS = KP / 16;
-    constexpr int GR = KS < kGroup ? KS : kGroup;
     static_assert(COUT % 16 == 0 && CIN % 4 == 0 && KP % 16 == 0, "tile shapes");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6;
     constexpr int WAVES = kInferThreads / 64;
-    constexpr int CHUNKS = (MT + CH - 1) / CH;
-    const int rq = 4 * (lane >> 4);           // this lane's k offset within a 16-deep step
-    for (int item = wave; item < NT * CHUNKS; item += WAVES) {
+    constexpr int CHUNKS = (MT + CH - 1) / CH, ITEMS = NT * CHUNKS;
+    constexpr bool SPLIT = 2 * ITEMS <= WAVES && KS >= 2 * kGroup;
+    if constexpr (!SPLIT) {
+        constexpr int GR = KS < kGroup ? KS : kGroup;
+        for (int item = wave; item < ITEMS; item += WAVES) {
+            const int nt = item % NT, ch = item / NT;
+            v4f acc[CH];
+            infer_conv_item<CIN, COUT, HI, WI, HO, WO, XS, CH, GR, 0, KS>(X, W, nt, ch, acc);
+            infer_conv_store<COUT, HO, WO, YS, CH>(Y, bias, nt, ch, acc);
+        }
+    } else {
+        constexpr int KH = (KS + 1) / 2;
+        constexpr int GR = KH < kGroup ? KH : kGroup;
+        static_assert(M * YS <= kKPartOff && kKPartOff * 2 + ITEMS * CH * 256 * 4 <= kInferBufB * 2 &&
+                          kKPartOff * 2 + ITEMS * CH * 256 * 4 <= kInferBufA * 2,
+                      "the second-half sums fit past the output");
+        const int lane = threadIdx.x & 63;
+        const bool live = wave < 2 * ITEMS;
+        const int item = wave % ITEMS, half = wave / ITEMS;
         const int nt = item % NT, ch = item / NT;
-        const int n = nt * 16 + (lane & 15);
-        // the rows of this lane's A fragments (one per row tile of the chunk)
-        int hob[CH], wob[CH];
-        bool mok[CH];
-#pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            const int m = (ch * CH + i) * 16 + (lane & 15);
-            mok[i] = (ch * CH + i) < MT && m < M;
-            hob[i] = 2 * (m / WO) - 1;
-            wob[i] = 2 * (m % WO) - 1;
-        }
         v4f acc[CH];
+        if (!live) {
 #pragma unroll
-        for (int i = 0; i < CH; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
-        for (int g0 = 0; g0 < KS; g0 += GR) {
-            // one 8-byte fragment per lane and step (the wave reads 512
-            // contiguous bytes), all of the group's steps in flight together
-            v4s b[GR];
-#pragma unroll
-            for (int g = 0; g < GR; ++g)
-                if (g0 + g < KS) b[g] = *(const v4s*)(W + ((size_t)((g0 + g) * NT + nt) * 64 + lane) * 4);
-#pragma unroll
-            for (int g = 0; g < GR; ++g) {
-                if (g0 + g >= KS) break;
-                const int kq = (g0 + g) * 16 + rq;
-                const int c = kq % CIN, tap = kq / CIN;
-                const int kh = tap / 3, kw = tap - 3 * kh;
-#pragma unroll
-                for (int i = 0; i < CH; ++i) {
-                    v4s a = v4s{0, 0, 0, 0};
-                    const int hi = hob[i] + kh, wi = wob[i] + kw;
-                    if (mok[i] && kq < KREAL && hi >= 0 && hi < HI && wi >= 0 && wi < WI)
-                        a = *(const v4s*)&X[(hi * WI + wi) * XS + c];
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[g], acc[i], 0, 0, 0);
-                }
-            }
+            for (int i = 0; i < CH; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+        } else if (half == 0) {
+            infer_conv_item<CIN, COUT, HI, WI, HO, WO, XS, CH, GR, 0, KH>(X, W, nt, ch, acc);
+        } else {
+            infer_conv_item<CIN, COUT, HI, WI, HO, WO, XS, CH, GR, KH, KS>(X, W, nt, ch, acc);
         }
-        const float bn = bias[n];
+        float* kp = (float*)(Y + kKPartOff) + ((size_t)item * CH * 64 + lane) * 4;
+        if (live && half == 1)
 #pragma unroll
-        for (int i = 0; i < CH; ++i)
+            for (int i = 0; i < CH; ++i)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = (ch * CH + i) * 16 + rq + r;
-                if ((ch * CH + i) >= MT || m >= M) continue;
-                float v = acc[i][r] + bn;
-                v = fmaxf(v, 0.0f);
-                Y[m * YS + n] = f2bf(v);
-            }
+                for (int r = 0; r < 4; ++r) kp[i * 256 + r] = acc[i][r];
+        __syncthreads();
+        if (live && half == 0) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[i][r] = acc[i][r] + kp[i * 256 + r];
+            infer_conv_store<COUT, HO, WO, YS, CH>(Y, bias, nt, ch, acc);
+        }
     }
 }
 
