@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "host_wait.h"
+#include "timing_events.h"
 #include "odom_internal.h"
 #include "odom_math.h"
 #include "slam_odom.h"
@@ -1110,11 +1111,9 @@ int odom_profile_track(odom_handle c, int rgbOnly, float icpWeight, int pyramid,
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
     const int nev = (int)sched.size() * (P.rgb ? 4 : 2);
-    std::vector<hipEvent_t> ev(nev);
-    hipEvent_t t0, t1;
-    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
-    HIPCHK(c, hipEventCreate(&t0));
-    HIPCHK(c, hipEventCreate(&t1));
+    TimingEvents ev(nev + 2);
+    HIPCHK(c, ev.create());
+    hipEvent_t t0 = ev[nev], t1 = ev[nev + 1];
     HIPCHK(c, hipEventRecord(t0, c->stream));
     int rc = enqueue_track(c, P, ev.data());
     HIPCHK(c, hipEventRecord(t1, c->stream));
@@ -1142,9 +1141,6 @@ int odom_profile_track(odom_handle c, int rgbOnly, float icpWeight, int pyramid,
         if (n_a[l]) us_a[l] /= (float)n_a[l];
         if (n_b[l]) us_b[l] /= (float)n_b[l];
     }
-    for (auto& x : ev) (void)hipEventDestroy(x);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     return rc;
 }
 
@@ -1163,9 +1159,9 @@ int odom_profile_steps(odom_handle c, int rgbOnly, float icpWeight, int reps, fl
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l * 64 + j);
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    TimingEvents ev(2);
+    HIPCHK(c, ev.create());
+    hipEvent_t e0 = ev[0], e1 = ev[1];
     int rc = ODOM_OK;
     for (int level = 0; level < kLevels && rc == ODOM_OK; ++level) {
         int k = -1;
@@ -1208,8 +1204,6 @@ int odom_profile_steps(odom_handle c, int rgbOnly, float icpWeight, int reps, fl
         }
         if (rc == ODOM_OK) rc = check_launch(c);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -1905,9 +1899,9 @@ int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
         if (int rc = write_field(c, &c->st->it[0].skip, &zero, 1)) return rc;
         BCHK(b, hipStreamSynchronize(c->stream));
     }
-    hipEvent_t e0, e1;
-    BCHK(b, hipEventCreate(&e0));
-    BCHK(b, hipEventCreate(&e1));
+    TimingEvents ev(2);
+    BCHK(b, ev.create());
+    hipEvent_t e0 = ev[0], e1 = ev[1];
     const int threads[] = {256, 512};
     const int ppts[] = {1, 2, 3, 4};
     int rc = ODOM_OK;
@@ -1948,8 +1942,6 @@ int odom_batch_autotune(odom_batch b, int rgbOnly, float icpWeight)
             b->shape[kernel][level] = best_sh;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc) return rc;
     return batch_plan(b);
 }
@@ -1983,8 +1975,8 @@ int odom_batch_profile_track(odom_batch b, int rgbOnly, float icpWeight, int pyr
     std::vector<int> sched;
     for (int l = kLevels - 1; l >= 0; --l)
         for (int j = 0; j < P.iters[l]; ++j) sched.push_back(l);
-    std::vector<hipEvent_t> ev(sched.size() * (P.rgb ? 4 : 2) + 2, nullptr);
-    for (auto& e : ev) BCHK(b, hipEventCreate(&e));
+    TimingEvents ev(sched.size() * (P.rgb ? 4 : 2) + 2);
+    BCHK(b, ev.create());
     BCHK(b, hipEventRecord(ev[ev.size() - 2], b->stream));
     batch_launches(b, cs, dp, ev.data());
     BCHK(b, hipEventRecord(ev[ev.size() - 1], b->stream));
@@ -2012,7 +2004,6 @@ int odom_batch_profile_track(odom_batch b, int rgbOnly, float icpWeight, int pyr
     float ms = 0;
     BCHK(b, hipEventElapsedTime(&ms, ev[ev.size() - 2], ev[ev.size() - 1]));
     *us_total = ms * 1000.0f;
-    for (auto& x : ev) (void)hipEventDestroy(x);
     return batch_finish(b, P);
 }
 

@@ -39,6 +39,7 @@
 #include <string.h>
 
 #include "slam_splat.h"
+#include "timing_events.h"
 
 namespace {
 
@@ -730,8 +731,8 @@ int splat_profile(splat_handle c, const float pose[16], const float* model_dev, 
         if (rc) return rc;
     }
     const int npix = c->cam.w * c->cam.h;
-    hipEvent_t ev[4];
-    for (auto& e : ev) SCHK(hipEventCreate(&e));
+    TimingEvents ev(4);
+    SCHK(ev.create());
     double acc[3] = {0, 0, 0};
     for (int r = 0; r < reps; ++r) {
         const bool last = r == reps - 1;
@@ -761,7 +762,6 @@ int splat_profile(splat_handle c, const float pose[16], const float* model_dev, 
         SCHK(hipMemcpy(&f, c->frag_count, 8, hipMemcpyDeviceToHost));
         *fragments = (long long)f;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     return ODOM_OK;
 }
 

@@ -32,6 +32,7 @@
 #include "host_wait.h"
 #include "odom_math.h"
 #include "slam_surfel.h"
+#include "timing_events.h"
 
 using slam::bexpf;
 
@@ -1238,12 +1239,13 @@ int surfel_profile(surfel_handle c, const float pose[16], int time, const odom_f
     if (rc) return rc;
     // keep the starting model: every rep fuses the same frame into it
     const int n0 = c->count, cur0 = c->cur;
-    float* start = nullptr;  // a copy of the starting model (the one allocation outside create)
-    SCHK(hipMalloc((void**)&start, (size_t)(n0 > 0 ? n0 : 1) * 48));
+    DeviceScratch start_buf;  // a copy of the starting model (the one allocation outside create)
+    SCHK(hipMalloc(&start_buf.p, (size_t)(n0 > 0 ? n0 : 1) * 48));
+    float* start = (float*)start_buf.p;
     if (n0 > 0)
         SCHK(hipMemcpyAsync(start, c->model[cur0], (size_t)n0 * 48, hipMemcpyDeviceToDevice, c->stream));
-    hipEvent_t ev[5];
-    for (auto& e : ev) SCHK(hipEventCreate(&e));
+    TimingEvents ev(5);
+    SCHK(ev.create());
     double acc[4] = {0, 0, 0, 0};
     int n = 0;
     for (int r = 0; r < reps; ++r) {
@@ -1275,8 +1277,6 @@ int surfel_profile(surfel_handle c, const float pose[16], int time, const odom_f
     SCHK(hipMemsetAsync(c->totals + 1, 0, 4, c->stream));
     SCHK(hipStreamSynchronize(c->stream));
     for (int k = 0; k < 4; ++k) us[k] = (float)(acc[k] * 1000.0 / reps);
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipFree(start);
     return ODOM_OK;
 }
 
