@@ -412,7 +412,7 @@ FORM_DOC = {
     "reference_conv": "previous-pose start, SO3, 10/5/4 with the convergence test",
     "plain_policy": "constant-velocity start, no SO3, 10/5/4 with the convergence test (the learned form "
                     "without the network)",
-    "learned": "network start (bf16 MFMA, batch 1, one launch) judged against the constant-velocity start "
+    "learned": "network start (bf16 MFMA, batch 1, two launches) judged against the constant-velocity start "
                "after the level-2 refinement of both, convergence test",
     "learned_full": "as learned, both starts refined through every level before the judge",
 }

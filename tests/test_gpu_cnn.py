@@ -122,7 +122,8 @@ def test_forward_is_the_same_in_any_inference_batch():
 
 
 def test_fused_inference_matches_forward_and_oracle(setup):
-    """cnn_infer (one launch, activations in LDS, implicit-GEMM convolutions)
+    """cnn_infer (the convolutions in one workgroup per sample with the
+    activations in LDS, implicit-GEMM; the FC layers in a second launch)
     computes the network of cnn_forward with the same rounding points: the
     two agree to far below the bf16 tolerance, and the fused result meets the
     forward's bound against the rounded oracle."""
