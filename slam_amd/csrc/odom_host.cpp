@@ -252,15 +252,6 @@ int resolve(odom_ctx* c, const odom_frame_view* v, int fmt_a, int fmt_b, void* s
 
 int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
 {
-    PyrJob py{};
-    py.count = 1;
-    py.p[0].kind = 2;
-    py.p[0].src0 = c->depth[0];
-    py.p[0].dst[1] = c->depth[1];
-    py.p[0].dst[2] = c->depth[2];
-    py.w0 = c->w[0];
-    py.h0 = c->h[0];
-    launch_pyramid(py, c->stream);
     VNLevels a{};
     for (int l = 0; l < kLevels; ++l) {
         a.depth[l] = c->depth[l];
@@ -274,7 +265,7 @@ int enqueue_depth_pyramid_maps(odom_ctx* c, float cutoff)
         a.cy[l] = c->cy[l];
     }
     a.cutoff = cutoff;
-    launch_vmap_nmap(a, c->stream);
+    launch_depth_pyramid_maps(a, c->stream);
     return check_launch(c);
 }
 

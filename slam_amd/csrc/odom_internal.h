@@ -37,7 +37,9 @@ struct VNLevels {
     float fx_inv[kLevels], fy_inv[kLevels], cx[kLevels], cy[kLevels];
     float cutoff;
 };
-void launch_vmap_nmap(const VNLevels& a, hipStream_t s);
+// initICP's depth levels 1 and 2 from a.depth[0] and the three levels'
+// vertex and normal maps, in one launch.
+void launch_depth_pyramid_maps(const VNLevels& a, hipStream_t s);
 // RGBA32F vertex/normal maps -> planar pyramids (copyMaps + resize, optional
 // rigid transform), plus the vertex z plane kept for verticesToDepth.
 struct ModelLevels {

@@ -203,10 +203,10 @@ static Seg3 make_seg(const int* n, int per_block)
 // ========================================================= a3 + a4 vmap/nmap
 // utils.cu:109-133 and 151-188, fused: the normal's two neighbours are
 // recomputed from depth with the same expression as the vertex map.
-__device__ __forceinline__ bool vpoint(const uint16_t* __restrict__ d, int w, int u, int v, float fxi,
-                                       float fyi, float cx, float cy, float cutoff, float* o)
+__device__ __forceinline__ bool vpoint(uint16_t dv, int u, int v, float fxi, float fyi, float cx, float cy,
+                                       float cutoff, float* o)
 {
-    const float z = (float)d[(size_t)v * w + u] / 1000.f;
+    const float z = (float)dv / 1000.f;
     if (z != 0 && z < cutoff) {
         o[0] = z * ((float)u - cx) * fxi;
         o[1] = z * ((float)v - cy) * fyi;
@@ -216,20 +216,18 @@ __device__ __forceinline__ bool vpoint(const uint16_t* __restrict__ d, int w, in
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_vmap_nmap(VNLevels a, Seg3 g)
+// Both maps at pixel (u, v) of level l; depth(x, y) gives the level's texels
+// (the right and lower neighbours are read only where the normal needs them).
+template <class Depth>
+__device__ __forceinline__ void vn_pixel(const VNLevels& a, int l, int u, int v, Depth depth)
 {
-    const int l = seg_level(g, blockIdx.x);
-    const int i = (blockIdx.x - g.b0[l]) * 256 + threadIdx.x;
     const int w = a.w[l], h = a.h[l];
-    if (i >= w * h) return;
-    const int v = i / w, u = i - v * w;
-    const size_t P = (size_t)w * h;
-    const uint16_t* d = a.depth[l];
+    const size_t i = (size_t)v * w + u, P = (size_t)w * h;
     float* vm = a.vmap[l];
     float* nm = a.nmap[l];
     const float fxi = a.fx_inv[l], fyi = a.fy_inv[l], cx = a.cx[l], cy = a.cy[l];
     float p00[3];
-    const bool ok00 = vpoint(d, w, u, v, fxi, fyi, cx, cy, a.cutoff, p00);
+    const bool ok00 = vpoint(depth(u, v), u, v, fxi, fyi, cx, cy, a.cutoff, p00);
     if (ok00) {
         vm[i] = p00[0]; vm[i + P] = p00[1]; vm[i + 2 * P] = p00[2];
     } else {
@@ -238,8 +236,8 @@ __global__ __launch_bounds__(256) void k_vmap_nmap(VNLevels a, Seg3 g)
     float n[3] = {QNAN, QNAN, QNAN};
     if (u != w - 1 && v != h - 1 && ok00) {
         float p01[3], p10[3];
-        const bool ok01 = vpoint(d, w, u + 1, v, fxi, fyi, cx, cy, a.cutoff, p01);
-        const bool ok10 = vpoint(d, w, u, v + 1, fxi, fyi, cx, cy, a.cutoff, p10);
+        const bool ok01 = vpoint(depth(u + 1, v), u + 1, v, fxi, fyi, cx, cy, a.cutoff, p01);
+        const bool ok10 = vpoint(depth(u, v + 1), u, v + 1, fxi, fyi, cx, cy, a.cutoff, p10);
         if (ok01 && ok10) {
             const float e0[3] = {p01[0] - p00[0], p01[1] - p00[1], p01[2] - p00[2]};
             const float e1[3] = {p10[0] - p00[0], p10[1] - p00[1], p10[2] - p00[2]};
@@ -250,14 +248,6 @@ __global__ __launch_bounds__(256) void k_vmap_nmap(VNLevels a, Seg3 g)
         }
     }
     nm[i] = n[0]; nm[i + P] = n[1]; nm[i + 2 * P] = n[2];
-}
-
-void launch_vmap_nmap(const VNLevels& a, hipStream_t s)
-{
-    int n[kLevels];
-    for (int l = 0; l < kLevels; ++l) n[l] = a.w[l] * a.h[l];
-    const Seg3 g = make_seg(n, 256);
-    hipLaunchKernelGGL(k_vmap_nmap, dim3(g.b0[kLevels]), dim3(256), 0, s, a, g);
 }
 
 // ================================================ a5 + a6 + a7 model maps
@@ -679,6 +669,112 @@ void launch_pyramid(const PyrJob& j, hipStream_t s)
         }
     g.b0[g.nseg] = nb;
     hipLaunchKernelGGL(k_pyramid, dim3(nb), dim3(256), 0, s, g);
+}
+
+// initICP's depth pyramid (a2) and the vertex and normal maps of its three
+// levels (a3, a4) in one launch.  Level-2 tiles come first, then level-1
+// tiles, then the level-0 maps one pixel per thread from the input.  A tile
+// computes its level's depth one output beyond its right and lower edges
+// too, in LDS only (the neighbouring tile stores those), so the normals of
+// its edge pixels, which read their right and lower neighbours, come from
+// the tile itself.  The depths are those of k_pyramid (the same windows and
+// sums) and every level's maps come from vn_pixel, the same expressions
+// whichever block computes them.
+constexpr int kDM1 = 16;                                      // level-1 tile edge
+constexpr int kDM1R = kDM1 + 1;                               // with the one-output margin
+constexpr int kDM1T = 2 * kDM1R + 3;                          // its level-0 texels (37)
+constexpr int kDM2X = 8, kDM2Y = 4;                           // level-2 tile
+constexpr int kDM2RX = kDM2X + 1, kDM2RY = kDM2Y + 1;         // with the margin
+constexpr int kDM21X = 2 * kDM2RX + 3, kDM21Y = 2 * kDM2RY + 3;   // its level-1 values (21 x 13)
+constexpr int kDM20X = 2 * kDM21X + 3, kDM20Y = 2 * kDM21Y + 3;   // their level-0 texels (45 x 29)
+static_assert(kDM1R * kDM1R <= 2 * 256 && kDM21X * kDM21Y <= 2 * 256 && kDM2RX * kDM2RY <= 256, "passes");
+
+struct DepthMapsGrid {
+    VNLevels a;                   // a.depth[0] is the input; a.depth[1], a.depth[2] are written
+    int n2, n1;                   // level-2 and level-1 tiles
+    int tx2, tx1;                 // tiles per row
+};
+
+__global__ __launch_bounds__(256) void k_depth_pyramid_maps(DepthMapsGrid g)
+{
+    constexpr int kS0 = kDM20X * kDM20Y > kDM1T * kDM1T ? kDM20X * kDM20Y : kDM1T * kDM1T;
+    constexpr int kS1 = kDM21X * kDM21Y > kDM1R * kDM1R ? kDM21X * kDM21Y : kDM1R * kDM1R;
+    __shared__ uint16_t s0[kS0];
+    __shared__ uint16_t s1[kS1];
+    __shared__ uint16_t s2[kDM2RX * kDM2RY];
+    const VNLevels& a = g.a;
+    const int tid = threadIdx.x;
+    const int w0 = a.w[0], h0 = a.h[0], w1 = a.w[1], h1 = a.h[1], w2 = a.w[2], h2 = a.h[2];
+    const uint16_t* d0 = a.depth[0];
+    auto load0 = [&](int x, int y) { return d0[(size_t)y * w0 + x]; };
+    int b = blockIdx.x;
+    if (b < g.n2) {
+        const int X0 = (b % g.tx2) * kDM2X, Y0 = (b / g.tx2) * kDM2Y;
+        const int o1x = 2 * X0 - 2, o1y = 2 * Y0 - 2;
+        const int o0x = 2 * o1x - 2, o0y = 2 * o1y - 2;
+        stage_tile<kDM20X * kDM20Y, 256, false>(s0, kDM20X, o0x, o0y, w0, h0, load0);
+        __syncthreads();
+        for (int e = tid; e < kDM21X * kDM21Y; e += 256) {
+            const int x1 = o1x + e % kDM21X, y1 = o1y + e / kDM21X;
+            if (x1 >= 0 && x1 < w1 && y1 >= 0 && y1 < h1)
+                s1[e] = pyr_at<2>((const uint16_t*)s0, kDM20X, o0x, o0y, w0, h0, x1, y1);
+        }
+        __syncthreads();
+        if (tid < kDM2RX * kDM2RY) {
+            const int cx = tid % kDM2RX, cy = tid / kDM2RX;
+            const int x = X0 + cx, y = Y0 + cy;
+            if (x < w2 && y < h2) {
+                const uint16_t v = pyr_at<2>((const uint16_t*)s1, kDM21X, o1x, o1y, w1, h1, x, y);
+                s2[tid] = v;
+                if (cx < kDM2X && cy < kDM2Y) ((uint16_t*)a.depth[2])[(size_t)y * w2 + x] = v;
+            }
+        }
+        __syncthreads();
+        if (tid < kDM2X * kDM2Y) {
+            const int x = X0 + tid % kDM2X, y = Y0 + tid / kDM2X;
+            if (x < w2 && y < h2)
+                vn_pixel(a, 2, x, y, [&](int u, int v) { return s2[(v - Y0) * kDM2RX + (u - X0)]; });
+        }
+        return;
+    }
+    b -= g.n2;
+    if (b < g.n1) {
+        const int X0 = (b % g.tx1) * kDM1, Y0 = (b / g.tx1) * kDM1;
+        const int ox = 2 * X0 - 2, oy = 2 * Y0 - 2;
+        stage_tile<kDM1T * kDM1T, 256, false>(s0, kDM1T, ox, oy, w0, h0, load0);
+        __syncthreads();
+        for (int e = tid; e < kDM1R * kDM1R; e += 256) {
+            const int cx = e % kDM1R, cy = e / kDM1R;
+            const int x = X0 + cx, y = Y0 + cy;
+            if (x < w1 && y < h1) {
+                const uint16_t v = pyr_at<2>((const uint16_t*)s0, kDM1T, ox, oy, w0, h0, x, y);
+                s1[e] = v;
+                if (cx < kDM1 && cy < kDM1) ((uint16_t*)a.depth[1])[(size_t)y * w1 + x] = v;
+            }
+        }
+        __syncthreads();
+        const int x = X0 + (tid & (kDM1 - 1)), y = Y0 + tid / kDM1;
+        if (x < w1 && y < h1) vn_pixel(a, 1, x, y, [&](int u, int v) { return s1[(v - Y0) * kDM1R + (u - X0)]; });
+        return;
+    }
+    b -= g.n1;
+    const int i = b * 256 + tid;
+    if (i < w0 * h0) {
+        const int y = i / w0, x = i - y * w0;
+        vn_pixel(a, 0, x, y, load0);
+    }
+}
+
+void launch_depth_pyramid_maps(const VNLevels& a, hipStream_t s)
+{
+    DepthMapsGrid g;
+    g.a = a;
+    g.tx2 = cdiv(a.w[2], kDM2X);
+    g.tx1 = cdiv(a.w[1], kDM1);
+    g.n2 = g.tx2 * cdiv(a.h[2], kDM2Y);
+    g.n1 = g.tx1 * cdiv(a.h[1], kDM1);
+    const int n0 = cdiv(a.w[0] * a.h[0], 256);
+    hipLaunchKernelGGL(k_depth_pyramid_maps, dim3(g.n2 + g.n1 + n0), dim3(256), 0, s, g);
 }
 
 // ================================ a12 derivatives + photometric source mask
