@@ -30,7 +30,7 @@ void launch_bilateral(const uint16_t* src, int src_pitch, uint16_t* dst, int w, 
                       hipStream_t s);
 // depth pyramid -> vertex+normal maps of all levels in one launch
 struct VNLevels {
-    const uint16_t* depth[kLevels];
+    uint16_t* depth[kLevels];       // level 0 is read; launch_depth_pyramid_maps writes levels 1, 2
     float* vmap[kLevels];
     float* nmap[kLevels];
     int w[kLevels], h[kLevels];

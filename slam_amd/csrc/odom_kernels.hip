@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256) void k_depth_pyramid_maps(DepthMapsGrid g)
             if (x < w2 && y < h2) {
                 const uint16_t v = pyr_at<2>((const uint16_t*)s1, kDM21X, o1x, o1y, w1, h1, x, y);
                 s2[tid] = v;
-                if (cx < kDM2X && cy < kDM2Y) ((uint16_t*)a.depth[2])[(size_t)y * w2 + x] = v;
+                if (cx < kDM2X && cy < kDM2Y) a.depth[2][(size_t)y * w2 + x] = v;
             }
         }
         __syncthreads();
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(256) void k_depth_pyramid_maps(DepthMapsGrid g)
             if (x < w1 && y < h1) {
                 const uint16_t v = pyr_at<2>((const uint16_t*)s0, kDM1T, ox, oy, w0, h0, x, y);
                 s1[e] = v;
-                if (cx < kDM1 && cy < kDM1) ((uint16_t*)a.depth[1])[(size_t)y * w1 + x] = v;
+                if (cx < kDM1 && cy < kDM1) a.depth[1][(size_t)y * w1 + x] = v;
             }
         }
         __syncthreads();
